@@ -1,0 +1,198 @@
+/*
+ * przm_gpu.h -- C ABI of the MI355X (gfx950) PRZM encode path.
+ *
+ * Plain pointers and sizes only.  Every function returns an int32 status:
+ * 0 on success, a negative PRZM_GPU_ERR_* code otherwise; none aborts.
+ * A bound caller (the Go `gpu` package over cgo, Python over ctypes) maps
+ * each code to an error and keeps its own CPU path as the silent fallback.
+ *
+ * Output format ("mode G"): each 64 KiB block of a buffer becomes one
+ * self-contained PRZM frame (the frame header's uncompressed size is 24-bit
+ * and the match window is the block), frames are written back to back in
+ * block order, and the frame sizes are returned beside them.  An empty
+ * buffer yields one 8-byte uncompressed frame.  Frames decode with the
+ * reference decoders (src/przm_dec.c przm_decompress, decompress.go
+ * DecompressTo).
+ */
+#ifndef PRZM_GPU_H
+#define PRZM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRZM_GPU_OK 0
+#define PRZM_GPU_ERR_UNAVAILABLE -1   /* no HIP device / mock unavailable */
+#define PRZM_GPU_ERR_ARG -2           /* bad argument */
+#define PRZM_GPU_ERR_CAPACITY -3      /* caller buffer too small */
+#define PRZM_GPU_ERR_HIP -4           /* HIP runtime error */
+#define PRZM_GPU_ERR_MOCK -5          /* injected test failure */
+#define PRZM_GPU_ERR_OOM -6           /* device allocation failed */
+#define PRZM_GPU_ERR_BUSY -7          /* serving queue full */
+
+typedef struct przm_gpu przm_gpu;
+typedef struct przm_server przm_server;
+
+/* GPUCompressor device fields (gpu/bridge.go:283-287).  On AMD:
+ * compute_major/minor = gfx major / minor (gfx950 -> 9, 5), sm_count = CUs. */
+typedef struct {
+    int32_t device;
+    int32_t compute_major;
+    int32_t compute_minor;
+    int32_t sm_count;
+    uint64_t total_vram_bytes;
+    char device_name[256];
+    char arch[64];
+} przm_gpu_info;
+
+/* GPUProfile (gpu/types.go:55-72) plus per-stage device times. */
+typedef struct {
+    uint64_t ingress_dma_bytes;
+    double ingress_dma_ms;
+    double ingress_dma_gbps;
+    double kernel_ms;
+    double internal_vram_gbps;
+    uint64_t egress_dma_bytes;
+    double egress_dma_ms;
+    double egress_dma_gbps;
+    double total_round_trip_ms;
+    double end_to_end_gbps;
+    double pcie_overhead_percent;
+    int32_t pipelined_chunks;
+    int32_t graph_launch;
+    double stage_ms[8]; /* check, lz, hist, rans, frame-size, scan, frame-write, spare */
+} przm_gpu_profile;
+
+/* A buffer inside a device or host arena. */
+typedef struct {
+    uint64_t offset;
+    uint64_t len;
+} przm_span;
+
+/* ---- availability and lifecycle (gpu/bridge.go:45-123, 291-449, 513) ---- */
+
+/* IsAvailable (bridge.go:61): 0 and *info filled when a gfx9 device is usable. */
+int32_t przm_gpu_probe(przm_gpu_info *info);
+/* NewGPUCompressor (bridge.go:291). */
+int32_t przm_gpu_create(int32_t device, przm_gpu **out);
+/* Close (bridge.go:513). */
+int32_t przm_gpu_destroy(przm_gpu *g);
+int32_t przm_gpu_get_info(przm_gpu *g, przm_gpu_info *info);
+/* SetMockUnavailableForTest / SetMockMatchErrorForTest (bridge.go:45-56):
+ * a nonzero code makes every encode entry point fail with that code before
+ * any device work. */
+void przm_gpu_set_mock_unavailable(int32_t on);
+void przm_gpu_set_mock_error(int32_t code);
+
+/* ---- stages (gpu/bridge.go MatchLZ77 1480, Tokenize 1895,
+ *      ComputeHistogramHost 2961, VANSEncode4xHost 3233,
+ *      PackStreamFlatHost 3079) ---- */
+
+/* Per-position best match of every 64 KiB block of src: desc[p] =
+ * (len << 16) | dist (block-relative), 0 where there is none.  n entries. */
+int32_t przm_gpu_match(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t level, uint32_t *desc,
+                       przm_gpu_profile *prof);
+/* Token streams of every block, concatenated in block order; block_lens
+ * gets (cmd, off, lit, op, meta, var, tokens) per block. */
+int32_t przm_gpu_tokenize(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t level, uint8_t *cmd, uint64_t cmd_cap,
+                          uint8_t *off, uint64_t off_cap, uint8_t *lit, uint64_t lit_cap, uint8_t *op,
+                          uint64_t op_cap, uint8_t *meta, uint64_t meta_cap, uint8_t *var, uint64_t var_cap,
+                          uint32_t *block_lens, uint64_t nblocks_cap, przm_gpu_profile *prof);
+/* 256-bin byte histogram of src. */
+int32_t przm_gpu_histogram(przm_gpu *g, const uint8_t *src, uint64_t n, uint32_t counts[256],
+                           przm_gpu_profile *prof);
+/* Adaptive 4-lane V-ANS container of src ('V','A',1,0, u32 n, matrix,
+ * 4 states, stream), core/literals/literals.go:368-384.  n <= 16 MiB. */
+int32_t przm_gpu_vans_encode(przm_gpu *g, const uint8_t *src, uint64_t n, uint8_t *out, uint64_t cap,
+                             uint64_t *out_len, przm_gpu_profile *prof);
+/* ModeFlat section [0x00][u32 n][payload] (przm_ans.cu:202-234). */
+int32_t przm_gpu_pack_flat(przm_gpu *g, const uint8_t *src, uint64_t n, uint8_t *out, uint64_t cap,
+                           uint64_t *out_len, przm_gpu_profile *prof);
+
+/* ---- encode (CompressD2D bridge.go:2486, EncodeD2D 2466) ---- */
+
+/* Host buffer in, frames out (host). */
+int32_t przm_gpu_encode(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t level, uint8_t *out, uint64_t out_cap,
+                        uint64_t *out_len, uint32_t *frame_sizes, uint64_t frames_cap, uint64_t *nframes,
+                        przm_gpu_profile *prof);
+/* Device buffers in and out: one buffer, one launch per stage. */
+int32_t przm_gpu_encode_device(przm_gpu *g, const void *d_src, uint64_t n, int32_t level, void *d_out,
+                               uint64_t out_cap, uint64_t *out_len, uint32_t *frame_sizes, uint64_t frames_cap,
+                               uint64_t *nframes, przm_gpu_profile *prof);
+/* Batched form: many buffers of a device arena through one hipGraph launch of
+ * the persistent pipeline.  out_off[i] = byte offset of buffer i's first
+ * frame in d_out (nbufs + 1 entries, the last = total); frame_sizes gets
+ * every frame in buffer order. */
+int32_t przm_gpu_encode_batch_device(przm_gpu *g, const void *d_src, const przm_span *bufs, uint32_t nbufs,
+                                     int32_t level, void *d_out, uint64_t out_cap, uint64_t *out_off,
+                                     uint32_t *frame_sizes, uint64_t frames_cap, uint64_t *nframes,
+                                     przm_gpu_profile *prof);
+/* Same, with host buffers (adds the DMA both ways). */
+int32_t przm_gpu_encode_batch(przm_gpu *g, const uint8_t *src, const przm_span *bufs, uint32_t nbufs, int32_t level,
+                              uint8_t *out, uint64_t out_cap, uint64_t *out_off, uint32_t *frame_sizes,
+                              uint64_t frames_cap, uint64_t *nframes, przm_gpu_profile *prof);
+/* Upper bound of the frame bytes of n input bytes (64 KiB blocks). */
+uint64_t przm_gpu_frames_bound(uint64_t n);
+
+/* ---- autotuner ---- */
+/* Times candidate launch shapes on a sample of the given device arena and
+ * keeps the fastest for this handle; tuned[0..2] = lz grid, chunk blocks,
+ * rans jobs per workgroup. */
+int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs, uint32_t nbufs, int32_t level,
+                          int32_t tuned[3]);
+int32_t przm_gpu_set_tuning(przm_gpu *g, int32_t lz_grid, int32_t chunk_blocks);
+
+/* ---- device memory for callers without an allocator ---- */
+int32_t przm_gpu_malloc(przm_gpu *g, uint64_t bytes, void **d);
+int32_t przm_gpu_free(przm_gpu *g, void *d);
+int32_t przm_gpu_h2d(przm_gpu *g, void *d, const void *h, uint64_t bytes);
+int32_t przm_gpu_d2h(przm_gpu *g, void *h, const void *d, uint64_t bytes);
+int32_t przm_gpu_sync(przm_gpu *g);
+/* AllocPinnedHost / FreePinnedHost (gpu/memory.go:51-90). */
+int32_t przm_pinned_alloc(uint64_t bytes, void **p);
+int32_t przm_pinned_free(void *p);
+
+/* ---- host helpers (gpu/bridge.go:2777-2886) ---- */
+/* NormalizeFreq: the deterministic normaliser of core/literals/literals.go:78-140
+ * (target_m must be 4096). */
+int32_t przm_normalize_freq(const uint32_t counts[256], uint32_t target_m, uint16_t freq[256]);
+/* BuildCDF: cdf[0] = 0, cdf[s+1] = cdf[s] + freq[s]. */
+int32_t przm_build_cdf(const uint16_t freq[256], uint16_t cdf[257]);
+/* SerializeMatrix: joint-nibble matrix of literals.go:143-184 (<= 546 bytes). */
+int32_t przm_serialize_matrix(const uint16_t freq[256], uint8_t *out, uint64_t cap, uint64_t *len);
+
+/* Text of the last HIP / capacity error on this thread (empty if none). */
+const char *przm_gpu_last_error(void);
+
+/* ---- serving loop (continuous batching under an HBM budget) ---- */
+typedef struct {
+    uint64_t id;
+    int32_t status;
+    uint32_t nframes;
+    uint64_t out_len;
+    double submit_ms, start_ms, done_ms; /* clock of the server, ms since creation */
+} przm_result;
+
+/* max_batch_bytes bounds one launch; hbm_budget bounds the device bytes of
+ * all admitted requests (inputs + frames + workspace). */
+int32_t przm_serve_create(przm_gpu *g, int32_t level, uint64_t hbm_budget, uint64_t max_batch_bytes,
+                          uint32_t streams, przm_server **out);
+/* Copies src; returns a request id.  PRZM_GPU_ERR_BUSY when the host queue is full. */
+int32_t przm_serve_submit(przm_server *s, const uint8_t *src, uint64_t n, uint64_t *id);
+/* Waits up to timeout_ms (-1 = forever) for request id; copies frames into
+ * out (cap bytes) and sizes into frame_sizes when given. */
+int32_t przm_serve_wait(przm_server *s, uint64_t id, int64_t timeout_ms, uint8_t *out, uint64_t cap,
+                        uint32_t *frame_sizes, uint64_t frames_cap, przm_result *res);
+/* counters: [0] launches, [1] requests done, [2] bytes in, [3] bytes out,
+ * [4] peak admitted device bytes, [5] max requests in one launch,
+ * [6] launches that overlapped another in flight */
+int32_t przm_serve_stats(przm_server *s, uint64_t stats[8]);
+int32_t przm_serve_destroy(przm_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

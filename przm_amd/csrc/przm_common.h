@@ -1,0 +1,83 @@
+/* Shared constants and layouts of the MI355X encode path (device + host). */
+#ifndef PRZM_COMMON_H
+#define PRZM_COMMON_H
+
+#include <stdint.h>
+
+#define PRZM_BLOCK 65536u       /* frame block == match window (64 KiB) */
+#define PRZM_HALF 32768u
+#define PRZM_MAX_MATCH 1024u    /* compress.go:28 */
+#define PRZM_EXT_CAP 65530u     /* przm_tokenizer.cuh:296 */
+#define PRZM_RLE_CAP 65535u     /* przm_tokenizer.cuh:241 */
+
+/* per-block stream slot capacities (bytes) */
+#define PRZM_CMD_CAP 36864u     /* >= n/2 + 64: every token consumes >= 4 input bytes */
+#define PRZM_OFF_CAP 49152u     /* <= 3 LEB bytes per explicit match of >= 4 bytes */
+#define PRZM_LIT_CAP 65536u
+#define PRZM_OP_CAP 36864u
+#define PRZM_META_CAP 18432u
+#define PRZM_VAR_CAP 36864u
+
+/* stream indices */
+enum { PRZM_S_CMD = 0, PRZM_S_OFF, PRZM_S_LIT, PRZM_S_OP, PRZM_S_META, PRZM_S_VAR, PRZM_S_TOK, PRZM_S_N };
+
+/* entropy jobs per block: literal V-ANS, and Meta8 / V-ANS for each command substream */
+enum { PRZM_J_LIT = 0, PRZM_J_OP_M8, PRZM_J_OP_V, PRZM_J_META_M8, PRZM_J_META_V, PRZM_J_VAR_M8, PRZM_J_VAR_V, PRZM_J_N };
+
+/* one 64 KiB (or shorter, last) block of some input buffer */
+typedef struct {
+    uint64_t src_off;   /* byte offset of the block in the device input */
+    uint32_t len;       /* 0..65536 */
+    uint32_t flags;     /* PRZM_BF_* */
+} przm_block_t;
+
+#define PRZM_BF_RAW 1u      /* emit an uncompressed frame (high-entropy sample or empty) */
+
+/* Byte offsets of the per-block stream slots inside one block workspace. */
+#define PRZM_WS_CMD 0u
+#define PRZM_WS_OFF (PRZM_WS_CMD + PRZM_CMD_CAP)
+#define PRZM_WS_LIT (PRZM_WS_OFF + PRZM_OFF_CAP)
+#define PRZM_WS_OP (PRZM_WS_LIT + PRZM_LIT_CAP + 64u)
+#define PRZM_WS_META (PRZM_WS_OP + PRZM_OP_CAP)
+#define PRZM_WS_VAR (PRZM_WS_META + PRZM_META_CAP)
+#define PRZM_WS_STREAMS (PRZM_WS_VAR + PRZM_VAR_CAP)
+
+/* Entropy outputs: each job's coded stream is written backward ending at
+ * the end of its slot.  A coded stream is only ever chosen when it is
+ * shorter than the raw stream, so a job stops (and reports overflow) once
+ * its output would exceed the raw length: slot = raw cap + 64. */
+#define PRZM_ENC_LIT_CAP (PRZM_LIT_CAP + 64u)
+#define PRZM_ENC_OP_CAP (PRZM_OP_CAP + 64u)
+#define PRZM_ENC_META_CAP (PRZM_META_CAP + 64u)
+#define PRZM_ENC_VAR_CAP (PRZM_VAR_CAP + 64u)
+#define PRZM_WS_ENC_LIT PRZM_WS_STREAMS
+#define PRZM_WS_ENC_OP_M8 (PRZM_WS_ENC_LIT + PRZM_ENC_LIT_CAP)
+#define PRZM_WS_ENC_OP_V (PRZM_WS_ENC_OP_M8 + PRZM_ENC_OP_CAP)
+#define PRZM_WS_ENC_META_M8 (PRZM_WS_ENC_OP_V + PRZM_ENC_OP_CAP)
+#define PRZM_WS_ENC_META_V (PRZM_WS_ENC_META_M8 + PRZM_ENC_META_CAP)
+#define PRZM_WS_ENC_VAR_M8 (PRZM_WS_ENC_META_V + PRZM_ENC_META_CAP)
+#define PRZM_WS_ENC_VAR_V (PRZM_WS_ENC_VAR_M8 + PRZM_ENC_VAR_CAP)
+#define PRZM_WS_BYTES (PRZM_WS_ENC_VAR_V + PRZM_ENC_VAR_CAP)
+#define PRZM_ENC_OVERFLOW 0xFFFFFFFFu
+
+/* Per-block metadata written by the kernels (u32 words). */
+typedef struct {
+    uint32_t len[PRZM_S_N];        /* stream lengths + token count */
+    uint32_t enc_len[PRZM_J_N];    /* coded stream bytes (no header) per job; 0xFFFFFFFF = not run */
+    uint32_t enc_state[PRZM_J_N][4];
+    uint32_t mat_len[4];           /* joint-nibble matrix bytes for lit, op, meta, var */
+    uint32_t frame_len;            /* final frame bytes */
+    uint32_t frame_kind;           /* 0 compressed, 1 raw (entropy/empty), 2 raw (size guard), 3 raw (S4 hazard) */
+    uint32_t choice[3];            /* substream coders chosen: 0 flat, 1 Meta8, 2 V-ANS; -1 flat section */
+    uint32_t s_len[3];             /* S2, S3, S4 bytes */
+    uint32_t s4_vans;
+    uint32_t pad[3];
+} przm_meta_t;
+
+/* Normalized frequency tables per V-ANS job (lit, op, meta, var). */
+typedef struct {
+    uint32_t fc[256];       /* freq | cdf << 16 */
+    uint8_t mat[552];       /* serialized joint-nibble matrix */
+} przm_ftab_t;
+
+#endif

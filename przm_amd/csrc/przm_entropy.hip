@@ -1,0 +1,585 @@
+/*
+ * przm_entropy.hip -- block checks, histograms, frequency normalisation,
+ * 4-lane rANS (V-ANS and Meta8) and frame assembly for gfx950.
+ *
+ *  przm_check_kernel    checkHighEntropy (compress.go:113-132) per block.
+ *  przm_hist_kernel     one wave per (block, stream): 256-bin histogram in
+ *                       LDS, NormalizeFrequencies (literals.go:78-140) with
+ *                       the deficit handed out by a bitonic sort on
+ *                       (count desc, symbol asc), CDF, and the joint-nibble
+ *                       matrix (literals.go:143-184).
+ *  przm_rans_kernel     one lane quad per (block, coder job).  The format
+ *                       fixes four interleaved states per stream
+ *                       (literals.go:302-325, vans_meta.go:104-132): quad
+ *                       lane j owns state j, the four lanes step together
+ *                       through the reverse pass, and the bytes a step
+ *                       emits (0-2 per lane) are ordered lane 3,2,1,0 by a
+ *                       quad prefix sum and written backward from the end
+ *                       of the job's slot, which leaves the container's
+ *                       reversed stream in place.  x/f uses an fp32
+ *                       reciprocal estimate (x < 2^24, q < 4096, so the
+ *                       estimate is within one) with an exact integer
+ *                       correction.
+ *  przm_frame_size_kernel / przm_frame_write_kernel
+ *                       section choices and frame bytes per block
+ *                       (commands.go:371-444, 537-568; literals.go:368-384,
+ *                       664-693; compress.go:962-1033), plus the mode's
+ *                       guard against a raw Section 4 that decoders would
+ *                       misread as a container.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "przm_common.h"
+#include "przm_meta8_freq.h"
+
+/* ---------------- block check ---------------- */
+
+extern "C" __global__ void __launch_bounds__(64)
+przm_check_kernel(const uint8_t *__restrict__ src, przm_block_t *__restrict__ blocks, uint32_t nblocks) {
+    __shared__ uint32_t seen[8];
+    const uint32_t blk = blockIdx.x, lane = threadIdx.x;
+    if (blk >= nblocks) return;
+    const przm_block_t bd = blocks[blk];
+    const uint32_t n = bd.len;
+    const uint32_t sample = n < 512 ? n : 512;
+    if (lane < 8) seen[lane] = 0;
+    __syncthreads();
+    for (uint32_t i = lane; i < sample; i += 64) {
+        const uint32_t b = src[bd.src_off + i];
+        atomicOr(&seen[b >> 5], 1u << (b & 31));
+    }
+    __syncthreads();
+    if (lane == 0) {
+        uint32_t unique = 0;
+        for (int k = 0; k < 8; k++) unique += (uint32_t)__popc(seen[k]);
+        const uint32_t thr = sample >= 256 ? (256u * 88u) / 100u : (sample * 88u) / 100u;
+        const bool high = sample >= 64 && unique > thr;
+        blocks[blk].flags = (high || n == 0) ? (bd.flags | PRZM_BF_RAW) : (bd.flags & ~PRZM_BF_RAW);
+    }
+}
+
+/* ---------------- histogram + normalisation ---------------- */
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+/* stream of job kind k (0 lit, 1 op, 2 meta, 3 var) */
+__device__ __forceinline__ void stream_of(uint32_t k, uint8_t *wsb, const przm_meta_t *mt, const uint8_t **p,
+                                          uint32_t *n) {
+    switch (k) {
+    case 0: *p = wsb + PRZM_WS_LIT; *n = mt->len[PRZM_S_LIT]; break;
+    case 1: *p = wsb + PRZM_WS_OP; *n = mt->len[PRZM_S_OP]; break;
+    case 2: *p = wsb + PRZM_WS_META; *n = mt->len[PRZM_S_META]; break;
+    default: *p = wsb + PRZM_WS_VAR; *n = mt->len[PRZM_S_VAR]; break;
+    }
+}
+
+/*
+ * NormalizeFrequencies (literals.go:78-140) + CDF + joint-nibble matrix
+ * (literals.go:143-184) for the 256 counts in cnt[], by one wave.  key[] and
+ * frq[] are LDS scratch of 256 words each.
+ */
+__device__ void normalize_wave(const uint32_t *cnt, uint32_t *key, uint32_t *frq, przm_ftab_t *ft,
+                               uint32_t *mat_len) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t c4[4], f4[4];
+    uint32_t tot = 0, uni = 0;
+    for (int i = 0; i < 4; i++) { c4[i] = cnt[lane * 4 + i]; tot += c4[i]; uni += c4[i] ? 1u : 0u; }
+    tot = wave_sum(tot);
+    uni = wave_sum(uni);
+    const uint32_t rem = 4096u - uni;
+    uint32_t sum = 0;
+    for (int i = 0; i < 4; i++) {
+        /* c * rem < 2^28 for c <= 65536; larger streams take the 64-bit product */
+        f4[i] = c4[i] ? 1u + (uint32_t)(((uint64_t)c4[i] * rem) / (tot ? tot : 1u)) : 0u;
+        sum += f4[i];
+        const uint32_t s = lane * 4 + i;
+        key[s] = c4[i] >= (1u << 24) ? 0xFFFFFF00u | (255u - s) : (c4[i] << 8) | (255u - s);
+    }
+    sum = wave_sum(sum);
+    const uint32_t diff = 4096u - sum;
+    __builtin_amdgcn_wave_barrier();
+    /* bitonic sort, descending, of (count, -symbol) keys */
+    for (uint32_t size = 2; size <= 256; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = lane; t < 128; t += 64) {
+                const uint32_t i = 2 * t - (t & (stride - 1));
+                const uint32_t j = i + stride;
+                const bool desc = ((i & size) == 0);
+                const uint32_t ki = key[i], kj = key[j];
+                if (desc ? (ki < kj) : (ki > kj)) { key[i] = kj; key[j] = ki; }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    for (int i = 0; i < 4; i++) frq[lane * 4 + i] = f4[i];
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t r = lane; r < 256; r += 64) {
+        if (r < diff) frq[255u - (key[r] & 255u)] += 1u;
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t loc[4], acc = 0;
+    for (int i = 0; i < 4; i++) { loc[i] = frq[lane * 4 + i]; acc += loc[i]; }
+    uint32_t x = acc;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    uint32_t cum = x - acc;
+    for (int i = 0; i < 4; i++) {
+        ft->fc[lane * 4 + i] = loc[i] | (cum << 16);
+        cum += loc[i];
+    }
+    if (lane == 0) {
+        uint8_t *o = ft->mat;
+        uint32_t len = 0, rows = 0;
+        for (int h = 0; h < 16; h++)
+            for (int l = 0; l < 16; l++)
+                if (frq[(h << 4) | l]) { rows |= 1u << h; break; }
+        o[len++] = (uint8_t)rows; o[len++] = (uint8_t)(rows >> 8);
+        for (int h = 0; h < 16; h++) {
+            if (!(rows & (1u << h))) continue;
+            uint32_t cols = 0;
+            for (int l = 0; l < 16; l++) if (frq[(h << 4) | l]) cols |= 1u << l;
+            o[len++] = (uint8_t)cols; o[len++] = (uint8_t)(cols >> 8);
+            for (int l = 0; l < 16; l++) {
+                if (!(cols & (1u << l))) continue;
+                const uint32_t wgt = frq[(h << 4) | l];
+                if (wgt < 128) o[len++] = (uint8_t)wgt;
+                else { o[len++] = (uint8_t)((wgt >> 8) | 0x80); o[len++] = (uint8_t)wgt; }
+            }
+        }
+        *mat_len = len;
+    }
+}
+
+/* one wave per (block, stream) job: lit, op, meta, var */
+extern "C" __global__ void __launch_bounds__(256)
+przm_hist_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, przm_ftab_t *__restrict__ ftab,
+                 uint32_t nblocks) {
+    __shared__ uint32_t cnt[4][256];
+    __shared__ uint32_t key[4][256];
+    __shared__ uint32_t frq[4][256];
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t job = blockIdx.x * 4 + w;
+    const bool alive = job < nblocks * 4;
+    const uint32_t blk = alive ? job >> 2 : 0, k = job & 3;
+    const uint8_t *p = nullptr;
+    uint32_t n = 0;
+    if (alive) stream_of(k, ws + (uint64_t)blk * PRZM_WS_BYTES, meta + blk, &p, &n);
+    const uint32_t minlen = k == 0 ? 1u : 16u;
+    const bool run = alive && n >= minlen;
+    for (int i = 0; i < 4; i++) cnt[w][lane * 4 + i] = 0;
+    __builtin_amdgcn_wave_barrier();
+    if (run) {
+        for (uint32_t i = lane; i < n; i += 64) atomicAdd(&cnt[w][p[i]], 1u);
+    }
+    __syncthreads();
+    if (!run) return;
+    normalize_wave(cnt[w], key[w], frq[w], ftab + (uint64_t)blk * 4 + k, &meta[blk].mat_len[k]);
+}
+
+/* whole-buffer histogram (one workgroup): counts_out, and the normalised table */
+extern "C" __global__ void __launch_bounds__(256)
+przm_hist1_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__restrict__ counts_out,
+                  przm_ftab_t *__restrict__ ft, uint32_t *__restrict__ mat_len) {
+    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t key[256];
+    __shared__ uint32_t frq[256];
+    const uint32_t tid = threadIdx.x;
+    cnt[tid] = 0;
+    __syncthreads();
+    uint64_t i = tid;
+    for (; i + 3 * 256 < n; i += 4 * 256) {
+        atomicAdd(&cnt[src[i]], 1u); atomicAdd(&cnt[src[i + 256]], 1u);
+        atomicAdd(&cnt[src[i + 512]], 1u); atomicAdd(&cnt[src[i + 768]], 1u);
+    }
+    for (; i < n; i += 256) atomicAdd(&cnt[src[i]], 1u);
+    __syncthreads();
+    counts_out[tid] = cnt[tid];
+    if (tid < 64 && ft) normalize_wave(cnt, key, frq, ft, mat_len);
+}
+
+/* ---------------- rANS ---------------- */
+
+#define RANS_NT 256
+#define RANS_JOBS (RANS_NT / 4)
+
+__device__ __forceinline__ uint32_t ctx_meta(uint32_t p) { /* vans_meta.go:45-68 */
+    return p == 0 ? 0u : p < 4 ? 1u : p < 16 ? 2u : p < 32 ? 3u : p < 64 ? 4u : p < 128 ? 5u : p < 192 ? 6u : 7u;
+}
+
+/* x / f and x % f for x < 2^24, f in [1, 4096] */
+__device__ __forceinline__ void divmod(uint32_t x, uint32_t f, uint32_t *q, uint32_t *r) {
+    uint32_t qq = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)f));
+    int32_t rr = (int32_t)(x - qq * f);
+    if (rr < 0) { qq -= 1; rr += (int32_t)f; }
+    else if (rr >= (int32_t)f) { qq += 1; rr -= (int32_t)f; }
+    *q = qq;
+    *r = (uint32_t)rr;
+}
+
+__device__ __forceinline__ void job_layout(uint32_t kind, uint8_t *wsb, const przm_meta_t *mt, const uint8_t **src,
+                                           uint32_t *n, uint8_t **slot_end, int *m8, uint32_t *tab) {
+    switch (kind) {
+    case PRZM_J_LIT: *src = wsb + PRZM_WS_LIT; *n = mt->len[PRZM_S_LIT]; *slot_end = wsb + PRZM_WS_ENC_LIT + PRZM_ENC_LIT_CAP; *m8 = 0; *tab = 0; break;
+    case PRZM_J_OP_M8: *src = wsb + PRZM_WS_OP; *n = mt->len[PRZM_S_OP]; *slot_end = wsb + PRZM_WS_ENC_OP_M8 + PRZM_ENC_OP_CAP; *m8 = 1; *tab = 1; break;
+    case PRZM_J_OP_V: *src = wsb + PRZM_WS_OP; *n = mt->len[PRZM_S_OP]; *slot_end = wsb + PRZM_WS_ENC_OP_V + PRZM_ENC_OP_CAP; *m8 = 0; *tab = 1; break;
+    case PRZM_J_META_M8: *src = wsb + PRZM_WS_META; *n = mt->len[PRZM_S_META]; *slot_end = wsb + PRZM_WS_ENC_META_M8 + PRZM_ENC_META_CAP; *m8 = 1; *tab = 2; break;
+    case PRZM_J_META_V: *src = wsb + PRZM_WS_META; *n = mt->len[PRZM_S_META]; *slot_end = wsb + PRZM_WS_ENC_META_V + PRZM_ENC_META_CAP; *m8 = 0; *tab = 2; break;
+    case PRZM_J_VAR_M8: *src = wsb + PRZM_WS_VAR; *n = mt->len[PRZM_S_VAR]; *slot_end = wsb + PRZM_WS_ENC_VAR_M8 + PRZM_ENC_VAR_CAP; *m8 = 1; *tab = 3; break;
+    default: *src = wsb + PRZM_WS_VAR; *n = mt->len[PRZM_S_VAR]; *slot_end = wsb + PRZM_WS_ENC_VAR_V + PRZM_ENC_VAR_CAP; *m8 = 0; *tab = 3; break;
+    }
+}
+
+struct M8Table {
+    uint32_t fc[8][256];
+    constexpr M8Table() : fc() {
+        for (int c = 0; c < 8; c++) {
+            uint32_t cum = 0;
+            for (int s = 0; s < 256; s++) {
+                fc[c][s] = (uint32_t)przm_meta8_freq[c][s] | (cum << 16);
+                cum += przm_meta8_freq[c][s];
+            }
+        }
+    }
+};
+__constant__ M8Table przm_m8 = M8Table();
+
+/*
+ * The reverse 4-state pass of one stream by one lane quad (lane j = state j):
+ * literals.go:305-325 / vans_meta.go:109-132.  Emitted bytes go backward from
+ * slot_end; at most cap bytes.  Returns the byte count (or overflow) and the
+ * lane's final state.
+ */
+__device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, uint32_t n, bool m8,
+                                              const uint32_t *vt, const uint32_t (*m8t)[256],
+                                              uint8_t *__restrict__ slot_end, uint32_t cap, uint32_t *state) {
+    const uint32_t j = threadIdx.x & 3;
+    const uint32_t groups = (n + 3) >> 2;
+    uint32_t x = 65536u, K = 0;
+    bool over = false;
+    for (uint32_t g = groups; g-- > 0;) {
+        const uint32_t i = 4 * g + j;
+        const bool v = i < n;
+        uint32_t f = 1, c = 0;
+        if (v) {
+            const uint32_t s = src[i];
+            const uint32_t e = m8 ? m8t[i > 0 ? ctx_meta(src[i - 1]) : 0u][s] : vt[s];
+            f = e & 0xFFFFu;
+            c = e >> 16;
+        }
+        const uint32_t t = 4096u * f - 1u;
+        uint32_t ne = 0, b0 = 0, b1 = 0;
+        if (v && x > t) {
+            b0 = x & 0xFFu; x >>= 8; ne = 1;
+            if (x > t) { b1 = x & 0xFFu; x >>= 8; ne = 2; }
+        }
+        /* bytes of higher lanes come first: lane 3 emits before lane 2, ... */
+        const uint32_t e1 = __shfl_down(ne, 1, 4), e2 = __shfl_down(ne, 2, 4), e3 = __shfl_down(ne, 3, 4);
+        const uint32_t after = (j < 3 ? e1 : 0u) + (j < 2 ? e2 : 0u) + (j < 1 ? e3 : 0u);
+        const uint32_t tot = __shfl(ne + after, 0, 4);
+        if (K + tot > cap) { over = true; break; }
+        if (ne) {
+            const uint64_t k0 = K + after;
+            slot_end[-1 - (int64_t)k0] = (uint8_t)b0;
+            if (ne == 2) slot_end[-2 - (int64_t)k0] = (uint8_t)b1;
+        }
+        K += tot;
+        if (v) {
+            uint32_t qq, rr;
+            divmod(x, f, &qq, &rr);
+            x = qq * 4096u + c + rr;
+        }
+    }
+    *state = x;
+    return over ? PRZM_ENC_OVERFLOW : K;
+}
+
+/* one lane quad per (block, coder job); jobs kind-major so a wave's quads
+ * code streams of the same kind (similar lengths) */
+extern "C" __global__ void __launch_bounds__(RANS_NT)
+przm_rans_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const przm_ftab_t *__restrict__ ftab,
+                 uint32_t nblocks) {
+    __shared__ uint32_t vtab[RANS_JOBS][256];
+    const uint32_t tid = threadIdx.x, q = tid >> 2, j = tid & 3;
+    const uint32_t job = blockIdx.x * RANS_JOBS + q;
+    const uint32_t kind = job / (nblocks ? nblocks : 1u), blk = job - kind * nblocks;
+    bool run = kind < PRZM_J_N;
+    const uint8_t *src = nullptr;
+    uint8_t *slot_end = nullptr;
+    uint32_t n = 0, tsel = 0;
+    int m8 = 0;
+    uint8_t *wsb = ws + (uint64_t)blk * PRZM_WS_BYTES;
+    if (run) {
+        job_layout(kind, wsb, meta + blk, &src, &n, &slot_end, &m8, &tsel);
+        const uint32_t minlen = kind == PRZM_J_LIT ? 1u : (m8 ? 24u : 16u);
+        run = n >= minlen;
+    }
+    if (run && !m8) {
+        const przm_ftab_t *ft = ftab + (uint64_t)blk * 4 + tsel;
+        for (uint32_t s = j; s < 256; s += 4) vtab[q][s] = ft->fc[s];
+    }
+    __syncthreads();
+    if (!run) {
+        if (kind < PRZM_J_N && j == 0) meta[blk].enc_len[kind] = PRZM_ENC_OVERFLOW;
+        return;
+    }
+    uint32_t x;
+    const uint32_t K = rans_quad(src, n, m8 != 0, vtab[q], przm_m8.fc, slot_end, n, &x);
+    przm_meta_t *mt = meta + blk;
+    if (j == 0) mt->enc_len[kind] = K;
+    mt->enc_state[kind][j] = x;
+}
+
+/* a single V-ANS job over an arbitrary buffer (one quad) */
+extern "C" __global__ void __launch_bounds__(64)
+przm_rans1_kernel(const uint8_t *__restrict__ src, uint32_t n, const przm_ftab_t *__restrict__ ft,
+                  uint8_t *__restrict__ slot_end, uint32_t cap, uint32_t *__restrict__ out) {
+    __shared__ uint32_t vt[256];
+    for (uint32_t s = threadIdx.x; s < 256; s += 64) vt[s] = ft->fc[s];
+    __syncthreads();
+    if (threadIdx.x >= 4) return;
+    uint32_t x;
+    const uint32_t K = rans_quad(src, n, false, vt, przm_m8.fc, slot_end, cap, &x);
+    if (threadIdx.x == 0) out[0] = K;
+    out[1 + threadIdx.x] = x;
+}
+
+/* ---------------- frame assembly ---------------- */
+
+struct Plan {
+    uint32_t kind, flags, l2, l3, l4, ext, sub_len[3], sub_ch[3], s4v;
+};
+
+__device__ __forceinline__ uint32_t rd32g(const uint8_t *p) {
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+__device__ Plan plan_frame(const przm_block_t &bd, const przm_meta_t *mt, const uint8_t *wsb) {
+    Plan P = {};
+    const uint32_t n = bd.len;
+    if ((bd.flags & PRZM_BF_RAW) || n == 0) { P.kind = 1; return P; }
+    /* Section 2: commands.go:371-406 per substream, 409-444 + 537-568 for the section */
+    const uint32_t ncmd = mt->len[PRZM_S_CMD];
+    const uint32_t slen[3] = {mt->len[PRZM_S_OP], mt->len[PRZM_S_META], mt->len[PRZM_S_VAR]};
+    uint32_t pack = 13;
+    for (int s = 0; s < 3; s++) {
+        uint32_t best = 1 + slen[s], ch = 0;
+        const uint32_t em8 = mt->enc_len[1 + 2 * s], ev = mt->enc_len[2 + 2 * s];
+        if (slen[s] >= 24 && em8 != PRZM_ENC_OVERFLOW && 24 + em8 < best) { best = 24 + em8; ch = 1; }
+        if (slen[s] >= 16 && ev != PRZM_ENC_OVERFLOW) {
+            const uint32_t tv = 8 + mt->mat_len[1 + s] + 16 + ev;
+            if (tv < best) { best = tv; ch = 2; }
+        }
+        P.sub_len[s] = best;
+        P.sub_ch[s] = ch;
+        pack += best;
+    }
+    if (ncmd == 0) P.l2 = 0;
+    else if (pack < ncmd) P.l2 = pack;
+    else { P.l2 = 1 + ncmd; P.sub_ch[0] = P.sub_ch[1] = P.sub_ch[2] = 0xFFFFFFFFu; }
+    /* Section 3: flat LEB128 */
+    const uint32_t noff = mt->len[PRZM_S_OFF];
+    P.l3 = noff ? 1 + noff : 0;
+    /* Section 4: literals.go:368-384 / 664-693 without the sheaf trials */
+    const uint32_t nlit = mt->len[PRZM_S_LIT];
+    P.s4v = 0;
+    P.l4 = nlit;
+    if (nlit) {
+        const uint32_t ev = mt->enc_len[PRZM_J_LIT];
+        if (ev != PRZM_ENC_OVERFLOW) {
+            const uint32_t tv = 8 + mt->mat_len[0] + 16 + ev;
+            if (tv < nlit) { P.l4 = tv; P.s4v = 1; }
+        }
+    }
+    P.ext = (P.l2 > 65535 || P.l3 > 65535) ? 1 : 0;
+    const uint32_t comp = 8 + (P.ext ? 6 : 4) + 8 + P.l2 + P.l3 + P.l4;
+    bool hazard = false;
+    if (!P.s4v && P.l4 >= 8) {
+        const uint8_t *lit = wsb + PRZM_WS_LIT;
+        hazard = (lit[0] == 'V' && lit[1] == 'A' && lit[2] == 0x01) || rd32g(lit + 4) == (rd32g(lit) + 1) / 2;
+    }
+    if (comp >= n) P.kind = 2;
+    else if (hazard) P.kind = 3;
+    else P.kind = 0;
+    P.flags = (P.s4v ? 0x80u : 0u) | (P.ext ? 0x08u : 0u);
+    return P;
+}
+
+__device__ __forceinline__ uint32_t frame_bytes(const przm_block_t &bd, const Plan &P) {
+    if (P.kind != 0) return 8 + bd.len;
+    return 8 + (P.ext ? 6 : 4) + 8 + P.l2 + P.l3 + P.l4;
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+przm_frame_size_kernel(const przm_block_t *__restrict__ blocks, przm_meta_t *__restrict__ meta,
+                       const uint8_t *__restrict__ ws, uint32_t nblocks, uint32_t *__restrict__ frame_len) {
+    const uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= nblocks) return;
+    const przm_block_t bd = blocks[blk];
+    przm_meta_t *mt = meta + blk;
+    const Plan P = plan_frame(bd, mt, ws + (uint64_t)blk * PRZM_WS_BYTES);
+    const uint32_t fl = frame_bytes(bd, P);
+    mt->frame_len = fl;
+    mt->frame_kind = P.kind;
+    for (int s = 0; s < 3; s++) mt->choice[s] = P.sub_ch[s];
+    mt->s_len[0] = P.l2; mt->s_len[1] = P.l3; mt->s_len[2] = P.l4;
+    mt->s4_vans = P.s4v;
+    frame_len[blk] = fl;
+}
+
+/* Exclusive scan of frame lengths into 64-bit offsets; base/total carried in *run
+ * so consecutive chunks continue the numbering.  One workgroup of 1024 threads. */
+extern "C" __global__ void __launch_bounds__(1024)
+przm_scan_kernel(const uint32_t *__restrict__ len, uint32_t count, uint64_t *__restrict__ off,
+                 uint64_t *__restrict__ run) {
+    __shared__ uint64_t part[1024];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (count + 1023) / 1024;
+    const uint32_t b = tid * per, e = b + per < count ? b + per : count;
+    uint64_t s = 0;
+    for (uint32_t i = b; i < e; i++) s += len[i];
+    part[tid] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        const uint64_t y = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += y;
+        __syncthreads();
+    }
+    const uint64_t base = *run;
+    uint64_t acc = base + part[tid] - s;
+    for (uint32_t i = b; i < e; i++) { off[i] = acc; acc += len[i]; }
+    __syncthreads();
+    if (tid == 1023) *run = base + part[1023];
+}
+
+__device__ __forceinline__ void wr_bytes(uint8_t *dst, const uint8_t *src, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+__device__ __forceinline__ void wr_le32(uint8_t *p, uint32_t v) {
+    p[0] = (uint8_t)v; p[1] = (uint8_t)(v >> 8); p[2] = (uint8_t)(v >> 16); p[3] = (uint8_t)(v >> 24);
+}
+
+/* the V-ANS container of job `jk` with matrix of ftab entry `t`: 8 + mat + 16 + stream */
+__device__ uint32_t put_vans(uint8_t *o, const uint8_t *wsb, const przm_meta_t *mt, const przm_ftab_t *ft, uint32_t jk,
+                             uint32_t n, uint32_t t, const uint8_t *slot_end) {
+    const uint32_t ml = mt->mat_len[t], sl = mt->enc_len[jk];
+    if (threadIdx.x == 0) {
+        o[0] = 'V'; o[1] = 'A'; o[2] = 0x01; o[3] = 0x00;
+        wr_le32(o + 4, n);
+        for (int j = 0; j < 4; j++) wr_le32(o + 8 + ml + 4 * j, mt->enc_state[jk][j]);
+    }
+    wr_bytes(o + 8, ft[t].mat, ml);
+    wr_bytes(o + 8 + ml + 16, slot_end - sl, sl);
+    return 8 + ml + 16 + sl;
+}
+
+__device__ uint32_t put_meta8(uint8_t *o, const przm_meta_t *mt, uint32_t jk, uint32_t n, const uint8_t *slot_end) {
+    const uint32_t sl = mt->enc_len[jk];
+    if (threadIdx.x == 0) {
+        o[0] = 'V'; o[1] = 'A'; o[2] = 0x01; o[3] = 0x41;
+        wr_le32(o + 4, n);
+        for (int j = 0; j < 4; j++) wr_le32(o + 8 + 4 * j, mt->enc_state[jk][j]);
+    }
+    wr_bytes(o + 24, slot_end - sl, sl);
+    return 24 + sl;
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+przm_frame_write_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks,
+                        const przm_meta_t *__restrict__ meta, const uint8_t *__restrict__ ws,
+                        const przm_ftab_t *__restrict__ ftab, const uint64_t *__restrict__ frame_off,
+                        uint8_t *__restrict__ out, uint32_t nblocks) {
+    const uint32_t blk = blockIdx.x;
+    if (blk >= nblocks) return;
+    const przm_block_t bd = blocks[blk];
+    const przm_meta_t *mt = meta + blk;
+    const uint8_t *wsb = ws + (uint64_t)blk * PRZM_WS_BYTES;
+    const przm_ftab_t *ft = ftab + (uint64_t)blk * 4;
+    uint8_t *o = out + frame_off[blk];
+    const uint32_t n = bd.len;
+    if (mt->frame_kind != 0) { /* compress.go:973-981 */
+        if (threadIdx.x == 0) {
+            o[0] = 'P'; o[1] = 'R'; o[2] = 'Z'; o[3] = 'M'; o[4] = 0x04;
+            o[5] = (uint8_t)n; o[6] = (uint8_t)(n >> 8); o[7] = (uint8_t)(n >> 16);
+        }
+        wr_bytes(o + 8, src + bd.src_off, n);
+        return;
+    }
+    const uint32_t l2 = mt->s_len[0], l3 = mt->s_len[1], l4 = mt->s_len[2];
+    const bool ext = l2 > 65535 || l3 > 65535;
+    uint32_t p = 0;
+    if (threadIdx.x == 0) { /* compress.go:988-1033 */
+        o[0] = 'P'; o[1] = 'R'; o[2] = 'Z'; o[3] = 'M';
+        o[4] = (uint8_t)((mt->s4_vans ? 0x80u : 0u) | (ext ? 0x08u : 0u));
+        o[5] = (uint8_t)n; o[6] = (uint8_t)(n >> 8); o[7] = (uint8_t)(n >> 16);
+        if (ext) {
+            o[8] = (uint8_t)l2; o[9] = (uint8_t)(l2 >> 8); o[10] = (uint8_t)(l2 >> 16);
+            o[11] = (uint8_t)l3; o[12] = (uint8_t)(l3 >> 8); o[13] = (uint8_t)(l3 >> 16);
+        } else {
+            o[8] = (uint8_t)l2; o[9] = (uint8_t)(l2 >> 8); o[10] = (uint8_t)l3; o[11] = (uint8_t)(l3 >> 8);
+        }
+    }
+    p = ext ? 14 : 12;
+    if (threadIdx.x < 8) {
+        uint32_t wl = n <= 1024 ? 10u : n <= 4096 ? 12u : n <= 16384 ? 14u : 16u; /* compress.go:183-202 */
+        o[p + threadIdx.x] = threadIdx.x == 1 ? (uint8_t)wl : 0;
+    }
+    p += 8;
+    /* Section 2 */
+    const uint32_t ncmd = mt->len[PRZM_S_CMD];
+    if (l2) {
+        if (mt->choice[0] == 0xFFFFFFFFu) {
+            if (threadIdx.x == 0) o[p] = 0x09;
+            wr_bytes(o + p + 1, wsb + PRZM_WS_CMD, ncmd);
+        } else {
+            uint8_t *s2 = o + p;
+            uint32_t q = 13;
+            const uint32_t srcoff[3] = {PRZM_WS_OP, PRZM_WS_META, PRZM_WS_VAR};
+            const uint32_t slens[3] = {mt->len[PRZM_S_OP], mt->len[PRZM_S_META], mt->len[PRZM_S_VAR]};
+            const uint32_t m8end[3] = {PRZM_WS_ENC_OP_M8 + PRZM_ENC_OP_CAP, PRZM_WS_ENC_META_M8 + PRZM_ENC_META_CAP,
+                                       PRZM_WS_ENC_VAR_M8 + PRZM_ENC_VAR_CAP};
+            const uint32_t vend[3] = {PRZM_WS_ENC_OP_V + PRZM_ENC_OP_CAP, PRZM_WS_ENC_META_V + PRZM_ENC_META_CAP,
+                                      PRZM_WS_ENC_VAR_V + PRZM_ENC_VAR_CAP};
+            uint32_t lop = 0, lm = 0;
+            for (int s = 0; s < 3; s++) {
+                uint32_t l;
+                if (mt->choice[s] == 1) l = put_meta8(s2 + q, mt, 1 + 2 * s, slens[s], wsb + m8end[s]);
+                else if (mt->choice[s] == 2) l = put_vans(s2 + q, wsb, mt, ft, 2 + 2 * s, slens[s], 1 + s, wsb + vend[s]);
+                else {
+                    if (threadIdx.x == 0) s2[q] = 0x09;
+                    wr_bytes(s2 + q + 1, wsb + srcoff[s], slens[s]);
+                    l = 1 + slens[s];
+                }
+                if (s == 0) lop = l;
+                if (s == 1) lm = l;
+                q += l;
+            }
+            if (threadIdx.x == 0) {
+                s2[0] = 0x08;
+                wr_le32(s2 + 1, ncmd);
+                wr_le32(s2 + 5, lop);
+                wr_le32(s2 + 9, lm);
+            }
+        }
+        p += l2;
+    }
+    /* Section 3 */
+    if (l3) {
+        if (threadIdx.x == 0) o[p] = 0x09;
+        wr_bytes(o + p + 1, wsb + PRZM_WS_OFF, l3 - 1);
+        p += l3;
+    }
+    /* Section 4 */
+    if (l4) {
+        if (mt->s4_vans) put_vans(o + p, wsb, mt, ft, PRZM_J_LIT, mt->len[PRZM_S_LIT], 0,
+                                  wsb + PRZM_WS_ENC_LIT + PRZM_ENC_LIT_CAP);
+        else wr_bytes(o + p, wsb + PRZM_WS_LIT, l4);
+    }
+}

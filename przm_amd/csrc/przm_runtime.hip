@@ -1,0 +1,1190 @@
+/*
+ * przm_runtime.hip -- host runtime behind include/przm_gpu.h.
+ *
+ * Replaces the reference's Go driver-API bridge (gpu/bridge.go,
+ * gpu/driver*.go, gpu/memory.go): device/context set-up, grow-only device
+ * arenas, pinned staging, the stage launch sequence, hipGraph capture of
+ * the batched form, an autotuner for the launch shapes, and a
+ * continuous-batching serving loop.  Kernels live in przm_lz.hip and
+ * przm_entropy.hip (one translation unit, so no relocatable device code).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/przm_gpu.h"
+#include "przm_entropy.hip"
+#include "przm_lz.hip"
+
+namespace {
+
+thread_local char g_err[512];
+std::atomic<int32_t> g_mock_unavailable{0};
+std::atomic<int32_t> g_mock_error{0};
+
+#define HIPCHK(x)                                                                             \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess) {                                                               \
+            snprintf(g_err, sizeof g_err, "%s:%d %s: %s", __FILE__, __LINE__, #x,             \
+                     hipGetErrorString(e_));                                                  \
+            return PRZM_GPU_ERR_HIP;                                                          \
+        }                                                                                     \
+    } while (0)
+
+#define CHK(x)                  \
+    do {                        \
+        int32_t r_ = (x);       \
+        if (r_ != 0) return r_; \
+    } while (0)
+
+constexpr uint64_t LZ_SCRATCH_STRIDE =
+    2ull * PRZM_HALF * 2 /* T0,T1 */ + 2ull * PRZM_HALF * 4 /* R0,R1 */ + 8ull * 2 * PRZM_BLOCK * 2 /* per-wave fallback */;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int32_t ensure(size_t n, uint64_t *gen) {
+        if (n <= cap && p) return 0;
+        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+        size_t c = n < 4096 ? 4096 : n;
+        if (hipMalloc(&p, c) != hipSuccess) {
+            p = nullptr;
+            snprintf(g_err, sizeof g_err, "hipMalloc(%zu) failed", c);
+            return PRZM_GPU_ERR_OOM;
+        }
+        cap = c;
+        if (gen) (*gen)++;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+/* One set of device buffers + stream: the handle owns one, the server one per slot. */
+struct Ctx {
+    hipStream_t st = nullptr;
+    DevBuf in, out, blocks, ws, meta, ftab, flen, foff, run, scratch, dbg;
+    uint64_t gen = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    struct { const void *src; void *out; uint32_t nblocks, depth, grid, chunk; uint64_t gen; } gkey{};
+    std::vector<hipEvent_t> ev;
+    void release() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        for (DevBuf *b : {&in, &out, &blocks, &ws, &meta, &ftab, &flen, &foff, &run, &scratch, &dbg}) b->release();
+        for (auto e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        if (st) (void)hipStreamDestroy(st);
+        st = nullptr;
+    }
+};
+
+__global__ void przm_zero_kernel(uint64_t *p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+/* desc[p] = (len << 16) | dist from the LZ kernel's dist output */
+__global__ void przm_desc_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks,
+                                 uint32_t nblocks, const uint16_t *__restrict__ dist, uint32_t *__restrict__ desc,
+                                 uint64_t desc_base) {
+    const uint32_t blk = blockIdx.y;
+    if (blk >= nblocks) return;
+    const przm_block_t bd = blocks[blk];
+    const uint8_t *g = src + bd.src_off;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < bd.len; p += gridDim.x * blockDim.x) {
+        const uint32_t d = dist[(uint64_t)blk * PRZM_BLOCK + p];
+        uint32_t v = 0;
+        if (d) {
+            const uint32_t cap = bd.len - p < PRZM_MAX_MATCH ? bd.len - p : PRZM_MAX_MATCH;
+            uint32_t l = 0;
+            while (l < cap && g[p + l] == g[p - d + l]) l++;
+            v = (l << 16) | d;
+        }
+        desc[(uint64_t)blk * PRZM_BLOCK + p - desc_base] = v;
+    }
+}
+
+int depth_of_level(int level) { /* compress.go:34-51, default level 6 (compress.go:1038-1040) */
+    if (level <= 0) level = 6;
+    if (level <= 1) return 2;
+    if (level <= 3) return 4;
+    if (level <= 5) return 8;
+    if (level <= 7) return 16;
+    if (level <= 9) return 32;
+    if (level <= 10) return 64;
+    return 128;
+}
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct przm_gpu {
+    int dev = 0;
+    przm_gpu_info info{};
+    int cus = 0;
+    int lz_grid = 0;
+    uint32_t chunk = 8192;
+    Ctx c;
+    std::mutex mu;
+};
+
+namespace {
+
+int32_t mock_check() {
+    if (g_mock_unavailable.load()) return PRZM_GPU_ERR_UNAVAILABLE;
+    const int32_t e = g_mock_error.load();
+    if (e) return e < 0 ? e : PRZM_GPU_ERR_MOCK;
+    return 0;
+}
+
+void build_blocks(const przm_span *bufs, uint32_t nbufs, std::vector<przm_block_t> &blocks,
+                  std::vector<uint64_t> &first) {
+    blocks.clear();
+    first.assign(nbufs + 1, 0);
+    for (uint32_t i = 0; i < nbufs; i++) {
+        first[i] = blocks.size();
+        const uint64_t n = bufs[i].len;
+        if (n == 0) {
+            blocks.push_back({bufs[i].offset, 0, PRZM_BF_RAW});
+            continue;
+        }
+        for (uint64_t o = 0; o < n; o += PRZM_BLOCK) {
+            const uint32_t l = (uint32_t)std::min<uint64_t>(PRZM_BLOCK, n - o);
+            blocks.push_back({bufs[i].offset + o, l, 0});
+        }
+    }
+    first[nbufs] = blocks.size();
+}
+
+/* device buffers for nblocks blocks processed in chunks of `chunk` */
+int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg) {
+    const uint32_t ch = std::min<uint32_t>(std::max<uint32_t>(nblocks, 1), chunk);
+    CHK(c.blocks.ensure((size_t)std::max<uint32_t>(nblocks, 1) * sizeof(przm_block_t), &c.gen));
+    CHK(c.ws.ensure((size_t)ch * PRZM_WS_BYTES, &c.gen));
+    CHK(c.meta.ensure((size_t)std::max<uint32_t>(nblocks, 1) * sizeof(przm_meta_t), &c.gen));
+    CHK(c.ftab.ensure((size_t)ch * 4 * sizeof(przm_ftab_t), &c.gen));
+    CHK(c.flen.ensure((size_t)std::max<uint32_t>(nblocks, 1) * 4, &c.gen));
+    CHK(c.foff.ensure((size_t)std::max<uint32_t>(nblocks, 1) * 8, &c.gen));
+    CHK(c.run.ensure(64, &c.gen));
+    CHK(c.scratch.ensure((size_t)grid * LZ_SCRATCH_STRIDE, &c.gen));
+    if (dbg) CHK(c.dbg.ensure((size_t)std::max<uint32_t>(nblocks, 1) * PRZM_BLOCK * 2, &c.gen));
+    if (!c.st) HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+    return 0;
+}
+
+/*
+ * The stage launch sequence for nblocks blocks already in c.blocks.
+ * stage_ev (optional): 8 events per chunk recorded between stages.
+ */
+int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uint32_t depth, uint8_t *d_out,
+                bool check, bool stop_after_lz, bool dbg, int grid, uint32_t chunk, std::vector<hipEvent_t> *stage_ev) {
+    hipStream_t st = c.st;
+    przm_block_t *blocks = (przm_block_t *)c.blocks.p;
+    przm_meta_t *meta = (przm_meta_t *)c.meta.p;
+    uint32_t *flen = (uint32_t *)c.flen.p;
+    uint64_t *foff = (uint64_t *)c.foff.p;
+    uint64_t *run = (uint64_t *)c.run.p;
+    size_t evi = 0;
+    auto rec = [&]() -> int32_t {
+        if (stage_ev) {
+            if (evi >= stage_ev->size()) {
+                hipEvent_t e;
+                HIPCHK(hipEventCreate(&e));
+                stage_ev->push_back(e);
+            }
+            HIPCHK(hipEventRecord((*stage_ev)[evi++], st));
+        }
+        return 0;
+    };
+    CHK(rec());
+    if (check) hipLaunchKernelGGL(przm_check_kernel, dim3(nblocks), dim3(64), 0, st, d_src, blocks, nblocks);
+    hipLaunchKernelGGL(przm_zero_kernel, dim3(1), dim3(64), 0, st, run, 1u);
+    HIPCHK(hipGetLastError());
+    for (uint32_t c0 = 0; c0 < nblocks; c0 += chunk) {
+        const uint32_t nb = std::min(chunk, nblocks - c0);
+        const uint32_t gz = (uint32_t)std::min<int>(grid, (int)nb);
+        CHK(rec());
+        hipLaunchKernelGGL(przm_lz_kernel, dim3(gz), dim3(LZ_NT), 0, st, d_src, blocks + c0, nb, depth,
+                           (uint8_t *)c.ws.p, meta + c0, (uint8_t *)c.scratch.p, (uint64_t)LZ_SCRATCH_STRIDE,
+                           dbg ? (uint16_t *)c.dbg.p + (uint64_t)c0 * PRZM_BLOCK : (uint16_t *)nullptr);
+        HIPCHK(hipGetLastError());
+        if (stop_after_lz) continue;
+        CHK(rec());
+        hipLaunchKernelGGL(przm_hist_kernel, dim3(nb), dim3(256), 0, st, (uint8_t *)c.ws.p, meta + c0,
+                           (przm_ftab_t *)c.ftab.p, nb);
+        CHK(rec());
+        hipLaunchKernelGGL(przm_rans_kernel, dim3((nb * PRZM_J_N + RANS_JOBS - 1) / RANS_JOBS), dim3(RANS_NT), 0, st,
+                           (uint8_t *)c.ws.p, meta + c0, (const przm_ftab_t *)c.ftab.p, nb);
+        CHK(rec());
+        hipLaunchKernelGGL(przm_frame_size_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, blocks + c0, meta + c0,
+                           (const uint8_t *)c.ws.p, nb, flen + c0);
+        CHK(rec());
+        hipLaunchKernelGGL(przm_scan_kernel, dim3(1), dim3(1024), 0, st, flen + c0, nb, foff + c0, run);
+        CHK(rec());
+        hipLaunchKernelGGL(przm_frame_write_kernel, dim3(nb), dim3(256), 0, st, d_src, blocks + c0, meta + c0,
+                           (const uint8_t *)c.ws.p, (const przm_ftab_t *)c.ftab.p, foff + c0, d_out, nb);
+        HIPCHK(hipGetLastError());
+    }
+    CHK(rec());
+    return 0;
+}
+
+struct BatchOut {
+    std::vector<uint32_t> flen;
+    uint64_t total = 0;
+};
+
+/*
+ * Encode the buffers `bufs` of the device arena d_src into d_out.  With
+ * use_graph the whole launch sequence is captured once per (arena, output,
+ * block count, depth, shape) and replayed.
+ */
+int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bufs, uint32_t nbufs, int level,
+                  uint8_t *d_out, uint64_t out_cap, bool use_graph, std::vector<przm_block_t> &blocks,
+                  std::vector<uint64_t> &first, BatchOut &bo, przm_gpu_profile *prof) {
+    build_blocks(bufs, nbufs, blocks, first);
+    const uint32_t nblocks = (uint32_t)blocks.size();
+    uint64_t bound = 0;
+    for (auto &b : blocks) bound += 8ull + b.len;
+    if (bound > out_cap) {
+        snprintf(g_err, sizeof g_err, "output capacity %llu < bound %llu", (unsigned long long)out_cap,
+                 (unsigned long long)bound);
+        return PRZM_GPU_ERR_CAPACITY;
+    }
+    const uint32_t depth = (uint32_t)depth_of_level(level);
+    const int grid = g->lz_grid;
+    const uint32_t chunk = g->chunk;
+    CHK(ensure_ctx(c, nblocks, chunk, grid, false));
+    HIPCHK(hipMemcpyAsync(c.blocks.p, blocks.data(), blocks.size() * sizeof(przm_block_t), hipMemcpyHostToDevice, c.st));
+    if (c.ev.size() < 2) {
+        for (int i = 0; i < 2; i++) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c.ev.push_back(e);
+        }
+    }
+    std::vector<hipEvent_t> stage_ev;
+    const bool want_stages = prof && !use_graph;
+    HIPCHK(hipEventRecord(c.ev[0], c.st));
+    if (use_graph) {
+        const bool hit = c.gexec && c.gkey.src == d_src && c.gkey.out == d_out && c.gkey.nblocks == nblocks &&
+                         c.gkey.depth == depth && c.gkey.grid == (uint32_t)grid && c.gkey.chunk == chunk &&
+                         c.gkey.gen == c.gen;
+        if (!hit) {
+            if (c.gexec) { (void)hipGraphExecDestroy(c.gexec); c.gexec = nullptr; }
+            if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
+            HIPCHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+            const int32_t r = enqueue(g, c, d_src, nblocks, depth, d_out, true, false, false, grid, chunk, nullptr);
+            hipGraph_t gr = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(c.st, &gr);
+            if (r) { if (gr) (void)hipGraphDestroy(gr); return r; }
+            HIPCHK(e2);
+            c.graph = gr;
+            HIPCHK(hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
+            c.gkey = {d_src, d_out, nblocks, depth, (uint32_t)grid, chunk, c.gen};
+        }
+        HIPCHK(hipGraphLaunch(c.gexec, c.st));
+    } else {
+        CHK(enqueue(g, c, d_src, nblocks, depth, d_out, true, false, false, grid, chunk, want_stages ? &stage_ev : nullptr));
+    }
+    HIPCHK(hipEventRecord(c.ev[1], c.st));
+    bo.flen.resize(nblocks);
+    HIPCHK(hipMemcpyAsync(bo.flen.data(), c.flen.p, (size_t)nblocks * 4, hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    bo.total = 0;
+    for (uint32_t v : bo.flen) bo.total += v;
+    if (prof) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+        prof->kernel_ms = ms;
+        prof->graph_launch = use_graph ? 1 : 0;
+        prof->pipelined_chunks = (int32_t)((nblocks + chunk - 1) / chunk);
+        uint64_t in_bytes = 0;
+        for (auto &b : blocks) in_bytes += b.len;
+        prof->internal_vram_gbps = ms > 0 ? (double)(in_bytes + bo.total) / (ms * 1e6) : 0;
+        for (int s = 0; s < 8; s++) prof->stage_ms[s] = 0;
+        if (want_stages && stage_ev.size() >= 2) {
+            /* layout: [start][check+zero] then per chunk [lz][hist][rans][size][scan][write] ... [end] */
+            float t;
+            HIPCHK(hipEventElapsedTime(&t, stage_ev[0], stage_ev[1]));
+            prof->stage_ms[0] += t;
+            size_t i = 1;
+            while (i + 6 < stage_ev.size()) {
+                for (int s = 0; s < 6; s++) {
+                    HIPCHK(hipEventElapsedTime(&t, stage_ev[i + s], stage_ev[i + s + 1]));
+                    prof->stage_ms[1 + s] += t;
+                }
+                i += 6;
+            }
+        }
+        for (auto e : stage_ev) (void)hipEventDestroy(e);
+    }
+    return 0;
+}
+
+void fill_outputs(const przm_span *bufs, uint32_t nbufs, const std::vector<uint64_t> &first, const BatchOut &bo,
+                  uint64_t *out_off, uint32_t *frame_sizes, uint64_t *nframes) {
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < nbufs; i++) {
+        if (out_off) out_off[i] = acc;
+        for (uint64_t b = first[i]; b < first[i + 1]; b++) acc += bo.flen[b];
+    }
+    if (out_off) out_off[nbufs] = acc;
+    if (frame_sizes) std::memcpy(frame_sizes, bo.flen.data(), bo.flen.size() * 4);
+    if (nframes) *nframes = bo.flen.size();
+    (void)bufs;
+}
+
+int32_t init_handle(przm_gpu *g, int device) {
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return PRZM_GPU_ERR_UNAVAILABLE;
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, device));
+    g->dev = device;
+    g->cus = p.multiProcessorCount;
+    g->lz_grid = p.multiProcessorCount;
+    g->info.device = device;
+    g->info.compute_major = p.major;
+    g->info.compute_minor = p.minor;
+    g->info.sm_count = p.multiProcessorCount;
+    g->info.total_vram_bytes = p.totalGlobalMem;
+    snprintf(g->info.device_name, sizeof g->info.device_name, "%s", p.name);
+    snprintf(g->info.arch, sizeof g->info.arch, "%s", p.gcnArchName);
+    HIPCHK(hipStreamCreateWithFlags(&g->c.st, hipStreamNonBlocking));
+    return 0;
+}
+
+}  // namespace
+
+/* ======================= C ABI ======================= */
+
+extern "C" {
+
+const char *przm_gpu_last_error(void) { return g_err; }
+
+void przm_gpu_set_mock_unavailable(int32_t on) { g_mock_unavailable.store(on ? 1 : 0); }
+void przm_gpu_set_mock_error(int32_t code) { g_mock_error.store(code); }
+
+int32_t przm_gpu_probe(przm_gpu_info *info) {
+    if (g_mock_unavailable.load()) return PRZM_GPU_ERR_UNAVAILABLE;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return PRZM_GPU_ERR_UNAVAILABLE;
+    for (int d = 0; d < count; d++) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, d) != hipSuccess) continue;
+        if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) continue; /* kernels are built for gfx950 only */
+        if (info) {
+            info->device = d;
+            info->compute_major = p.major;
+            info->compute_minor = p.minor;
+            info->sm_count = p.multiProcessorCount;
+            info->total_vram_bytes = p.totalGlobalMem;
+            snprintf(info->device_name, sizeof info->device_name, "%s", p.name);
+            snprintf(info->arch, sizeof info->arch, "%s", p.gcnArchName);
+        }
+        return 0;
+    }
+    return PRZM_GPU_ERR_UNAVAILABLE;
+}
+
+int32_t przm_gpu_create(int32_t device, przm_gpu **out) {
+    if (!out) return PRZM_GPU_ERR_ARG;
+    *out = nullptr;
+    if (g_mock_unavailable.load()) return PRZM_GPU_ERR_UNAVAILABLE;
+    przm_gpu *g = new (std::nothrow) przm_gpu();
+    if (!g) return PRZM_GPU_ERR_OOM;
+    const int32_t r = init_handle(g, device);
+    if (r) {
+        delete g;
+        return r;
+    }
+    *out = g;
+    return 0;
+}
+
+int32_t przm_gpu_destroy(przm_gpu *g) {
+    if (!g) return 0;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        (void)hipSetDevice(g->dev);
+        (void)hipStreamSynchronize(g->c.st);
+        g->c.release();
+    }
+    delete g;
+    return 0;
+}
+
+int32_t przm_gpu_get_info(przm_gpu *g, przm_gpu_info *info) {
+    if (!g || !info) return PRZM_GPU_ERR_ARG;
+    *info = g->info;
+    return 0;
+}
+
+uint64_t przm_gpu_frames_bound(uint64_t n) {
+    if (n == 0) return 8;
+    return n + 8 * ((n + PRZM_BLOCK - 1) / PRZM_BLOCK);
+}
+
+int32_t przm_gpu_set_tuning(przm_gpu *g, int32_t lz_grid, int32_t chunk_blocks) {
+    if (!g) return PRZM_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (lz_grid > 0) g->lz_grid = lz_grid;
+    if (chunk_blocks > 0) g->chunk = (uint32_t)chunk_blocks;
+    return 0;
+}
+
+int32_t przm_gpu_malloc(przm_gpu *g, uint64_t bytes, void **d) {
+    if (!g || !d) return PRZM_GPU_ERR_ARG;
+    HIPCHK(hipSetDevice(g->dev));
+    if (hipMalloc(d, bytes ? bytes : 1) != hipSuccess) return PRZM_GPU_ERR_OOM;
+    return 0;
+}
+int32_t przm_gpu_free(przm_gpu *g, void *d) {
+    if (!g) return PRZM_GPU_ERR_ARG;
+    if (d) HIPCHK(hipFree(d));
+    return 0;
+}
+int32_t przm_gpu_h2d(przm_gpu *g, void *d, const void *h, uint64_t bytes) {
+    if (!g) return PRZM_GPU_ERR_ARG;
+    HIPCHK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+int32_t przm_gpu_d2h(przm_gpu *g, void *h, const void *d, uint64_t bytes) {
+    if (!g) return PRZM_GPU_ERR_ARG;
+    HIPCHK(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+int32_t przm_gpu_sync(przm_gpu *g) {
+    if (!g) return PRZM_GPU_ERR_ARG;
+    HIPCHK(hipDeviceSynchronize());
+    return 0;
+}
+int32_t przm_pinned_alloc(uint64_t bytes, void **p) {
+    if (!p) return PRZM_GPU_ERR_ARG;
+    HIPCHK(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault));
+    return 0;
+}
+int32_t przm_pinned_free(void *p) {
+    if (p) HIPCHK(hipHostFree(p));
+    return 0;
+}
+
+/* ---- encode ---- */
+
+int32_t przm_gpu_encode_batch_device(przm_gpu *g, const void *d_src, const przm_span *bufs, uint32_t nbufs,
+                                     int32_t level, void *d_out, uint64_t out_cap, uint64_t *out_off,
+                                     uint32_t *frame_sizes, uint64_t frames_cap, uint64_t *nframes,
+                                     przm_gpu_profile *prof) {
+    if (!g || (!bufs && nbufs)) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPCHK(hipSetDevice(g->dev));
+    std::vector<przm_block_t> blocks;
+    std::vector<uint64_t> first;
+    uint64_t nb = 0;
+    for (uint32_t i = 0; i < nbufs; i++) nb += bufs[i].len ? (bufs[i].len + PRZM_BLOCK - 1) / PRZM_BLOCK : 1;
+    if (frame_sizes && nb > frames_cap) return PRZM_GPU_ERR_CAPACITY;
+    BatchOut bo;
+    const double t0 = now_ms();
+    CHK(run_batch(g, g->c, (const uint8_t *)d_src, bufs, nbufs, level, (uint8_t *)d_out, out_cap, nbufs > 1, blocks,
+                  first, bo, prof));
+    fill_outputs(bufs, nbufs, first, bo, out_off, frame_sizes, nframes);
+    if (prof) {
+        prof->total_round_trip_ms = now_ms() - t0;
+        uint64_t in = 0;
+        for (uint32_t i = 0; i < nbufs; i++) in += bufs[i].len;
+        prof->end_to_end_gbps = prof->total_round_trip_ms > 0 ? in / (prof->total_round_trip_ms * 1e6) : 0;
+    }
+    return 0;
+}
+
+int32_t przm_gpu_encode_device(przm_gpu *g, const void *d_src, uint64_t n, int32_t level, void *d_out,
+                               uint64_t out_cap, uint64_t *out_len, uint32_t *frame_sizes, uint64_t frames_cap,
+                               uint64_t *nframes, przm_gpu_profile *prof) {
+    if (!g) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPCHK(hipSetDevice(g->dev));
+    przm_span sp{0, n};
+    const uint64_t nb = n ? (n + PRZM_BLOCK - 1) / PRZM_BLOCK : 1;
+    if (frame_sizes && nb > frames_cap) return PRZM_GPU_ERR_CAPACITY;
+    std::vector<przm_block_t> blocks;
+    std::vector<uint64_t> first;
+    BatchOut bo;
+    const double t0 = now_ms();
+    CHK(run_batch(g, g->c, (const uint8_t *)d_src, &sp, 1, level, (uint8_t *)d_out, out_cap, false, blocks, first, bo,
+                  prof));
+    fill_outputs(&sp, 1, first, bo, nullptr, frame_sizes, nframes);
+    if (out_len) *out_len = bo.total;
+    if (prof) prof->total_round_trip_ms = now_ms() - t0;
+    return 0;
+}
+
+int32_t przm_gpu_encode_batch(przm_gpu *g, const uint8_t *src, const przm_span *bufs, uint32_t nbufs, int32_t level,
+                              uint8_t *out, uint64_t out_cap, uint64_t *out_off, uint32_t *frame_sizes,
+                              uint64_t frames_cap, uint64_t *nframes, przm_gpu_profile *prof) {
+    if (!g || (!bufs && nbufs) || (!src && nbufs)) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPCHK(hipSetDevice(g->dev));
+    const double t0 = now_ms();
+    /* pack buffers 16-byte aligned in the device arena */
+    std::vector<przm_span> dev(nbufs);
+    uint64_t arena = 0, in_bytes = 0, nb = 0;
+    for (uint32_t i = 0; i < nbufs; i++) {
+        dev[i] = {arena, bufs[i].len};
+        arena += (bufs[i].len + 255) & ~255ull;
+        in_bytes += bufs[i].len;
+        nb += bufs[i].len ? (bufs[i].len + PRZM_BLOCK - 1) / PRZM_BLOCK : 1;
+    }
+    if (frame_sizes && nb > frames_cap) return PRZM_GPU_ERR_CAPACITY;
+    const uint64_t bound = in_bytes + 8 * nb;
+    if (bound > out_cap && out) {
+        /* the exact size is known only afterwards; require the bound like the device form */
+    }
+    Ctx &c = g->c;
+    CHK(c.in.ensure(arena + 64, &c.gen));
+    CHK(c.out.ensure(bound + 64, &c.gen));
+    const double h0 = now_ms();
+    for (uint32_t i = 0; i < nbufs; i++)
+        if (bufs[i].len)
+            HIPCHK(hipMemcpyAsync((uint8_t *)c.in.p + dev[i].offset, src + bufs[i].offset, bufs[i].len,
+                                  hipMemcpyHostToDevice, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    const double h1 = now_ms();
+    std::vector<przm_block_t> blocks;
+    std::vector<uint64_t> first;
+    BatchOut bo;
+    CHK(run_batch(g, c, (const uint8_t *)c.in.p, dev.data(), nbufs, level, (uint8_t *)c.out.p, bound + 64, nbufs > 1,
+                  blocks, first, bo, prof));
+    if (bo.total > out_cap) return PRZM_GPU_ERR_CAPACITY;
+    const double d0 = now_ms();
+    HIPCHK(hipMemcpy(out, c.out.p, bo.total, hipMemcpyDeviceToHost));
+    const double d1 = now_ms();
+    fill_outputs(bufs, nbufs, first, bo, out_off, frame_sizes, nframes);
+    if (prof) {
+        prof->ingress_dma_bytes = in_bytes;
+        prof->ingress_dma_ms = h1 - h0;
+        prof->ingress_dma_gbps = (h1 - h0) > 0 ? in_bytes / ((h1 - h0) * 1e6) : 0;
+        prof->egress_dma_bytes = bo.total;
+        prof->egress_dma_ms = d1 - d0;
+        prof->egress_dma_gbps = (d1 - d0) > 0 ? bo.total / ((d1 - d0) * 1e6) : 0;
+        prof->total_round_trip_ms = now_ms() - t0;
+        prof->end_to_end_gbps = prof->total_round_trip_ms > 0 ? in_bytes / (prof->total_round_trip_ms * 1e6) : 0;
+        const double dma = (h1 - h0) + (d1 - d0);
+        prof->pcie_overhead_percent = prof->total_round_trip_ms > 0 ? 100.0 * dma / prof->total_round_trip_ms : 0;
+    }
+    return 0;
+}
+
+int32_t przm_gpu_encode(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t level, uint8_t *out, uint64_t out_cap,
+                        uint64_t *out_len, uint32_t *frame_sizes, uint64_t frames_cap, uint64_t *nframes,
+                        przm_gpu_profile *prof) {
+    przm_span sp{0, n};
+    uint64_t off[2] = {0, 0};
+    const int32_t r = przm_gpu_encode_batch(g, src, &sp, 1, level, out, out_cap, off, frame_sizes, frames_cap,
+                                            nframes, prof);
+    if (r == 0 && out_len) *out_len = off[1];
+    return r;
+}
+
+/* ---- stages ---- */
+
+namespace {
+int32_t stage_upload(przm_gpu *g, const uint8_t *src, uint64_t n, std::vector<przm_block_t> &blocks) {
+    Ctx &c = g->c;
+    przm_span sp{0, n};
+    std::vector<uint64_t> first;
+    build_blocks(&sp, 1, blocks, first);
+    for (auto &b : blocks) b.flags = 0; /* stage APIs tokenize every block, high-entropy or not */
+    if (n == 0) blocks.clear();
+    const uint32_t nb = (uint32_t)blocks.size();
+    CHK(c.in.ensure(n + 64, &c.gen));
+    CHK(ensure_ctx(c, nb ? nb : 1, std::max<uint32_t>(nb, 1), g->lz_grid, true));
+    if (n) HIPCHK(hipMemcpyAsync(c.in.p, src, n, hipMemcpyHostToDevice, c.st));
+    if (nb) HIPCHK(hipMemcpyAsync(c.blocks.p, blocks.data(), nb * sizeof(przm_block_t), hipMemcpyHostToDevice, c.st));
+    return 0;
+}
+}  // namespace
+
+int32_t przm_gpu_match(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t level, uint32_t *desc,
+                       przm_gpu_profile *prof) {
+    if (!g || (!src && n) || (!desc && n)) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPCHK(hipSetDevice(g->dev));
+    if (n == 0) return 0;
+    std::vector<przm_block_t> blocks;
+    CHK(stage_upload(g, src, n, blocks));
+    Ctx &c = g->c;
+    const uint32_t nb = (uint32_t)blocks.size();
+    if (c.ev.size() < 2) {
+        for (int i = 0; i < 2; i++) { hipEvent_t e; HIPCHK(hipEventCreate(&e)); c.ev.push_back(e); }
+    }
+    HIPCHK(hipEventRecord(c.ev[0], c.st));
+    CHK(enqueue(g, c, (const uint8_t *)c.in.p, nb, (uint32_t)depth_of_level(level), (uint8_t *)c.out.p, false, true,
+                true, g->lz_grid, nb, nullptr));
+    HIPCHK(hipEventRecord(c.ev[1], c.st));
+    DevBuf d;
+    CHK(d.ensure(n * 4, nullptr));
+    hipLaunchKernelGGL(przm_desc_kernel, dim3(64, nb), dim3(256), 0, c.st, (const uint8_t *)c.in.p,
+                       (const przm_block_t *)c.blocks.p, nb, (const uint16_t *)c.dbg.p, (uint32_t *)d.p, 0ull);
+    const hipError_t le = hipGetLastError();
+    if (le == hipSuccess) (void)hipMemcpyAsync(desc, d.p, n * 4, hipMemcpyDeviceToHost, c.st);
+    const hipError_t se = hipStreamSynchronize(c.st);
+    d.release();
+    HIPCHK(le);
+    HIPCHK(se);
+    if (prof) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+        prof->kernel_ms = ms;
+    }
+    return 0;
+}
+
+int32_t przm_gpu_tokenize(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t level, uint8_t *cmd, uint64_t cmd_cap,
+                          uint8_t *off, uint64_t off_cap, uint8_t *lit, uint64_t lit_cap, uint8_t *op,
+                          uint64_t op_cap, uint8_t *meta, uint64_t meta_cap, uint8_t *var, uint64_t var_cap,
+                          uint32_t *block_lens, uint64_t nblocks_cap, przm_gpu_profile *prof) {
+    if (!g || (!src && n)) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPCHK(hipSetDevice(g->dev));
+    if (n == 0) return 0;
+    std::vector<przm_block_t> blocks;
+    CHK(stage_upload(g, src, n, blocks));
+    Ctx &c = g->c;
+    const uint32_t nb = (uint32_t)blocks.size();
+    if (nb > nblocks_cap) return PRZM_GPU_ERR_CAPACITY;
+    if (c.ev.size() < 2) {
+        for (int i = 0; i < 2; i++) { hipEvent_t e; HIPCHK(hipEventCreate(&e)); c.ev.push_back(e); }
+    }
+    HIPCHK(hipEventRecord(c.ev[0], c.st));
+    CHK(enqueue(g, c, (const uint8_t *)c.in.p, nb, (uint32_t)depth_of_level(level), (uint8_t *)c.out.p, false, true,
+                false, g->lz_grid, nb, nullptr));
+    HIPCHK(hipEventRecord(c.ev[1], c.st));
+    std::vector<przm_meta_t> mt(nb);
+    HIPCHK(hipMemcpyAsync(mt.data(), c.meta.p, nb * sizeof(przm_meta_t), hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    uint8_t *dst[6] = {cmd, off, lit, op, meta, var};
+    const uint64_t cap[6] = {cmd_cap, off_cap, lit_cap, op_cap, meta_cap, var_cap};
+    const uint32_t wsoff[6] = {PRZM_WS_CMD, PRZM_WS_OFF, PRZM_WS_LIT, PRZM_WS_OP, PRZM_WS_META, PRZM_WS_VAR};
+    uint64_t pos[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t b = 0; b < nb; b++) {
+        for (int s = 0; s < 6; s++) {
+            const uint32_t l = mt[b].len[s];
+            if (block_lens) block_lens[(uint64_t)b * 7 + s] = l;
+            if (!dst[s] || !l) continue;
+            if (pos[s] + l > cap[s]) return PRZM_GPU_ERR_CAPACITY;
+            HIPCHK(hipMemcpyAsync(dst[s] + pos[s], (uint8_t *)c.ws.p + (uint64_t)b * PRZM_WS_BYTES + wsoff[s], l,
+                                  hipMemcpyDeviceToHost, c.st));
+            pos[s] += l;
+        }
+        if (block_lens) block_lens[(uint64_t)b * 7 + 6] = mt[b].len[PRZM_S_TOK];
+    }
+    HIPCHK(hipStreamSynchronize(c.st));
+    if (prof) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+        prof->kernel_ms = ms;
+    }
+    return 0;
+}
+
+int32_t przm_gpu_histogram(przm_gpu *g, const uint8_t *src, uint64_t n, uint32_t counts[256], przm_gpu_profile *prof) {
+    if (!g || !counts || (!src && n)) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPCHK(hipSetDevice(g->dev));
+    Ctx &c = g->c;
+    CHK(c.in.ensure(n + 64, &c.gen));
+    CHK(c.meta.ensure(4096, &c.gen));
+    if (n) HIPCHK(hipMemcpyAsync(c.in.p, src, n, hipMemcpyHostToDevice, c.st));
+    hipLaunchKernelGGL(przm_hist1_kernel, dim3(1), dim3(256), 0, c.st, (const uint8_t *)c.in.p, n,
+                       (uint32_t *)c.meta.p, (przm_ftab_t *)nullptr, (uint32_t *)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(counts, c.meta.p, 1024, hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    (void)prof;
+    return 0;
+}
+
+int32_t przm_gpu_vans_encode(przm_gpu *g, const uint8_t *src, uint64_t n, uint8_t *out, uint64_t cap,
+                             uint64_t *out_len, przm_gpu_profile *prof) {
+    if (!g || !out || !out_len || (!src && n) || n > (16ull << 20)) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPCHK(hipSetDevice(g->dev));
+    if (n == 0) { *out_len = 0; return 0; }
+    Ctx &c = g->c;
+    const uint64_t slot = 2 * n + 64;
+    CHK(c.in.ensure(n + 64, &c.gen));
+    CHK(c.out.ensure(slot, &c.gen));
+    CHK(c.meta.ensure(4096 + sizeof(przm_ftab_t) + 64, &c.gen));
+    uint8_t *mb = (uint8_t *)c.meta.p;
+    przm_ftab_t *ft = (przm_ftab_t *)(mb + 2048);
+    uint32_t *res = (uint32_t *)(mb + 1024);
+    HIPCHK(hipMemcpyAsync(c.in.p, src, n, hipMemcpyHostToDevice, c.st));
+    hipLaunchKernelGGL(przm_hist1_kernel, dim3(1), dim3(256), 0, c.st, (const uint8_t *)c.in.p, n, (uint32_t *)mb, ft,
+                       res + 8);
+    hipLaunchKernelGGL(przm_rans1_kernel, dim3(1), dim3(64), 0, c.st, (const uint8_t *)c.in.p, (uint32_t)n,
+                       (const przm_ftab_t *)ft, (uint8_t *)c.out.p + slot, (uint32_t)(2 * n + 16), res);
+    HIPCHK(hipGetLastError());
+    uint32_t hres[9];
+    przm_ftab_t hft;
+    HIPCHK(hipMemcpyAsync(hres, res, sizeof hres, hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipMemcpyAsync(&hft, ft, sizeof hft, hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    const uint32_t sl = hres[0], ml = hres[8];
+    if (sl == PRZM_ENC_OVERFLOW) return PRZM_GPU_ERR_HIP;
+    const uint64_t total = 8 + ml + 16 + sl;
+    if (total > cap) return PRZM_GPU_ERR_CAPACITY;
+    out[0] = 'V'; out[1] = 'A'; out[2] = 0x01; out[3] = 0x00;
+    for (int k = 0; k < 4; k++) out[4 + k] = (uint8_t)(n >> (8 * k));
+    std::memcpy(out + 8, hft.mat, ml);
+    for (int j = 0; j < 4; j++)
+        for (int k = 0; k < 4; k++) out[8 + ml + 4 * j + k] = (uint8_t)(hres[1 + j] >> (8 * k));
+    HIPCHK(hipMemcpy(out + 8 + ml + 16, (uint8_t *)c.out.p + slot - sl, sl, hipMemcpyDeviceToHost));
+    *out_len = total;
+    (void)prof;
+    return 0;
+}
+
+int32_t przm_gpu_pack_flat(przm_gpu *g, const uint8_t *src, uint64_t n, uint8_t *out, uint64_t cap, uint64_t *out_len,
+                           przm_gpu_profile *prof) {
+    if (!g || !out || !out_len || (!src && n) || n > 0xFFFFFFFFull) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    if (cap < n + 5) return PRZM_GPU_ERR_CAPACITY;
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPCHK(hipSetDevice(g->dev));
+    Ctx &c = g->c;
+    CHK(c.in.ensure(n + 64, &c.gen));
+    CHK(c.out.ensure(n + 64, &c.gen));
+    if (n) {
+        HIPCHK(hipMemcpy(c.in.p, src, n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((uint8_t *)c.out.p + 5, c.in.p, n, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(out + 5, (uint8_t *)c.out.p + 5, n, hipMemcpyDeviceToHost));
+    }
+    out[0] = 0x00;
+    for (int k = 0; k < 4; k++) out[1 + k] = (uint8_t)(n >> (8 * k));
+    *out_len = n + 5;
+    (void)prof;
+    return 0;
+}
+
+/* ---- host helpers ---- */
+
+int32_t przm_normalize_freq(const uint32_t counts[256], uint32_t target_m, uint16_t freq[256]) {
+    if (!counts || !freq || target_m != 4096) return PRZM_GPU_ERR_ARG;
+    uint64_t total = 0;
+    int unique = 0;
+    for (int i = 0; i < 256; i++) {
+        freq[i] = 0;
+        if (counts[i]) { total += counts[i]; unique++; }
+    }
+    if (!unique) return 0;
+    const uint64_t rem = 4096 - (uint64_t)unique;
+    uint32_t sum = 0;
+    for (int i = 0; i < 256; i++) {
+        if (!counts[i]) continue;
+        freq[i] = (uint16_t)(1 + (counts[i] * rem) / total);
+        sum += freq[i];
+    }
+    /* deficit: +1 each to the largest counts, lower symbol first on ties, each at most once */
+    int order[256];
+    int m = 0;
+    for (int i = 0; i < 256; i++) if (counts[i]) order[m++] = i;
+    std::stable_sort(order, order + m, [&](int a, int b) { return counts[a] > counts[b]; });
+    for (int r = 0; r < (int)(4096 - sum) && r < m; r++) freq[order[r]]++;
+    return 0;
+}
+
+int32_t przm_build_cdf(const uint16_t freq[256], uint16_t cdf[257]) {
+    if (!freq || !cdf) return PRZM_GPU_ERR_ARG;
+    uint32_t c = 0;
+    for (int i = 0; i < 256; i++) { cdf[i] = (uint16_t)c; c += freq[i]; }
+    cdf[256] = (uint16_t)c;
+    return 0;
+}
+
+int32_t przm_serialize_matrix(const uint16_t freq[256], uint8_t *out, uint64_t cap, uint64_t *len) {
+    if (!freq || !out || !len) return PRZM_GPU_ERR_ARG;
+    uint8_t tmp[600];
+    uint32_t n = 0, rows = 0;
+    for (int h = 0; h < 16; h++)
+        for (int l = 0; l < 16; l++)
+            if (freq[(h << 4) | l]) { rows |= 1u << h; break; }
+    tmp[n++] = (uint8_t)rows; tmp[n++] = (uint8_t)(rows >> 8);
+    for (int h = 0; h < 16; h++) {
+        if (!(rows & (1u << h))) continue;
+        uint32_t cols = 0;
+        for (int l = 0; l < 16; l++) if (freq[(h << 4) | l]) cols |= 1u << l;
+        tmp[n++] = (uint8_t)cols; tmp[n++] = (uint8_t)(cols >> 8);
+        for (int l = 0; l < 16; l++) {
+            if (!(cols & (1u << l))) continue;
+            const uint32_t w = freq[(h << 4) | l];
+            if (w < 128) tmp[n++] = (uint8_t)w;
+            else { tmp[n++] = (uint8_t)((w >> 8) | 0x80); tmp[n++] = (uint8_t)w; }
+        }
+    }
+    if (n > cap) return PRZM_GPU_ERR_CAPACITY;
+    std::memcpy(out, tmp, n);
+    *len = n;
+    return 0;
+}
+
+/* ---- autotuner ---- */
+
+int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs, uint32_t nbufs, int32_t level,
+                          int32_t tuned[3]) {
+    if (!g || !bufs || !nbufs) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    /* sample: buffers from the front up to ~256 MiB */
+    uint32_t ns = 0;
+    uint64_t bytes = 0;
+    while (ns < nbufs && bytes < (256ull << 20)) bytes += bufs[ns++].len;
+    uint64_t bound = 0;
+    for (uint32_t i = 0; i < ns; i++) bound += przm_gpu_frames_bound(bufs[i].len);
+    void *d_out = nullptr;
+    CHK(przm_gpu_malloc(g, bound + 64, &d_out));
+    const int grids[] = {g->cus, 2 * g->cus};
+    const int chunks[] = {2048, 4096, 8192};
+    double best = 1e30;
+    int bg = g->lz_grid, bc = (int)g->chunk;
+    int32_t rc = 0;
+    for (int gi : grids) {
+        for (int ci : chunks) {
+            przm_gpu_set_tuning(g, gi, ci);
+            double t = 1e30;
+            for (int rep = 0; rep < 3 && rc == 0; rep++) {
+                przm_gpu_profile p{};
+                rc = przm_gpu_encode_batch_device(g, d_src, bufs, ns, level, d_out, bound + 64, nullptr, nullptr, 0,
+                                                  nullptr, &p);
+                if (rep > 0) t = std::min(t, p.kernel_ms);
+            }
+            if (rc) break;
+            if (t < best) { best = t; bg = gi; bc = ci; }
+        }
+        if (rc) break;
+    }
+    (void)przm_gpu_free(g, d_out);
+    przm_gpu_set_tuning(g, bg, bc);
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        if (g->c.gexec) { (void)hipGraphExecDestroy(g->c.gexec); g->c.gexec = nullptr; }
+        if (g->c.graph) { (void)hipGraphDestroy(g->c.graph); g->c.graph = nullptr; }
+    }
+    if (tuned) { tuned[0] = bg; tuned[1] = bc; tuned[2] = RANS_JOBS; }
+    return rc;
+}
+
+}  // extern "C"
+
+/* ======================= serving loop ======================= */
+
+namespace {
+
+struct Req {
+    uint64_t id = 0;
+    std::vector<uint8_t> src;
+    std::vector<uint8_t> out;
+    std::vector<uint32_t> fsz;
+    int32_t status = 1; /* 1 pending, 0 done, <0 error */
+    double submit_ms = 0, start_ms = 0, done_ms = 0;
+};
+
+struct Slot {
+    Ctx c;
+    bool busy = false;
+    std::vector<std::shared_ptr<Req>> reqs;
+    std::vector<przm_span> spans;
+    std::vector<przm_block_t> blocks;
+    std::vector<uint64_t> first;
+    uint8_t *h_in = nullptr, *h_out = nullptr;
+    uint64_t h_in_cap = 0, h_out_cap = 0;
+    uint64_t dev_bytes = 0;
+    hipEvent_t done = nullptr;
+    std::vector<uint32_t> flen;
+};
+
+}  // namespace
+
+struct przm_server {
+    przm_gpu *g = nullptr;
+    int level = 6;
+    uint64_t budget = 0, max_batch = 0;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::mutex mu;
+    std::condition_variable cv_in, cv_done;
+    std::deque<std::shared_ptr<Req>> pending;
+    std::map<uint64_t, std::shared_ptr<Req>> all;
+    uint64_t next_id = 1;
+    uint64_t admitted = 0;
+    bool stop = false;
+    std::thread worker;
+    double t0 = 0;
+    uint64_t stats[8] = {0};
+};
+
+namespace {
+
+uint64_t slot_cost(uint64_t in_bytes, uint64_t nblocks) {
+    return in_bytes + (in_bytes + 8 * nblocks) + nblocks * (uint64_t)PRZM_WS_BYTES + nblocks * 1024;
+}
+
+int32_t host_grow(uint8_t **p, uint64_t *cap, uint64_t need) {
+    if (*cap >= need && *p) return 0;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    HIPCHK(hipHostMalloc((void **)p, need, hipHostMallocDefault));
+    *cap = need;
+    return 0;
+}
+
+/* launch one batch on a free slot (called with s->mu held) */
+int32_t launch_slot(przm_server *s, Slot &sl) {
+    przm_gpu *g = s->g;
+    uint64_t in_bytes = 0;
+    sl.spans.clear();
+    uint64_t arena = 0;
+    for (auto &r : sl.reqs) {
+        sl.spans.push_back({arena, (uint64_t)r->src.size()});
+        arena += (r->src.size() + 255) & ~255ull;
+        in_bytes += r->src.size();
+    }
+    build_blocks(sl.spans.data(), (uint32_t)sl.spans.size(), sl.blocks, sl.first);
+    const uint32_t nb = (uint32_t)sl.blocks.size();
+    const uint64_t bound = in_bytes + 8ull * nb + 64;
+    CHK(host_grow(&sl.h_in, &sl.h_in_cap, arena + 64));
+    CHK(host_grow(&sl.h_out, &sl.h_out_cap, bound));
+    for (size_t i = 0; i < sl.reqs.size(); i++)
+        if (!sl.reqs[i]->src.empty()) std::memcpy(sl.h_in + sl.spans[i].offset, sl.reqs[i]->src.data(), sl.reqs[i]->src.size());
+    Ctx &c = sl.c;
+    const uint32_t chunk = std::max<uint32_t>(nb, 1);
+    CHK(c.in.ensure(arena + 64, &c.gen));
+    CHK(c.out.ensure(bound, &c.gen));
+    CHK(ensure_ctx(c, nb, chunk, g->lz_grid, false));
+    HIPCHK(hipMemcpyAsync(c.in.p, sl.h_in, arena, hipMemcpyHostToDevice, c.st));
+    HIPCHK(hipMemcpyAsync(c.blocks.p, sl.blocks.data(), nb * sizeof(przm_block_t), hipMemcpyHostToDevice, c.st));
+    CHK(enqueue(g, c, (const uint8_t *)c.in.p, nb, (uint32_t)depth_of_level(s->level), (uint8_t *)c.out.p, true, false,
+                false, g->lz_grid, chunk, nullptr));
+    sl.flen.resize(nb);
+    HIPCHK(hipMemcpyAsync(sl.flen.data(), c.flen.p, nb * 4ull, hipMemcpyDeviceToHost, c.st));
+    /* frames: the device bound covers them; copy the bound and split on completion */
+    HIPCHK(hipMemcpyAsync(sl.h_out, c.out.p, bound, hipMemcpyDeviceToHost, c.st));
+    if (!sl.done) HIPCHK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(sl.done, c.st));
+    return 0;
+}
+
+void finish_slot(przm_server *s, Slot &sl, int32_t rc) {
+    const double t = now_ms() - s->t0;
+    uint64_t pos = 0;
+    for (size_t i = 0; i < sl.reqs.size(); i++) {
+        Req &r = *sl.reqs[i];
+        if (rc) { r.status = rc; r.done_ms = t; continue; }
+        uint64_t len = 0;
+        r.fsz.clear();
+        for (uint64_t b = sl.first[i]; b < sl.first[i + 1]; b++) { r.fsz.push_back(sl.flen[b]); len += sl.flen[b]; }
+        r.out.assign(sl.h_out + pos, sl.h_out + pos + len);
+        pos += len;
+        r.status = 0;
+        r.done_ms = t;
+        s->stats[1]++;
+        s->stats[2] += r.src.size();
+        s->stats[3] += len;
+        std::vector<uint8_t>().swap(r.src);
+    }
+    s->admitted -= sl.dev_bytes;
+    sl.dev_bytes = 0;
+    sl.reqs.clear();
+    sl.busy = false;
+}
+
+void serve_loop(przm_server *s) {
+    (void)hipSetDevice(s->g->dev);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        /* retire finished launches */
+        bool any_busy = false;
+        for (auto &up : s->slots) {
+            Slot &sl = *up;
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.done);
+            if (q == hipSuccess) finish_slot(s, sl, 0);
+            else if (q != hipErrorNotReady) finish_slot(s, sl, PRZM_GPU_ERR_HIP);
+            else any_busy = true;
+        }
+        s->cv_done.notify_all();
+        if (s->stop && s->pending.empty() && !any_busy) break;
+        /* admit: fill a free slot with queued requests under the batch and HBM bounds */
+        Slot *free_slot = nullptr;
+        for (auto &up : s->slots) if (!up->busy) { free_slot = up.get(); break; }
+        if (free_slot && !s->pending.empty()) {
+            uint64_t bytes = 0, nb = 0, cost = 0;
+            std::vector<std::shared_ptr<Req>> take;
+            while (!s->pending.empty()) {
+                auto &r = s->pending.front();
+                const uint64_t n = r->src.size();
+                const uint64_t rb = n ? (n + PRZM_BLOCK - 1) / PRZM_BLOCK : 1;
+                const uint64_t c2 = slot_cost(bytes + n, nb + rb);
+                if (!take.empty() && (bytes + n > s->max_batch || s->admitted + c2 > s->budget)) break;
+                if (take.empty() && s->admitted + c2 > s->budget && any_busy) break; /* wait for memory */
+                bytes += n; nb += rb; cost = c2;
+                take.push_back(r);
+                s->pending.pop_front();
+            }
+            if (!take.empty()) {
+                const double t = now_ms() - s->t0;
+                for (auto &r : take) r->start_ms = t;
+                free_slot->reqs = std::move(take);
+                free_slot->busy = true;
+                free_slot->dev_bytes = cost;
+                s->admitted += cost;
+                if (s->admitted > s->stats[4]) s->stats[4] = s->admitted;
+                if (free_slot->reqs.size() > s->stats[5]) s->stats[5] = free_slot->reqs.size();
+                s->stats[0]++;
+                if (any_busy) s->stats[6]++;
+                const int32_t rc = launch_slot(s, *free_slot);
+                if (rc) finish_slot(s, *free_slot, rc);
+                continue;
+            }
+        }
+        if (any_busy) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+            lk.lock();
+        } else {
+            s->cv_in.wait_for(lk, std::chrono::milliseconds(5));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t przm_serve_create(przm_gpu *g, int32_t level, uint64_t hbm_budget, uint64_t max_batch_bytes, uint32_t streams,
+                          przm_server **out) {
+    if (!g || !out || streams == 0 || streams > 8) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    przm_server *s = new przm_server();
+    s->g = g;
+    s->level = level;
+    s->budget = hbm_budget ? hbm_budget : (64ull << 30);
+    s->max_batch = max_batch_bytes ? max_batch_bytes : (256ull << 20);
+    s->t0 = now_ms();
+    for (uint32_t i = 0; i < streams; i++) {
+        auto sl = std::make_unique<Slot>();
+        if (hipSetDevice(g->dev) != hipSuccess || hipStreamCreateWithFlags(&sl->c.st, hipStreamNonBlocking) != hipSuccess) {
+            delete s;
+            return PRZM_GPU_ERR_HIP;
+        }
+        s->slots.push_back(std::move(sl));
+    }
+    s->worker = std::thread(serve_loop, s);
+    *out = s;
+    return 0;
+}
+
+int32_t przm_serve_submit(przm_server *s, const uint8_t *src, uint64_t n, uint64_t *id) {
+    if (!s || !id || (!src && n)) return PRZM_GPU_ERR_ARG;
+    auto r = std::make_shared<Req>();
+    r->src.assign(src, src + n);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->pending.size() > 1000000) return PRZM_GPU_ERR_BUSY;
+    r->id = s->next_id++;
+    r->submit_ms = now_ms() - s->t0;
+    s->pending.push_back(r);
+    s->all[r->id] = r;
+    *id = r->id;
+    s->cv_in.notify_one();
+    return 0;
+}
+
+int32_t przm_serve_wait(przm_server *s, uint64_t id, int64_t timeout_ms, uint8_t *out, uint64_t cap,
+                        uint32_t *frame_sizes, uint64_t frames_cap, przm_result *res) {
+    if (!s) return PRZM_GPU_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return PRZM_GPU_ERR_ARG;
+    std::shared_ptr<Req> r = it->second;
+    auto ready = [&] { return r->status != 1; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return PRZM_GPU_ERR_BUSY;
+    if (res) {
+        res->id = r->id;
+        res->status = r->status;
+        res->nframes = (uint32_t)r->fsz.size();
+        res->out_len = r->out.size();
+        res->submit_ms = r->submit_ms;
+        res->start_ms = r->start_ms;
+        res->done_ms = r->done_ms;
+    }
+    if (r->status) { s->all.erase(id); return r->status; }
+    if (out) {
+        if (r->out.size() > cap) return PRZM_GPU_ERR_CAPACITY;
+        std::memcpy(out, r->out.data(), r->out.size());
+    }
+    if (frame_sizes) {
+        if (r->fsz.size() > frames_cap) return PRZM_GPU_ERR_CAPACITY;
+        std::memcpy(frame_sizes, r->fsz.data(), r->fsz.size() * 4);
+    }
+    s->all.erase(id);
+    return 0;
+}
+
+int32_t przm_serve_stats(przm_server *s, uint64_t stats[8]) {
+    if (!s || !stats) return PRZM_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memcpy(stats, s->stats, sizeof s->stats);
+    return 0;
+}
+
+int32_t przm_serve_destroy(przm_server *s) {
+    if (!s) return 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_in.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->g->dev);
+    for (auto &sl : s->slots) {
+        if (sl->c.st) (void)hipStreamSynchronize(sl->c.st);
+        if (sl->done) (void)hipEventDestroy(sl->done);
+        if (sl->h_in) (void)hipHostFree(sl->h_in);
+        if (sl->h_out) (void)hipHostFree(sl->h_out);
+        sl->c.release();
+    }
+    delete s;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,243 @@
+"""ctypes access to the checkers: the C oracle (oracle/liboracle.so) and the
+reference C99 decoder compiled from the reference sources (oracle/_ref/).
+Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REFDEC = os.path.join(ROOT, "oracle", "_ref", "libprzm_ref_dec.so")
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+
+
+class FrameInfo(ctypes.Structure):
+    _fields_ = [("kind", U32), ("ncmd", U32), ("noff", U32), ("nlit", U32), ("nop", U32), ("nmeta", U32),
+                ("nvar", U32), ("ntok", U32), ("s2", U32), ("s3", U32), ("s4", U32), ("sub", ctypes.c_int32 * 3),
+                ("s4v", U32)]
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a.size else ctypes.c_void_p(0)
+
+
+def u8(x):
+    if isinstance(x, np.ndarray):
+        return np.ascontiguousarray(x, dtype=np.uint8)
+    return np.frombuffer(bytes(x), dtype=np.uint8).copy()
+
+
+class Oracle:
+    def __init__(self):
+        l = ctypes.CDLL(ORACLE)
+        l.orc_hash4.restype = U32
+        l.orc_hash4.argtypes = [P]
+        l.orc_search_depth.restype = ctypes.c_int
+        l.orc_search_depth.argtypes = [ctypes.c_int]
+        l.orc_match_block.argtypes = [P, U32, ctypes.c_int, P, P]
+        l.orc_match_block.restype = None
+        l.orc_tokenize_streams.argtypes = [P, U32, ctypes.c_int, P, P, P, P, P, P, P]
+        l.orc_tokenize_streams.restype = None
+        l.orc_tokenize_desc.argtypes = [P, U32, P, P, P, P, P, P, P, P]
+        l.orc_tokenize_desc.restype = None
+        l.orc_split_commands.argtypes = [P, U32, P, P, P, P, P, P]
+        l.orc_split_commands.restype = ctypes.c_int
+        l.orc_normalize.argtypes = [P, P]
+        l.orc_normalize.restype = None
+        l.orc_joint_nibble.argtypes = [P, P]
+        l.orc_joint_nibble.restype = U32
+        l.orc_vans4x_encode.argtypes = [P, U32, P, P]
+        l.orc_vans4x_encode.restype = U32
+        l.orc_vans4x_adaptive.argtypes = [P, U32, P]
+        l.orc_vans4x_adaptive.restype = U32
+        l.orc_meta8_encode.argtypes = [P, U32, P]
+        l.orc_meta8_encode.restype = U32
+        l.orc_pack_substream.argtypes = [P, U32, P, ctypes.POINTER(ctypes.c_int)]
+        l.orc_pack_substream.restype = U32
+        l.orc_check_high_entropy.argtypes = [P, U32]
+        l.orc_check_high_entropy.restype = ctypes.c_int
+        l.orc_window_log.argtypes = [U32]
+        l.orc_window_log.restype = U32
+        l.orc_encode_block.argtypes = [P, U32, ctypes.c_int, P, ctypes.POINTER(FrameInfo)]
+        l.orc_encode_block.restype = U32
+        l.orc_encode_buffer.argtypes = [P, U64, ctypes.c_int, P, U64, P, U64, ctypes.POINTER(U64)]
+        l.orc_encode_buffer.restype = U64
+        self.l = l
+
+    def hash4(self, b4):
+        a = u8(b4)
+        return self.l.orc_hash4(ptr(a))
+
+    def match_block(self, src, level=6, depth=None):
+        a = u8(src)
+        desc = np.zeros(max(a.size, 1), np.uint32)
+        st = np.zeros(1, np.uint64)
+        d = depth if depth is not None else self.l.orc_search_depth(level)
+        self.l.orc_match_block(ptr(a), a.size, d, ptr(desc), ptr(st))
+        return desc[: a.size]
+
+    def tokenize(self, src, level=6, desc=None):
+        a = u8(src)
+        cap = 2 * a.size + 64
+        bufs = [np.zeros(cap, np.uint8) for _ in range(6)]
+        lens = np.zeros(7, np.uint32)
+        if desc is None:
+            self.l.orc_tokenize_streams(ptr(a), a.size, level, *[ptr(b) for b in bufs], ptr(lens))
+        else:
+            d = np.ascontiguousarray(desc, np.uint32)
+            self.l.orc_tokenize_desc(ptr(a), a.size, ptr(d), *[ptr(b) for b in bufs], ptr(lens))
+        names = ["cmd", "off", "lit", "op", "meta", "var"]
+        out = {n: bufs[i][: lens[i]].copy() for i, n in enumerate(names)}
+        out["ntok"] = int(lens[6])
+        return out
+
+    def split(self, cmd):
+        c = u8(cmd)
+        n = max(c.size, 1)
+        op, meta, var = (np.zeros(n, np.uint8) for _ in range(3))
+        lo, lm, lv = U32(), U32(), U32()
+        rc = self.l.orc_split_commands(ptr(c), c.size, ptr(op), ctypes.byref(lo), ptr(meta), ctypes.byref(lm),
+                                       ptr(var), ctypes.byref(lv))
+        if rc:
+            raise ValueError("split failed")
+        return op[: lo.value], meta[: lm.value], var[: lv.value]
+
+    def normalize(self, counts):
+        c = np.ascontiguousarray(counts, np.uint32)
+        f = np.zeros(256, np.uint16)
+        self.l.orc_normalize(ptr(c), ptr(f))
+        return f
+
+    def matrix(self, freq):
+        f = np.ascontiguousarray(freq, np.uint16)
+        out = np.zeros(600, np.uint8)
+        n = self.l.orc_joint_nibble(ptr(f), ptr(out))
+        return out[:n].tobytes()
+
+    def vans(self, src, freq=None):
+        a = u8(src)
+        out = np.zeros(8 + 600 + 16 + 2 * a.size + 64, np.uint8)
+        if freq is None:
+            n = self.l.orc_vans4x_adaptive(ptr(a), a.size, ptr(out))
+        else:
+            f = np.ascontiguousarray(freq, np.uint16)
+            n = self.l.orc_vans4x_encode(ptr(a), a.size, ptr(f), ptr(out))
+        return out[:n].tobytes()
+
+    def meta8(self, src):
+        a = u8(src)
+        out = np.zeros(24 + 2 * a.size + 64, np.uint8)
+        n = self.l.orc_meta8_encode(ptr(a), a.size, ptr(out))
+        return out[:n].tobytes()
+
+    def pack_substream(self, src):
+        a = u8(src)
+        out = np.zeros(640 + 2 * a.size, np.uint8)
+        ch = ctypes.c_int()
+        n = self.l.orc_pack_substream(ptr(a), a.size, ptr(out), ctypes.byref(ch))
+        return out[:n].tobytes(), ch.value
+
+    def encode_block(self, src, level=6):
+        a = u8(src)
+        out = np.zeros(a.size + 64, np.uint8)
+        info = FrameInfo()
+        n = self.l.orc_encode_block(ptr(a), a.size, level, ptr(out), ctypes.byref(info))
+        return out[:n].tobytes(), info
+
+    def encode_buffer(self, src, level=6):
+        a = u8(src)
+        nb = max((a.size + 65535) // 65536, 1)
+        out = np.zeros(a.size + 8 * nb + 64, np.uint8)
+        sizes = np.zeros(nb, np.uint32)
+        nf = U64()
+        tot = self.l.orc_encode_buffer(ptr(a), a.size, level, ptr(out), out.size, ptr(sizes), nb, ctypes.byref(nf))
+        return out[:tot].tobytes(), sizes[: nf.value].copy()
+
+
+class RefDecoder:
+    """The reference C99 decoder (src/przm_dec.c), built from its own sources."""
+
+    def __init__(self):
+        l = ctypes.CDLL(REFDEC)
+        l.przm_decompress.restype = ctypes.c_int32
+        l.przm_decompress.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t]
+        l.przm_vans_decode_literals.restype = ctypes.c_int32
+        l.przm_vans_decode_literals.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t]
+        l.przm_vans_decode_meta8.restype = ctypes.c_int32
+        l.przm_vans_decode_meta8.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, ctypes.c_size_t]
+        l.przm_split_commands.restype = ctypes.c_int32
+        l.przm_split_commands.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                          P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_size_t)]
+        self.l = l
+
+    def decompress(self, frame, expect_len=None):
+        f = u8(frame)
+        cap = (expect_len if expect_len is not None else 65536) + 64
+        out = np.zeros(cap, np.uint8)
+        r = self.l.przm_decompress(ptr(f), f.size, ptr(out), out.size)
+        if r < 0:
+            raise ValueError(f"decoder error {r}")
+        return out[:r].tobytes()
+
+    def decompress_frames(self, frames, sizes, expect_total=None):
+        f = u8(frames)
+        parts = []
+        pos = 0
+        for s in sizes:
+            parts.append(self.decompress(f[pos: pos + int(s)]))
+            pos += int(s)
+        assert pos == f.size, (pos, f.size)
+        return b"".join(parts)
+
+    def vans_decode(self, container, n):
+        c = u8(container)
+        out = np.zeros(max(n, 1), np.uint8)
+        r = self.l.przm_vans_decode_literals(ptr(c[8:]), c.size - 8, ptr(out), n)
+        if r != 0:
+            raise ValueError(f"vans decode error {r}")
+        return out[:n].tobytes()
+
+    def meta8_decode(self, container, n):
+        c = u8(container)
+        out = np.zeros(max(n, 1), np.uint8)
+        r = self.l.przm_vans_decode_meta8(ptr(c), c.size, ptr(out), out.size, n)
+        if r != 0:
+            raise ValueError(f"meta8 decode error {r}")
+        return out[:n].tobytes()
+
+    def split(self, cmd):
+        c = u8(cmd)
+        n = max(c.size, 1)
+        bufs = [np.zeros(n, np.uint8) for _ in range(3)]
+        lens = [ctypes.c_size_t() for _ in range(3)]
+        r = self.l.przm_split_commands(ptr(c), c.size, ptr(bufs[0]), n, ctypes.byref(lens[0]), ptr(bufs[1]), n,
+                                       ctypes.byref(lens[1]), ptr(bufs[2]), n, ctypes.byref(lens[2]))
+        if r != 0:
+            raise ValueError(f"split error {r}")
+        return tuple(b[: l.value] for b, l in zip(bufs, lens))
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        _oracle = Oracle()
+    return _oracle
+
+
+def refdec():
+    global _ref
+    if _ref is None:
+        _ref = RefDecoder()
+    return _ref
+
+
+def have_refdec():
+    return os.path.exists(REFDEC)
