@@ -64,6 +64,12 @@ typedef struct {
     int32_t pipelined_chunks;
     int32_t graph_launch;
     double stage_ms[8]; /* check, lz, hist, rans, frame-size, scan, frame-write, spare */
+    /* filled by profiled (non-graph) runs: bytes of the blocks the LZ kernel
+     * tokenized, token-stream bytes it wrote, frame bytes, uncompressed frames */
+    uint64_t lz_in_bytes;
+    uint64_t lz_stream_bytes;
+    uint64_t frame_bytes;
+    uint64_t raw_frames;
 } przm_gpu_profile;
 
 /* A buffer inside a device or host arena. */
@@ -144,6 +150,8 @@ uint64_t przm_gpu_frames_bound(uint64_t n);
 int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs, uint32_t nbufs, int32_t level,
                           int32_t tuned[3]);
 int32_t przm_gpu_set_tuning(przm_gpu *g, int32_t lz_grid, int32_t chunk_blocks);
+/* use_graph = 0 runs the batched form as direct launches (per-stage profiling). */
+int32_t przm_gpu_set_options(przm_gpu *g, int32_t use_graph);
 
 /* ---- device memory for callers without an allocator ---- */
 int32_t przm_gpu_malloc(przm_gpu *g, uint64_t bytes, void **d);

@@ -39,7 +39,8 @@ class GpuProfile(ctypes.Structure):
                 ("egress_dma_ms", ctypes.c_double), ("egress_dma_gbps", ctypes.c_double),
                 ("total_round_trip_ms", ctypes.c_double), ("end_to_end_gbps", ctypes.c_double),
                 ("pcie_overhead_percent", ctypes.c_double), ("pipelined_chunks", I32), ("graph_launch", I32),
-                ("stage_ms", ctypes.c_double * 8)]
+                ("stage_ms", ctypes.c_double * 8), ("lz_in_bytes", U64), ("lz_stream_bytes", U64),
+                ("frame_bytes", U64), ("raw_frames", U64)]
 
 
 class Span(ctypes.Structure):
@@ -74,6 +75,7 @@ SIGNATURES = {
     "przm_gpu_frames_bound": (U64, [U64]),
     "przm_gpu_autotune": (I32, [P, P, ctypes.POINTER(Span), U32, I32, ctypes.POINTER(I32)]),
     "przm_gpu_set_tuning": (I32, [P, I32, I32]),
+    "przm_gpu_set_options": (I32, [P, I32]),
     "przm_gpu_malloc": (I32, [P, U64, ctypes.POINTER(P)]),
     "przm_gpu_free": (I32, [P, P]),
     "przm_gpu_h2d": (I32, [P, P, P, U64]),

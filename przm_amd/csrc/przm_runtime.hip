@@ -146,6 +146,7 @@ struct przm_gpu {
     int cus = 0;
     int lz_grid = 0;
     uint32_t chunk = 8192;
+    bool use_graph = true;
     Ctx c;
     std::mutex mu;
 };
@@ -340,6 +341,20 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
             }
         }
         for (auto e : stage_ev) (void)hipEventDestroy(e);
+        prof->frame_bytes = bo.total;
+        if (want_stages) {
+            std::vector<przm_meta_t> mt(nblocks);
+            HIPCHK(hipMemcpy(mt.data(), c.meta.p, (size_t)nblocks * sizeof(przm_meta_t), hipMemcpyDeviceToHost));
+            uint64_t lin = 0, lst = 0, raw = 0;
+            for (uint32_t b = 0; b < nblocks; b++) {
+                if (mt[b].frame_kind != 0) raw++;
+                const uint32_t s6 = mt[b].len[0] + mt[b].len[1] + mt[b].len[2] + mt[b].len[3] + mt[b].len[4] + mt[b].len[5];
+                if (s6) { lin += blocks[b].len; lst += s6; }
+            }
+            prof->lz_in_bytes = lin;
+            prof->lz_stream_bytes = lst;
+            prof->raw_frames = raw;
+        }
     }
     return 0;
 }
@@ -457,6 +472,13 @@ int32_t przm_gpu_set_tuning(przm_gpu *g, int32_t lz_grid, int32_t chunk_blocks) 
     return 0;
 }
 
+int32_t przm_gpu_set_options(przm_gpu *g, int32_t use_graph) {
+    if (!g) return PRZM_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->use_graph = use_graph != 0;
+    return 0;
+}
+
 int32_t przm_gpu_malloc(przm_gpu *g, uint64_t bytes, void **d) {
     if (!g || !d) return PRZM_GPU_ERR_ARG;
     HIPCHK(hipSetDevice(g->dev));
@@ -510,8 +532,8 @@ int32_t przm_gpu_encode_batch_device(przm_gpu *g, const void *d_src, const przm_
     if (frame_sizes && nb > frames_cap) return PRZM_GPU_ERR_CAPACITY;
     BatchOut bo;
     const double t0 = now_ms();
-    CHK(run_batch(g, g->c, (const uint8_t *)d_src, bufs, nbufs, level, (uint8_t *)d_out, out_cap, nbufs > 1, blocks,
-                  first, bo, prof));
+    CHK(run_batch(g, g->c, (const uint8_t *)d_src, bufs, nbufs, level, (uint8_t *)d_out, out_cap,
+                  g->use_graph && nbufs > 1, blocks, first, bo, prof));
     fill_outputs(bufs, nbufs, first, bo, out_off, frame_sizes, nframes);
     if (prof) {
         prof->total_round_trip_ms = now_ms() - t0;
@@ -547,7 +569,9 @@ int32_t przm_gpu_encode_device(przm_gpu *g, const void *d_src, uint64_t n, int32
 int32_t przm_gpu_encode_batch(przm_gpu *g, const uint8_t *src, const przm_span *bufs, uint32_t nbufs, int32_t level,
                               uint8_t *out, uint64_t out_cap, uint64_t *out_off, uint32_t *frame_sizes,
                               uint64_t frames_cap, uint64_t *nframes, przm_gpu_profile *prof) {
-    if (!g || (!bufs && nbufs) || (!src && nbufs)) return PRZM_GPU_ERR_ARG;
+    if (!g || (!bufs && nbufs)) return PRZM_GPU_ERR_ARG;
+    for (uint32_t i = 0; i < nbufs; i++)
+        if (bufs[i].len && !src) return PRZM_GPU_ERR_ARG;
     CHK(mock_check());
     std::lock_guard<std::mutex> lk(g->mu);
     HIPCHK(hipSetDevice(g->dev));
