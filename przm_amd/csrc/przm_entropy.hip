@@ -206,7 +206,7 @@ przm_hist1_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__restr
 
 /* ---------------- rANS ---------------- */
 
-#define RANS_NT 256
+#define RANS_NT 64
 #define RANS_JOBS (RANS_NT / 4)
 
 __device__ __forceinline__ uint32_t ctx_meta(uint32_t p) { /* vans_meta.go:45-68 */
@@ -256,6 +256,24 @@ __constant__ M8Table przm_m8 = M8Table();
  * slot_end; at most cap bytes.  Returns the byte count (or overflow) and the
  * lane's final state.
  */
+/* value of lane sel[j] of the quad for quad lane j; ctrl = sel0 | sel1 << 2 | sel2 << 4 | sel3 << 6 */
+#define QPERM(v, ctrl) ((uint32_t)__builtin_amdgcn_mov_dpp((int)(v), (ctrl), 0xF, 0xF, false))
+
+__device__ __forceinline__ uint4 ld16(const uint8_t *p, int64_t c) {
+    return c >= 0 ? *(const uint4 *)(p + 16 * c) : make_uint4(0, 0, 0, 0);
+}
+__device__ __forceinline__ uint32_t u4get(const uint4 &v, uint32_t k) {
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
+
+/*
+ * The reverse 4-state pass of one stream by one lane quad (lane j = state j):
+ * literals.go:305-325 / vans_meta.go:109-132.  Symbols arrive 16 at a time
+ * (one 16-byte load per four steps, the next one in flight); src must be
+ * 16-byte aligned and readable 16 bytes past n.  Emitted bytes go backward
+ * from slot_end; at most cap bytes.  Returns the byte count (or overflow)
+ * and the lane's final state.
+ */
 __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, uint32_t n, bool m8,
                                               const uint32_t *vt, const uint32_t (*m8t)[256],
                                               uint8_t *__restrict__ slot_end, uint32_t cap, uint32_t *state) {
@@ -263,15 +281,29 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
     const uint32_t groups = (n + 3) >> 2;
     uint32_t x = 65536u, K = 0;
     bool over = false;
+    int64_t c = groups ? (int64_t)((groups - 1) >> 2) : -1;
+    uint4 cur = ld16(src, c), nxt = ld16(src, c - 1);
     for (uint32_t g = groups; g-- > 0;) {
+        if ((int64_t)(g >> 2) != c) { c--; cur = nxt; nxt = ld16(src, c - 1); }
         const uint32_t i = 4 * g + j;
         const bool v = i < n;
-        uint32_t f = 1, c = 0;
+        uint32_t f = 1, cdf = 0;
         if (v) {
-            const uint32_t s = src[i];
-            const uint32_t e = m8 ? m8t[i > 0 ? ctx_meta(src[i - 1]) : 0u][s] : vt[s];
+            const uint32_t dw = u4get(cur, g & 3u);
+            const uint32_t sym = (dw >> (8 * j)) & 0xFFu;
+            uint32_t e;
+            if (m8) {
+                uint32_t prev = 0;
+                bool has = i > 0;
+                if (j > 0) prev = (dw >> (8 * (j - 1))) & 0xFFu;
+                else if ((g & 3u) > 0) prev = u4get(cur, (g & 3u) - 1) >> 24;
+                else prev = nxt.w >> 24;
+                e = m8t[has ? ctx_meta(prev) : 0u][sym];
+            } else {
+                e = vt[sym];
+            }
             f = e & 0xFFFFu;
-            c = e >> 16;
+            cdf = e >> 16;
         }
         const uint32_t t = 4096u * f - 1u;
         uint32_t ne = 0, b0 = 0, b1 = 0;
@@ -279,10 +311,11 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
             b0 = x & 0xFFu; x >>= 8; ne = 1;
             if (x > t) { b1 = x & 0xFFu; x >>= 8; ne = 2; }
         }
-        /* bytes of higher lanes come first: lane 3 emits before lane 2, ... */
-        const uint32_t e1 = __shfl_down(ne, 1, 4), e2 = __shfl_down(ne, 2, 4), e3 = __shfl_down(ne, 3, 4);
+        /* bytes of higher lanes come first: lane 3 emits before lane 2, ...
+         * (quad permutes: lane j reads lanes j+1, j+2, j+3 and lane 0 of its quad) */
+        const uint32_t e1 = QPERM(ne, 0xF9), e2 = QPERM(ne, 0xFE), e3 = QPERM(ne, 0xFF);
         const uint32_t after = (j < 3 ? e1 : 0u) + (j < 2 ? e2 : 0u) + (j < 1 ? e3 : 0u);
-        const uint32_t tot = __shfl(ne + after, 0, 4);
+        const uint32_t tot = QPERM(ne + after, 0x00);
         if (K + tot > cap) { over = true; break; }
         if (ne) {
             const uint64_t k0 = K + after;
@@ -293,7 +326,7 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
         if (v) {
             uint32_t qq, rr;
             divmod(x, f, &qq, &rr);
-            x = qq * 4096u + c + rr;
+            x = qq * 4096u + cdf + rr;
         }
     }
     *state = x;

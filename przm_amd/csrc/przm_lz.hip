@@ -38,13 +38,24 @@
 
 #define LZ_NT 512
 #define LZ_NW 8
-#define SEG 64
-#define NSEG (PRZM_HALF / SEG)
-#define WS_CAP 2048
+#define QP 16384                  /* parse tile: a quarter block */
+#define SEG 32                    /* speculative-walk segment */
+#define NSEG (QP / SEG)           /* 512: one per thread */
+#define WS_CAP 2048               /* bucket elements kept in LDS per wave */
 #define NONE16 0xFFFFu
 #define NONE32 0xFFFFFFFFu
 #define XCROSS 0x80000000u
 #define LEN_CROSS 0xFFFFFFFFu
+#define NX_STOP 0xFFFEu
+#define NX_END 0xFFFFu
+/* padded LDS indices: one spare u16 / four spare bytes per 32 entries, so the
+ * lanes of a wave walking consecutive segments hit different banks */
+#define DIX(i) ((i) + ((i) >> 5))
+#define LIX(i) ((i) + (((i) >> 5) << 2))
+#define DSZ (DIX(QP + 1) + 2)
+#define LSZ (LIX(QP + 1) + 8)
+#define GWS_SLOT (2u * PRZM_BLOCK * 2u + 256u * 4u) /* one large bucket: positions, order, group ends */
+#define GWS_STRIDE (2u * GWS_SLOT)                   /* per wave: the queued bucket + one worked in place */
 
 enum { T_LIT = 0, T_RLE = 1, T_MATCH = 2 };
 
@@ -52,29 +63,43 @@ enum { T_LIT = 0, T_RLE = 1, T_MATCH = 2 };
 enum {
     ST_ENTRY = 0, ST_LITSTART, ST_R0, ST_R1, ST_R2,
     ST_CMD, ST_OFF, ST_LIT, ST_OP, ST_META, ST_VAR, ST_TOK,
-    ST_BUCKET, ST_DISTB, ST_CNT0, ST_CNT1, ST_NMATCH, ST_N
+    ST_BUCKET, ST_DISTB, ST_LB, ST_NMATCH, ST_N
 };
 
 struct LzShared {
     uint8_t in[PRZM_BLOCK + 64];
     union {
         uint16_t stage[PRZM_HALF];
-        uint16_t ws[LZ_NW][2][WS_CAP];
-        uint16_t dist[PRZM_HALF + 4];
+        struct {
+            uint16_t pos[LZ_NW][WS_CAP];
+            uint16_t srt[LZ_NW][WS_CAP];
+        } ws;
+        struct {
+            uint16_t dist[DSZ];
+            uint8_t lc[LSZ];
+            uint16_t tlen[NSEG][8]; /* cached non-literal tokens of each segment (swizzled) */
+            uint8_t toff[NSEG][8];  /* start offset in segment | kind << 5 */
+        } q;
     } big;
     union {
         uint32_t tab[LZ_NW][256];
         struct {
-            uint64_t bmp[NSEG];
+            uint32_t bmp[NSEG];
             uint32_t xpos[NSEG];
-            uint32_t tstart[NSEG];
             uint32_t xlen[NSEG];
-            uint16_t tdist[PRZM_HALF / 4];
+            uint32_t ent[NSEG];
+            uint16_t nxt[NSEG];
+            uint16_t jmp[NSEG];
+            uint16_t prv[NSEG];
+            uint8_t vis[NSEG];
+            uint8_t ntk[NSEG];
+            uint16_t tdist[QP / 4];
         } ps;
     } sm;
     uint32_t bstart[2][257];
     uint32_t wsum[LZ_NW];
     uint32_t st[ST_N + 3];
+    uint32_t bigq[LZ_NW];
 };
 
 static_assert(sizeof(LzShared) <= 163840, "LDS budget");
@@ -231,25 +256,40 @@ __device__ __forceinline__ uint32_t put_leb(uint8_t *b, uint32_t n, uint32_t v) 
 
 struct Tok { uint32_t kind, len; };
 
+__device__ __forceinline__ uint32_t dget(const LzShared &S, uint32_t i) { return S.big.q.dist[DIX(i)]; }
+__device__ __forceinline__ uint32_t lget(const LzShared &S, uint32_t i) { return S.big.q.lc[LIX(i)]; }
+
+/* exact descriptor length at p (tile index i) with distance d; lc keeps min(L, 255) */
+template <bool COOP>
+__device__ __forceinline__ uint32_t desc_len(const LzShared &S, uint32_t p, uint32_t i, uint32_t d, uint32_t n) {
+    const uint32_t l = lget(S, i);
+    if (l < 255) return l;
+    const uint32_t cap = n - p < PRZM_MAX_MATCH ? n - p : PRZM_MAX_MATCH;
+    return 255 + LCP<COOP>(S.in, p + 255, p - d + 255, cap - 255);
+}
+
 /*
- * The token the reference tokenizer emits at position p (p in [a, b) with
- * dist[] holding [a, b]), restated from przm_tokenizer.cuh:171-320 for a
- * walk that only needs exact token ends below `lim`.  A match / run whose
- * end is at or beyond lim comes back with len == LEN_CROSS.
+ * The token the reference tokenizer emits at position p of the tile [a, b)
+ * (dist / lc hold [a, b]), restated from przm_tokenizer.cuh:171-320.  The
+ * descriptor length L of every position comes from the search, so a match
+ * shorter than the 1024 cap has its exact length without any compare; only a
+ * capped match (extension) or a byte run may need compares.  Those whose end
+ * is at or beyond lim come back with len == LEN_CROSS.
  */
 template <bool COOP>
 __device__ Tok eval_tok(const LzShared &S, uint32_t p, uint32_t a, uint32_t n, uint32_t lim) {
     const uint8_t *in = S.in;
-    const uint32_t d = S.big.dist[p - a];
+    const uint32_t i = p - a;
+    const uint32_t d = dget(S, i);
     if (!d) return {T_LIT, 1};
-    const uint32_t cap = n - p < PRZM_MAX_MATCH ? n - p : PRZM_MAX_MATCH;
-    const uint32_t L = LCP<COOP>(in, p, p - d, cap); /* descriptor length (>= 4) */
+    const uint32_t L = desc_len<COOP>(S, p, i, d, n);
     if (p + 1 < n) { /* lookahead, :196-216 */
-        const uint32_t d1 = S.big.dist[p + 1 - a];
+        const uint32_t d1 = dget(S, i + 1);
         if (d1) {
-            uint32_t c1 = n - p - 1 < PRZM_MAX_MATCH ? n - p - 1 : PRZM_MAX_MATCH;
-            if (c1 > L + 2) c1 = L + 2;
-            if (LCP<COOP>(in, p + 1, p + 1 - d1, c1) > L + 1) return {T_LIT, 1};
+            const uint32_t l1c = lget(S, i + 1);
+            uint32_t l1 = l1c;
+            if (l1c == 255 && L + 1 >= 255) l1 = desc_len<COOP>(S, p + 1, i + 1, d1, n);
+            if (l1 > L + 1) return {T_LIT, 1};
         }
     }
     const uint32_t room = lim - p;
@@ -260,13 +300,14 @@ __device__ Tok eval_tok(const LzShared &S, uint32_t p, uint32_t a, uint32_t n, u
             const uint32_t maxr = n - p < PRZM_RLE_CAP ? n - p : PRZM_RLE_CAP;
             if (need <= maxr && RUN<COOP>(in, p, b0, need) >= need) {
                 const uint32_t c = maxr < room ? maxr : room;
-                const uint32_t r = RUN<COOP>(in, p, b0, c);
+                const uint32_t r = c <= need ? c : need + RUN<COOP>(in, p + need, b0, c - need);
                 if (r < c || maxr <= room) return {T_RLE, r};
                 return {T_RLE, LEN_CROSS};
             }
         }
     }
-    /* match: validated by construction (d <= p, L >= 4, 4 bytes equal); extend, :288-320 */
+    /* match (valid by construction); the extension only runs past a capped length, :288-320 */
+    if (L < PRZM_MAX_MATCH || L >= n - p) return {T_MATCH, L};
     const uint32_t capx = n - p < PRZM_EXT_CAP ? n - p : PRZM_EXT_CAP;
     const uint32_t c = capx < room ? capx : room;
     if (L >= c) {
@@ -304,67 +345,143 @@ __device__ int lru_index(const LzShared &S, uint32_t i, uint32_t d) {
     return -1;
 }
 
+/* Orders LDS accesses of one wave for the compiler only: the LDS executes a
+ * wave's DS instructions in issue order, so no hardware wait is needed.  It
+ * also marks the points where the code relies on the 64 lanes moving in
+ * lockstep (all lanes' reads of an instruction before any lane's later write). */
+__device__ __forceinline__ void lds_fence() {
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
 /* ---------------- phase C: one bucket ---------------- */
 
+struct Scr {
+    const uint16_t *T0, *T1;
+    uint32_t *R0, *R1;
+    uint16_t *L0, *L1;
+    uint8_t *gws; /* per-wave workspaces, GWS_STRIDE bytes each */
+};
+
+/* search elements [i0, i1) (sorted order) of a bucket whose position array is
+ * P, sorted permutation SRT and per-digit group ends GE */
+template <typename PT, typename ET>
+__device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, uint32_t dgt, uint32_t n,
+                                                    uint32_t depth, const PT *P, const PT *SRT, const ET *GE,
+                                                    uint32_t i0, uint32_t i1, uint32_t step) {
+    const uint8_t *in = S.in;
+    const uint32_t lane = __lane_id();
+    const uint32_t s0 = S.bstart[0][dgt], m0 = S.bstart[0][dgt + 1] - s0;
+    const uint32_t s1 = S.bstart[1][dgt];
+    for (uint32_t c0 = i0; c0 < i1; c0 += step) {
+        const uint32_t i = c0 + lane;
+        if (i >= i1) continue;
+        const uint32_t j = SRT[i];
+        const uint32_t p = P[j];
+        const uint32_t low = hash4(in, p) & 0xFFu;
+        const uint32_t gs = low ? GE[low - 1] : 0u;
+        const uint32_t cap = n - p < PRZM_MAX_MATCH ? n - p : PRZM_MAX_MATCH;
+        uint32_t best = 0, bd = 0;
+        uint32_t kmax = i - gs;
+        if (kmax > depth) kmax = depth;
+        uint32_t q = kmax ? (uint32_t)P[SRT[i - 1]] : 0u;
+        for (uint32_t k = 1; k <= kmax; k++) {
+            const uint32_t qn = k < kmax ? (uint32_t)P[SRT[i - k - 1]] : 0u; /* next candidate in flight */
+            if (in[p + best] == in[q + best]) {
+                const uint32_t l = lcp(in, p, q, cap);
+                if (l > best) { best = l; bd = p - q; }
+                if (best >= cap) break;
+            }
+            q = qn;
+        }
+        const bool m = best >= 4;
+        const uint32_t rv = p | ((m ? bd : 0u) << 16);
+        const uint16_t lv = (uint16_t)(m ? best : 0u);
+        if (j < m0) { sc.R0[s0 + j] = rv; sc.L0[s0 + j] = lv; }
+        else { sc.R1[s1 + j - m0] = rv; sc.L1[s1 + j - m0] = lv; }
+        if (p == PRZM_HALF) { S.st[ST_DISTB] = m ? bd : 0u; S.st[ST_LB] = m ? best : 0u; }
+    }
+}
+
+/* group the bucket by the hash's low byte (stable), then search it unless it
+ * is large; returns true when the bucket was queued for the all-wave search */
 template <bool LDSWS>
-__device__ void bucket_search(LzShared &S, uint32_t w, uint32_t dgt, uint32_t n, uint32_t depth,
-                              const uint16_t *__restrict__ T0, const uint16_t *__restrict__ T1,
-                              uint32_t *__restrict__ R0, uint32_t *__restrict__ R1, uint16_t *gwpos,
-                              uint16_t *gwprev) {
+__device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt, uint32_t n, uint32_t depth,
+                            bool may_queue) {
+    const uint8_t *in = S.in;
     const uint32_t lane = __lane_id();
     const uint64_t lt = (1ull << lane) - 1ull;
     const uint32_t s0 = S.bstart[0][dgt], m0 = S.bstart[0][dgt + 1] - s0;
     const uint32_t s1 = S.bstart[1][dgt], m1 = S.bstart[1][dgt + 1] - s1;
-    const uint32_t m = m0 + m1, tag = dgt + 1;
-    uint16_t *wpos = LDSWS ? S.big.ws[w][0] : gwpos;
-    uint16_t *wprev = LDSWS ? S.big.ws[w][1] : gwprev;
-    const uint8_t *in = S.in;
+    const uint32_t m = m0 + m1;
+    uint8_t *gw = sc.gws + (uint64_t)w * GWS_STRIDE + (may_queue ? 0u : GWS_SLOT);
+    uint16_t *P = LDSWS ? S.big.ws.pos[w] : (uint16_t *)gw;
+    uint16_t *SRT = LDSWS ? S.big.ws.srt[w] : (uint16_t *)gw + PRZM_BLOCK;
+    uint32_t *cnt = S.sm.tab[w];
+    for (int k = 0; k < 4; k++) cnt[lane * 4 + k] = 0;
+    lds_fence();
+    for (uint32_t c0 = 0; c0 < m; c0 += 256) {
+        uint32_t pv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t j = c0 + u * 64 + lane;
+            pv[u] = j < m ? (j < m0 ? sc.T0[s0 + j] : sc.T1[s1 + j - m0]) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t j = c0 + u * 64 + lane;
+            if (j < m) {
+                P[j] = (uint16_t)pv[u];
+                atomicAdd(&cnt[hash4(in, pv[u]) & 0xFFu], 1u);
+            }
+        }
+    }
+    if (!LDSWS) __threadfence_block();
+    lds_fence();
+    /* exclusive scan of the 256 digit counts (4 per lane) */
+    {
+        uint32_t c4[4], t = 0;
+        for (int k = 0; k < 4; k++) { c4[k] = cnt[lane * 4 + k]; t += c4[k]; }
+        uint32_t x = t;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        uint32_t run = x - t;
+        for (int k = 0; k < 4; k++) { cnt[lane * 4 + k] = run; run += c4[k]; }
+    }
+    lds_fence();
     for (uint32_t c0 = 0; c0 < m; c0 += 64) {
         const uint32_t j = c0 + lane;
         const bool valid = j < m;
-        uint32_t pos = 0;
-        if (valid) pos = j < m0 ? T0[s0 + j] : T1[s1 + j - m0];
-        const uint32_t low = valid ? (hash4(in, pos) & 0xFFu) : 0u;
+        const uint32_t low = valid ? (hash4(in, P[j]) & 0xFFu) : 0u;
         const uint64_t pm = peers8(low, valid);
-        uint32_t pred = NONE16;
+        const uint32_t base = valid ? cnt[low] : 0u;
+        lds_fence();
         if (valid) {
-            const uint64_t below = pm & lt;
-            if (below) pred = c0 + 63u - (uint32_t)__clzll((unsigned long long)below);
-            else {
-                const uint32_t hv = S.sm.tab[w][low];
-                if ((hv >> 16) == tag) pred = hv & 0xFFFFu;
-            }
-            wpos[j] = (uint16_t)pos;
-            wprev[j] = (uint16_t)pred;
-            if ((pm >> lane) == 1ull) S.sm.tab[w][low] = (tag << 16) | j;
+            SRT[base + (uint32_t)__popcll(pm & lt)] = (uint16_t)j;
+            if ((pm & lt) == 0) cnt[low] = base + (uint32_t)__popcll(pm);
         }
-        if (!LDSWS) __threadfence_block();
-        __builtin_amdgcn_wave_barrier();
-        if (valid) {
-            const uint32_t cap = n - pos < PRZM_MAX_MATCH ? n - pos : PRZM_MAX_MATCH;
-            uint32_t best = 0, bd = 0, cand = pred;
-            for (uint32_t k = 0; cand != NONE16 && k < depth; k++) {
-                const uint32_t q = wpos[cand];
-                if (in[pos + best] == in[q + best]) {
-                    const uint32_t l = lcp(in, pos, q, cap);
-                    if (l > best) { best = l; bd = pos - q; }
-                    if (best >= cap) break;
-                }
-                cand = wprev[cand];
-            }
-            const uint32_t dist = best >= 4 ? bd : 0u;
-            const uint32_t rv = pos | (dist << 16);
-            if (j < m0) R0[s0 + j] = rv;
-            else R1[s1 + j - m0] = rv;
-            if (pos == PRZM_HALF) S.st[ST_DISTB] = dist;
-        }
-        __builtin_amdgcn_wave_barrier();
+        lds_fence();
     }
+    if (!LDSWS) __threadfence_block();
+    lds_fence();
+    /* cnt[d] now holds the end of digit d's group */
+    if (!LDSWS && may_queue) {
+        uint32_t *ge = (uint32_t *)(gw + 2u * PRZM_BLOCK * 2u);
+        for (int k = 0; k < 4; k++) ge[lane * 4 + k] = cnt[lane * 4 + k];
+        __threadfence_block();
+        return true;
+    }
+    bucket_search_range(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64);
+    return false;
 }
 
-/* ---------------- parse of one half block ---------------- */
+/* ---------------- parse of one tile ---------------- */
 
-__device__ void parse_half(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint8_t *__restrict__ wsb) {
+__device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint8_t *__restrict__ wsb,
+                           uint64_t *__restrict__ pts) {
+#define QTS(i) do { if (pts && threadIdx.x == 0) pts[i] = __builtin_amdgcn_s_memrealtime(); } while (0)
     const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
     uint8_t *cmd = wsb + PRZM_WS_CMD, *off = wsb + PRZM_WS_OFF, *lit = wsb + PRZM_WS_LIT;
     uint8_t *op = wsb + PRZM_WS_OP, *meta = wsb + PRZM_WS_META, *var = wsb + PRZM_WS_VAR;
@@ -372,14 +489,14 @@ __device__ void parse_half(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
     const uint32_t s = tid, ss = a + s * SEG;
     const bool segv = ss < b;
     const uint32_t se = segv ? (ss + SEG < b ? ss + SEG : b) : ss;
+    const uint32_t nseg = (b - a + SEG - 1) / SEG;
 
     /* 1. speculative walk of every segment from its start */
     if (segv) {
-        uint64_t bm = 0;
-        uint32_t p = ss, xp = 0;
+        uint32_t bm = 0, p = ss, xp = 0;
         bool cross = false;
         while (p < se) {
-            bm |= 1ull << (p - ss);
+            bm |= 1u << (p - ss);
             const Tok t = eval_tok<false>(S, p, a, n, se);
             if (t.kind == T_LIT) { p++; continue; }
             if (t.len == LEN_CROSS) { cross = true; xp = p; break; }
@@ -388,46 +505,150 @@ __device__ void parse_half(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
         S.sm.ps.bmp[s] = bm;
         S.sm.ps.xpos[s] = cross ? (xp | XCROSS) : p;
     }
-    S.sm.ps.tstart[s] = NONE32;
+    S.sm.ps.ent[s] = NONE32;
     __syncthreads();
-
-    /* 2. follow the true path (wave 0) */
-    if (w == 0) {
-        uint32_t cur = S.st[ST_ENTRY];
-        while (cur < b) {
-            const uint32_t sg = (cur - a) / SEG, sgs = a + sg * SEG, sge = sgs + SEG < b ? sgs + SEG : b;
-            uint32_t xp = S.sm.ps.xpos[sg];
-            if (!((S.sm.ps.bmp[sg] >> (cur - sgs)) & 1ull)) {
-                /* the true entry is off the speculative path: walk the segment again */
-                uint32_t p = cur;
-                bool cross = false;
-                while (p < sge) {
-                    const Tok t = eval_tok<false>(S, p, a, n, sge);
+    /* 1b. entries from the previous segment's exit, in parallel: when that exit
+     * lands in this segment off its path, walk from it until the old path is met */
+    for (int round = 0; round < 2; round++) {
+        uint32_t xp = NONE32;
+        if (segv && s > 0) xp = S.sm.ps.xpos[s - 1];
+        __syncthreads();
+        if (segv && s > 0 && !(xp & XCROSS) && xp >= ss && xp < se) {
+            const uint32_t old = S.sm.ps.bmp[s];
+            if (!((old >> (xp - ss)) & 1u)) {
+                uint32_t p = xp, nb = 0;
+                bool done = false;
+                while (p < se) {
+                    if ((old >> (p - ss)) & 1u) { S.sm.ps.bmp[s] = old | nb; done = true; break; }
+                    nb |= 1u << (p - ss);
+                    const Tok t = eval_tok<false>(S, p, a, n, se);
                     if (t.kind == T_LIT) { p++; continue; }
-                    if (t.len == LEN_CROSS) { cross = true; xp = p | XCROSS; break; }
+                    if (t.len == LEN_CROSS) { S.sm.ps.xpos[s] = p | XCROSS; S.sm.ps.bmp[s] = nb; done = true; break; }
                     p += t.len;
                 }
-                if (!cross) xp = p;
-                if (lane == 0) S.sm.ps.xpos[sg] = xp;
-            }
-            if (lane == 0) S.sm.ps.tstart[sg] = cur;
-            if (xp & XCROSS) {
-                const uint32_t tp = xp & ~XCROSS;
-                const Tok t = eval_tok<true>(S, tp, a, n, NONE32);
-                if (lane == 0) S.sm.ps.xlen[sg] = t.len | (t.kind << 30);
-                cur = tp + t.len;
-            } else {
-                cur = xp;
+                if (!done) { S.sm.ps.xpos[s] = p; S.sm.ps.bmp[s] = nb; }
             }
         }
-        if (lane == 0) S.st[ST_ENTRY] = cur;
+        __syncthreads();
+    }
+    QTS(0);
+
+    /* 2. the true path (wave 0): orbit of the entry segment under
+     *    segment -> segment-of-exit by pointer doubling; entries that are off
+     *    their segment's speculative path are re-walked (merging into the old
+     *    path where they meet) and the tokens crossing segment ends with an
+     *    unknown length are resolved, until the path is consistent. */
+    if (w == 0) {
+        uint32_t e0 = S.st[ST_ENTRY];
+        uint32_t iters = 0;
+        for (;;) {
+            iters++;
+            const uint32_t s0 = (e0 - a) / SEG;
+            for (uint32_t k = 0; k < 8; k++) {
+                const uint32_t sg = lane * 8 + k;
+                if (sg < s0 || sg >= nseg) continue;
+                const uint32_t x = S.sm.ps.xpos[sg];
+                const uint32_t nx = (x & XCROSS) ? NX_STOP : (x >= b ? NX_END : (x - a) / SEG);
+                S.sm.ps.nxt[sg] = (uint16_t)nx;
+                S.sm.ps.jmp[sg] = (uint16_t)nx;
+                S.sm.ps.vis[sg] = sg == s0;
+            }
+            lds_fence();
+            for (int r = 0; r < 9; r++) {
+                uint32_t jn[8];
+                for (uint32_t k = 0; k < 8; k++) {
+                    const uint32_t sg = lane * 8 + k;
+                    jn[k] = NX_END;
+                    if (sg < s0 || sg >= nseg) continue;
+                    const uint32_t jm = S.sm.ps.jmp[sg];
+                    if (jm < NSEG) {
+                        if (S.sm.ps.vis[sg]) S.sm.ps.vis[jm] = 1;
+                        jn[k] = S.sm.ps.jmp[jm];
+                    } else jn[k] = jm;
+                }
+                lds_fence();
+                for (uint32_t k = 0; k < 8; k++) {
+                    const uint32_t sg = lane * 8 + k;
+                    if (sg >= s0 && sg < nseg) S.sm.ps.jmp[sg] = (uint16_t)jn[k];
+                }
+                lds_fence();
+            }
+            for (uint32_t k = 0; k < 8; k++) {
+                const uint32_t sg = lane * 8 + k;
+                if (sg < s0 || sg >= nseg) continue;
+                const uint32_t nx = S.sm.ps.nxt[sg];
+                if (S.sm.ps.vis[sg] && nx < NSEG) S.sm.ps.prv[nx] = (uint16_t)sg;
+            }
+            lds_fence();
+            uint32_t bad = NSEG;
+            uint32_t okm = 0; /* bit k: segment lane*8+k is visited and consistent */
+            for (uint32_t k = 0; k < 8; k++) {
+                const uint32_t sg = lane * 8 + k;
+                if (sg < s0 || sg >= nseg) continue;
+                if (!S.sm.ps.vis[sg]) { S.sm.ps.ent[sg] = NONE32; continue; }
+                const uint32_t e = sg == s0 ? e0 : S.sm.ps.xpos[S.sm.ps.prv[sg]];
+                S.sm.ps.ent[sg] = e;
+                const bool ok = (S.sm.ps.bmp[sg] >> (e - (a + sg * SEG))) & 1u;
+                if (ok) okm |= 1u << k;
+                if ((!ok || S.sm.ps.nxt[sg] == NX_STOP) && sg < bad) bad = sg;
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const uint32_t y = __shfl_xor(bad, o);
+                bad = y < bad ? y : bad;
+            }
+            lds_fence();
+            if (bad == NSEG) {
+                /* consistent: the quarter's exit is the exit of the visited segment leaving it */
+                for (uint32_t k = 0; k < 8; k++) {
+                    const uint32_t sg = lane * 8 + k;
+                    if (sg >= s0 && sg < nseg && S.sm.ps.vis[sg] && S.sm.ps.nxt[sg] == NX_END)
+                        S.st[ST_ENTRY] = S.sm.ps.xpos[sg];
+                }
+                if (pts && lane == 0) pts[10] = iters;
+                break;
+            }
+            /* re-walk every visited segment whose entry is off its path */
+            for (uint32_t k = 0; k < 8; k++) {
+                const uint32_t sg = lane * 8 + k;
+                if (sg < bad || sg >= nseg || !S.sm.ps.vis[sg] || (okm >> k) & 1u) continue;
+                const uint32_t sgs = a + sg * SEG, sge = sgs + SEG < b ? sgs + SEG : b;
+                uint32_t p = S.sm.ps.ent[sg], nb = 0;
+                const uint32_t old = S.sm.ps.bmp[sg];
+                bool done = false;
+                while (p < sge) {
+                    if ((old >> (p - sgs)) & 1u) { S.sm.ps.bmp[sg] = old | nb; done = true; break; }
+                    nb |= 1u << (p - sgs);
+                    const Tok t = eval_tok<false>(S, p, a, n, sge);
+                    if (t.kind == T_LIT) { p++; continue; }
+                    if (t.len == LEN_CROSS) { S.sm.ps.xpos[sg] = p | XCROSS; S.sm.ps.bmp[sg] = nb; done = true; break; }
+                    p += t.len;
+                }
+                if (!done) { S.sm.ps.xpos[sg] = p; S.sm.ps.bmp[sg] = nb; }
+            }
+            lds_fence();
+            /* the first problem segment, if consistent now, may still end in an unresolved token */
+            const uint32_t eb = S.sm.ps.ent[bad];
+            const uint32_t xb = S.sm.ps.xpos[bad];
+            const bool okb = (S.sm.ps.bmp[bad] >> (eb - (a + bad * SEG))) & 1u;
+            if (okb && (xb & XCROSS)) {
+                const uint32_t tp = xb & ~XCROSS;
+                const Tok t = eval_tok<true>(S, tp, a, n, NONE32);
+                if (lane == 0) {
+                    S.sm.ps.xlen[bad] = t.len | (t.kind << 30);
+                    S.sm.ps.xpos[bad] = tp + t.len;
+                }
+                lds_fence();
+            }
+            e0 = eb;
+        }
     }
     __syncthreads();
 
-    /* 3. count literals / runs / matches per segment on the true path */
-    const uint32_t t0 = S.sm.ps.tstart[s];
+    QTS(1);
+    /* 3. walk the true path of every segment once, caching its tokens */
+    const uint32_t t0 = S.sm.ps.ent[s];
     const bool live = segv && t0 != NONE32;
-    uint32_t nl = 0, nr = 0, nm = 0, last_end = 0;
+    uint32_t nl = 0, nr = 0, nm = 0, last_end = 0, nt = 0;
     if (live) {
         uint32_t p = t0;
         while (p < se) {
@@ -436,60 +657,64 @@ __device__ void parse_half(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
             if (t.len == LEN_CROSS) t.len = S.sm.ps.xlen[s] & 0x3FFFFFFFu;
             if (t.kind == T_RLE) nr++;
             else nm++;
+            const uint32_t k = nt ^ (s & 7u);
+            S.big.q.tlen[s][k] = (uint16_t)t.len;
+            S.big.q.toff[s][k] = (uint8_t)((p - ss) | (t.kind == T_RLE ? 0x20u : 0u));
+            nt++;
             p += t.len;
             last_end = p;
         }
     }
+    if (segv) S.sm.ps.ntk[s] = (uint8_t)nt;
     uint32_t tot_l, tot_m;
     const uint32_t lit_base = wg_scan(S, nl + nr, &tot_l) + S.st[ST_LIT];
     const uint32_t m_base = wg_scan(S, nm, &tot_m);
     uint32_t ls_in = wg_scan_max(S, last_end);
     const uint32_t carried_ls = S.st[ST_LITSTART];
     if (ls_in < carried_ls) ls_in = carried_ls;
+    QTS(2);
 
     /* 4. match distances in order */
     if (live) {
-        uint32_t p = t0, mi = m_base;
-        while (p < se) {
-            Tok t = eval_tok<false>(S, p, a, n, se);
-            if (t.kind == T_LIT) { p++; continue; }
-            if (t.len == LEN_CROSS) t.len = S.sm.ps.xlen[s] & 0x3FFFFFFFu;
-            if (t.kind == T_MATCH) S.sm.ps.tdist[mi++] = S.big.dist[p - a];
-            p += t.len;
+        uint32_t mi = m_base;
+        for (uint32_t k = 0; k < nt; k++) {
+            const uint32_t sk = k ^ (s & 7u);
+            const uint32_t o = S.big.q.toff[s][sk];
+            if (!(o & 0x20u)) S.sm.ps.tdist[mi++] = (uint16_t)dget(S, ss + (o & 31u) - a);
         }
     }
     __syncthreads();
+    QTS(3);
 
     /* 5. byte counts per segment */
     uint32_t ccmd = 0, coff = 0, cop = 0, cmeta = 0, cvar = 0;
     if (live) {
-        uint32_t p = t0, mi = m_base, ls = ls_in;
-        while (p < se) {
-            Tok t = eval_tok<false>(S, p, a, n, se);
-            if (t.kind == T_LIT) { p++; continue; }
-            if (t.len == LEN_CROSS) t.len = S.sm.ps.xlen[s] & 0x3FFFFFFFu;
+        uint32_t mi = m_base, ls = ls_in;
+        for (uint32_t k = 0; k < nt; k++) {
+            const uint32_t sk = k ^ (s & 7u);
+            const uint32_t o = S.big.q.toff[s][sk], len = S.big.q.tlen[s][sk];
+            const uint32_t p = ss + (o & 31u);
             const uint32_t ll = p - ls;
-            if (t.kind == T_RLE) {
+            if (o & 0x20u) {
                 if (ll) { ccmd += 1; cop += 1; if (ll > 8) { const uint32_t q = leb_len(ll); ccmd += q; cvar += q; } }
-                const uint32_t q = leb_len(t.len);
+                const uint32_t q = leb_len(len);
                 ccmd += 1 + q; cop += 1; cvar += q;
             } else {
-                const uint32_t d = S.big.dist[p - a];
-                const int k = lru_index(S, mi, d);
+                const uint32_t d = dget(S, p - a);
+                const int kk = lru_index(S, mi, d);
                 mi++;
-                const uint32_t keep = k >= 0 ? (t.len <= 7 ? 7u : 15u) : 3u;
+                const uint32_t keep = kk >= 0 ? (len <= 7 ? 7u : 15u) : 3u;
                 if (ll > keep) {
                     const uint32_t f = ll - keep;
                     ccmd += 1; cop += 1;
                     if (f > 8) { const uint32_t q = leb_len(f); ccmd += q; cvar += q; }
                 }
-                const uint32_t xq = t.len - 4 > 14 ? leb_len(t.len - 19) : 0u;
-                if (k >= 0 && t.len <= 7) { ccmd += 1; cop += 1; }
-                else if (k >= 0) { ccmd += 2 + xq; cop += 1; cmeta += 1; cvar += xq; }
+                const uint32_t xq = len - 4 > 14 ? leb_len(len - 19) : 0u;
+                if (kk >= 0 && len <= 7) { ccmd += 1; cop += 1; }
+                else if (kk >= 0) { ccmd += 2 + xq; cop += 1; cmeta += 1; cvar += xq; }
                 else { ccmd += 1 + xq; cop += 1; cvar += xq; coff += leb_len(d); }
             }
-            p += t.len;
-            ls = p;
+            ls = p + len;
         }
     }
     uint32_t t_cmd, t_off, t_op, t_meta, t_var;
@@ -498,16 +723,20 @@ __device__ void parse_half(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
     uint32_t b_op = wg_scan(S, cop, &t_op) + S.st[ST_OP];
     uint32_t b_meta = wg_scan(S, cmeta, &t_meta) + S.st[ST_META];
     uint32_t b_var = wg_scan(S, cvar, &t_var) + S.st[ST_VAR];
+    QTS(4);
 
-    /* 6. write bytes */
+    /* 6. write bytes: literal gaps, then each token */
     if (live) {
-        uint32_t p = t0, mi = m_base, ls = ls_in, lb = lit_base;
-        while (p < se) {
-            Tok t = eval_tok<false>(S, p, a, n, se);
-            if (t.kind == T_LIT) { lit[lb++] = in[p]; p++; continue; }
-            if (t.len == LEN_CROSS) t.len = S.sm.ps.xlen[s] & 0x3FFFFFFFu;
+        uint32_t mi = m_base, ls = ls_in, lb = lit_base, cur = t0;
+        for (uint32_t k = 0; k <= nt; k++) {
+            const uint32_t sk = k ^ (s & 7u);
+            const uint32_t o = k < nt ? S.big.q.toff[s][sk] : 0u;
+            const uint32_t len = k < nt ? S.big.q.tlen[s][sk] : 0u;
+            const uint32_t p = k < nt ? ss + (o & 31u) : se;
+            for (uint32_t q = cur; q < p; q++) lit[lb++] = in[q];
+            if (k == nt) break;
             const uint32_t ll = p - ls;
-            if (t.kind == T_RLE) {
+            if (o & 0x20u) {
                 if (ll) {
                     if (ll <= 8) { cmd[b_cmd++] = (uint8_t)(0xF2 + ll); op[b_op++] = (uint8_t)(0xF2 + ll); }
                     else {
@@ -516,13 +745,13 @@ __device__ void parse_half(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
                     }
                 }
                 cmd[b_cmd++] = 0xF1; op[b_op++] = 0xF1;
-                b_cmd = put_leb(cmd, b_cmd, t.len); b_var = put_leb(var, b_var, t.len);
+                b_cmd = put_leb(cmd, b_cmd, len); b_var = put_leb(var, b_var, len);
                 lit[lb++] = in[p];
             } else {
-                const uint32_t d = S.big.dist[p - a];
-                const int k = lru_index(S, mi, d);
+                const uint32_t d = dget(S, p - a);
+                const int kk = lru_index(S, mi, d);
                 mi++;
-                const uint32_t keep = k >= 0 ? (t.len <= 7 ? 7u : 15u) : 3u;
+                const uint32_t keep = kk >= 0 ? (len <= 7 ? 7u : 15u) : 3u;
                 uint32_t l2 = ll;
                 if (ll > keep) {
                     const uint32_t f = ll - keep;
@@ -533,31 +762,31 @@ __device__ void parse_half(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
                     }
                     l2 = keep;
                 }
-                const uint32_t base = t.len - 4 > 14 ? 15u : t.len - 4;
-                if (k >= 0 && t.len <= 7) {
-                    const uint8_t o = (uint8_t)((l2 << 4) | ((t.len - 4) << 2) | (uint32_t)k);
-                    cmd[b_cmd++] = o; op[b_op++] = o;
-                } else if (k >= 0) {
+                const uint32_t base = len - 4 > 14 ? 15u : len - 4;
+                if (kk >= 0 && len <= 7) {
+                    const uint8_t oc = (uint8_t)((l2 << 4) | ((len - 4) << 2) | (uint32_t)kk);
+                    cmd[b_cmd++] = oc; op[b_op++] = oc;
+                } else if (kk >= 0) {
                     const uint8_t tok = (uint8_t)((l2 << 4) | base);
-                    cmd[b_cmd++] = (uint8_t)(0xC0 | k); cmd[b_cmd++] = tok;
-                    op[b_op++] = (uint8_t)(0xC0 | k); meta[b_meta++] = tok;
-                    if (base == 15) { b_cmd = put_leb(cmd, b_cmd, t.len - 19); b_var = put_leb(var, b_var, t.len - 19); }
+                    cmd[b_cmd++] = (uint8_t)(0xC0 | kk); cmd[b_cmd++] = tok;
+                    op[b_op++] = (uint8_t)(0xC0 | kk); meta[b_meta++] = tok;
+                    if (base == 15) { b_cmd = put_leb(cmd, b_cmd, len - 19); b_var = put_leb(var, b_var, len - 19); }
                 } else {
-                    const uint8_t o = (uint8_t)(0x80 | (l2 << 4) | base);
-                    cmd[b_cmd++] = o; op[b_op++] = o;
-                    if (base == 15) { b_cmd = put_leb(cmd, b_cmd, t.len - 19); b_var = put_leb(var, b_var, t.len - 19); }
+                    const uint8_t oc = (uint8_t)(0x80 | (l2 << 4) | base);
+                    cmd[b_cmd++] = oc; op[b_op++] = oc;
+                    if (base == 15) { b_cmd = put_leb(cmd, b_cmd, len - 19); b_var = put_leb(var, b_var, len - 19); }
                     b_off = put_leb(off, b_off, d);
                 }
             }
-            p += t.len;
-            ls = p;
+            ls = p + len;
+            cur = p + len;
         }
     }
     __syncthreads();
+    QTS(5);
 
-    /* 7. carry the uniform state into the next half */
+    /* 7. carry the uniform state into the next tile */
     if (tid == 0) {
-        uint32_t r0 = S.st[ST_R0], r1 = S.st[ST_R1], r2 = S.st[ST_R2];
         uint32_t out[3];
         int no = 0;
         for (uint32_t j = tot_m; j-- > 0 && no < 3;) {
@@ -566,7 +795,7 @@ __device__ void parse_half(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
             for (int q = 0; q < no; q++) dup |= out[q] == v;
             if (!dup) out[no++] = v;
         }
-        const uint32_t rr[3] = {r0, r1, r2};
+        const uint32_t rr[3] = {S.st[ST_R0], S.st[ST_R1], S.st[ST_R2]};
         for (int j = 0; j < 3 && no < 3; j++) {
             bool dup = false;
             for (int q = 0; q < no; q++) dup |= out[q] == rr[j];
@@ -575,24 +804,22 @@ __device__ void parse_half(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
         S.st[ST_R0] = out[0]; S.st[ST_R1] = out[1]; S.st[ST_R2] = out[2];
         S.st[ST_CMD] += t_cmd; S.st[ST_OFF] += t_off; S.st[ST_OP] += t_op;
         S.st[ST_META] += t_meta; S.st[ST_VAR] += t_var; S.st[ST_LIT] += tot_l;
-        /* last non-literal end of this half (or the carried one) */
         S.st[ST_NMATCH] = tot_m;
     }
-    /* litStart for the next half: the exclusive max over all segments plus this one */
     {
-        uint32_t v = live ? last_end : 0u;
+        const uint32_t v = live ? last_end : 0u;
         const uint32_t ex = wg_scan_max(S, v);
         const uint32_t incl = ex > v ? ex : v;
-        /* thread NT-1 holds the maximum over all segments */
         if (tid == LZ_NT - 1) {
             const uint32_t c = S.st[ST_LITSTART];
             S.st[ST_LITSTART] = incl > c ? incl : c;
         }
     }
-    uint32_t ntok_half;
-    (void)wg_scan(S, nr + nm, &ntok_half);
-    if (tid == 0) S.st[ST_TOK] += ntok_half;
+    uint32_t ntok;
+    (void)wg_scan(S, nr + nm, &ntok);
+    if (tid == 0) S.st[ST_TOK] += ntok;
     __syncthreads();
+    QTS(6);
     (void)lane;
 }
 
@@ -601,21 +828,29 @@ __device__ void parse_half(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
 extern "C" __global__ void __launch_bounds__(LZ_NT)
 przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks, uint32_t nblocks,
                uint32_t depth, uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta,
-               uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint16_t *__restrict__ dist_dbg) {
+               uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint16_t *__restrict__ dist_dbg,
+               uint64_t *__restrict__ phase_ts) {
     __shared__ LzShared S;
     const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
     uint8_t *my = scratch + (uint64_t)blockIdx.x * scratch_stride;
-    uint16_t *T0 = (uint16_t *)my;
-    uint16_t *T1 = T0 + PRZM_HALF;
-    uint32_t *R0 = (uint32_t *)(T1 + PRZM_HALF);
-    uint32_t *R1 = R0 + PRZM_HALF;
-    uint16_t *gws = (uint16_t *)(R1 + PRZM_HALF) + (uint64_t)w * 2u * PRZM_BLOCK;
+    Scr sc;
+    sc.T0 = (const uint16_t *)my;
+    sc.T1 = sc.T0 + PRZM_HALF;
+    sc.R0 = (uint32_t *)(my + 2u * PRZM_HALF * 2u);
+    sc.R1 = sc.R0 + PRZM_HALF;
+    sc.L0 = (uint16_t *)(sc.R1 + PRZM_HALF);
+    sc.L1 = sc.L0 + PRZM_HALF;
+    sc.gws = (uint8_t *)(sc.L1 + PRZM_HALF);
+    uint16_t *T0w = (uint16_t *)my, *T1w = T0w + PRZM_HALF;
 
     for (uint32_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
         const przm_block_t bd = blocks[blk];
         const uint32_t n = bd.len;
         przm_meta_t *mt = meta + blk;
         uint8_t *wsb = ws + (uint64_t)blk * PRZM_WS_BYTES;
+        uint64_t *ts = phase_ts ? phase_ts + (uint64_t)blk * 32 : nullptr;
+#define PTS(i) do { if (ts && tid == 0) ts[i] = __builtin_amdgcn_s_memrealtime(); } while (0)
+        PTS(0);
         if ((bd.flags & PRZM_BF_RAW) || n == 0) {
             if (tid < PRZM_S_N) mt->len[tid] = 0;
             continue;
@@ -634,10 +869,12 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             if (i + 16 <= PRZM_BLOCK + 64) *(uint4 *)(S.in + i) = v;
         }
         if (tid < ST_N) S.st[tid] = 0;
+        if (tid < LZ_NW) S.bigq[tid] = 0;
         __syncthreads();
         if (tid == 0) { S.st[ST_R0] = 1; S.st[ST_R1] = 4; S.st[ST_R2] = 8; }
 
         const uint32_t nh = n >= 4 ? n - 3 : 0; /* positions p with p + 4 <= n */
+        PTS(1);
 
         /* A. bucket positions by the hash's high byte, per half */
         for (uint32_t H = 0; H < 2; H++) {
@@ -676,22 +913,23 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
                 const bool valid = p < end;
                 const uint32_t dg = valid ? hash4(S.in, p) >> 8 : 0u;
                 const uint64_t pm = peers8(dg, valid);
+                const uint32_t cur = valid ? S.sm.tab[w][dg] : 0u;
+                lds_fence();
                 if (valid) {
-                    const uint32_t cur = S.sm.tab[w][dg];
                     S.big.stage[cur + (uint32_t)__popcll(pm & lt)] = (uint16_t)p;
                     if ((pm & lt) == 0) S.sm.tab[w][dg] = cur + (uint32_t)__popcll(pm);
                 }
-                __builtin_amdgcn_wave_barrier();
+                lds_fence();
             }
             __syncthreads();
             const uint32_t cnt = S.bstart[H][256];
-            uint16_t *T = H ? T1 : T0;
+            uint16_t *T = H ? T1w : T0w;
             for (uint32_t i = tid; i < cnt; i += LZ_NT) T[i] = S.big.stage[i];
             __syncthreads();
         }
+        PTS(2);
 
-        /* C. per bucket: chains by the low byte, then the chain search */
-        for (uint32_t i = tid; i < LZ_NW * 256; i += LZ_NT) (&S.sm.tab[0][0])[i] = 0;
+        /* C. per bucket: group by the low byte, search; large buckets are searched by all waves */
         if (tid == 0) S.st[ST_BUCKET] = 0;
         __threadfence_block();
         __syncthreads();
@@ -702,30 +940,63 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             if (dg >= 256) break;
             const uint32_t m = (S.bstart[0][dg + 1] - S.bstart[0][dg]) + (S.bstart[1][dg + 1] - S.bstart[1][dg]);
             if (m == 0) continue;
-            if (m <= WS_CAP) bucket_search<true>(S, w, dg, n, depth, T0, T1, R0, R1, nullptr, nullptr);
-            else bucket_search<false>(S, w, dg, n, depth, T0, T1, R0, R1, gws, gws + PRZM_BLOCK);
+            if (m <= WS_CAP) bucket_sort<true>(S, sc, w, dg, n, depth, false);
+            else {
+                const bool q = bucket_sort<false>(S, sc, w, dg, n, depth, S.bigq[w] == 0);
+                if (q && lane == 0) S.bigq[w] = dg + 1;
+            }
         }
         __threadfence_block();
         __syncthreads();
+        PTS(13);
+        for (uint32_t ow = 0; ow < LZ_NW; ow++) {
+            const uint32_t dq = S.bigq[ow];
+            if (!dq) continue;
+            const uint32_t dg = dq - 1;
+            const uint32_t m = (S.bstart[0][dg + 1] - S.bstart[0][dg]) + (S.bstart[1][dg + 1] - S.bstart[1][dg]);
+            const uint8_t *gw = sc.gws + (uint64_t)ow * GWS_STRIDE;
+            const uint16_t *P = (const uint16_t *)gw;
+            const uint16_t *SRT = P + PRZM_BLOCK;
+            const uint32_t *GE = (const uint32_t *)(gw + 2u * PRZM_BLOCK * 2u);
+            bucket_search_range(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
+        }
+        __threadfence_block();
+        __syncthreads();
+        PTS(3);
 
-        /* E + parse, per half */
-        const uint32_t nhalves = n > PRZM_HALF ? 2u : 1u;
-        for (uint32_t H = 0; H < nhalves; H++) {
-            const uint32_t a = H * PRZM_HALF, b = n < a + PRZM_HALF ? n : a + PRZM_HALF;
-            for (uint32_t i = tid; i <= b - a; i += LZ_NT) S.big.dist[i] = 0;
+        /* E + parse, per quarter tile */
+        for (uint32_t Q = 0; Q < 4; Q++) {
+            const uint32_t a = Q * QP;
+            if (a >= n) break;
+            const uint32_t b = n < a + QP ? n : a + QP;
+            const uint32_t H = Q >> 1;
+            for (uint32_t i = tid; i <= b - a; i += LZ_NT) { S.big.q.dist[DIX(i)] = 0; S.big.q.lc[LIX(i)] = 0; }
             __syncthreads();
-            const uint32_t cnt = S.bstart[H][256];
-            const uint32_t *R = H ? R1 : R0;
-            for (uint32_t i = tid; i < cnt; i += LZ_NT) {
+            const uint32_t hs = S.bstart[H][256];
+            const uint32_t *R = H ? sc.R1 : sc.R0;
+            const uint16_t *RL = H ? sc.L1 : sc.L0;
+            for (uint32_t i = tid; i < hs; i += LZ_NT) {
                 const uint32_t rv = R[i];
-                S.big.dist[(rv & 0xFFFFu) - a] = (uint16_t)(rv >> 16);
+                const uint32_t p = rv & 0xFFFFu;
+                if (p >= a && p <= b) {
+                    const uint32_t l = RL[i];
+                    S.big.q.dist[DIX(p - a)] = (uint16_t)(rv >> 16);
+                    S.big.q.lc[LIX(p - a)] = (uint8_t)(l < 255 ? l : 255);
+                }
             }
-            if (H == 0 && b < n && tid == 0) S.big.dist[b - a] = (uint16_t)S.st[ST_DISTB];
+            if (b == PRZM_HALF && b < n && tid == 0) {
+                const uint32_t l = S.st[ST_LB];
+                S.big.q.dist[DIX(b - a)] = (uint16_t)S.st[ST_DISTB];
+                S.big.q.lc[LIX(b - a)] = (uint8_t)(l < 255 ? l : 255);
+            }
             __syncthreads();
             if (dist_dbg)
-                for (uint32_t i = tid; i < b - a; i += LZ_NT) dist_dbg[(uint64_t)blk * PRZM_BLOCK + a + i] = S.big.dist[i];
-            if (S.st[ST_ENTRY] < b) parse_half(S, a, b, n, wsb);
+                for (uint32_t i = tid; i < b - a; i += LZ_NT)
+                    dist_dbg[(uint64_t)blk * PRZM_BLOCK + a + i] = S.big.q.dist[DIX(i)];
+            PTS(4 + 2 * Q);
+            if (S.st[ST_ENTRY] < b) parse_tile(S, a, b, n, wsb, (ts && Q == 0) ? ts + 16 : nullptr);
             __syncthreads();
+            PTS(5 + 2 * Q);
         }
 
         /* trailing literal run (przm_tokenizer.cuh:440-457) */
@@ -746,6 +1017,7 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             mt->len[PRZM_S_VAR] = nvar;
             mt->len[PRZM_S_TOK] = S.st[ST_TOK];
         }
+        PTS(12);
         __syncthreads();
     }
 }

@@ -30,6 +30,7 @@
 namespace {
 
 thread_local char g_err[512];
+uint64_t *g_phase_ts = nullptr; /* optional per-block phase timestamps of the LZ kernel (device memory) */
 std::atomic<int32_t> g_mock_unavailable{0};
 std::atomic<int32_t> g_mock_error{0};
 
@@ -49,8 +50,8 @@ std::atomic<int32_t> g_mock_error{0};
         if (r_ != 0) return r_; \
     } while (0)
 
-constexpr uint64_t LZ_SCRATCH_STRIDE =
-    2ull * PRZM_HALF * 2 /* T0,T1 */ + 2ull * PRZM_HALF * 4 /* R0,R1 */ + 8ull * 2 * PRZM_BLOCK * 2 /* per-wave fallback */;
+constexpr uint64_t LZ_SCRATCH_STRIDE = 2ull * PRZM_HALF * 2 /* T0,T1 */ + 2ull * PRZM_HALF * 4 /* R0,R1 */ +
+                                       2ull * PRZM_HALF * 2 /* L0,L1 */ + (uint64_t)LZ_NW * GWS_STRIDE;
 
 struct DevBuf {
     void *p = nullptr;
@@ -229,7 +230,8 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         CHK(rec());
         hipLaunchKernelGGL(przm_lz_kernel, dim3(gz), dim3(LZ_NT), 0, st, d_src, blocks + c0, nb, depth,
                            (uint8_t *)c.ws.p, meta + c0, (uint8_t *)c.scratch.p, (uint64_t)LZ_SCRATCH_STRIDE,
-                           dbg ? (uint16_t *)c.dbg.p + (uint64_t)c0 * PRZM_BLOCK : (uint16_t *)nullptr);
+                           dbg ? (uint16_t *)c.dbg.p + (uint64_t)c0 * PRZM_BLOCK : (uint16_t *)nullptr,
+                           g_phase_ts ? g_phase_ts + (uint64_t)c0 * 32 : (uint64_t *)nullptr);
         HIPCHK(hipGetLastError());
         if (stop_after_lz) continue;
         CHK(rec());
@@ -400,6 +402,10 @@ int32_t init_handle(przm_gpu *g, int device) {
 extern "C" {
 
 const char *przm_gpu_last_error(void) { return g_err; }
+
+/* Development hook: when set, the LZ kernel writes 16 s_memrealtime stamps per block
+ * (100 MHz) into this device array (16 x blocks u64). */
+void przm_gpu_debug_phase_ts(void *d) { g_phase_ts = (uint64_t *)d; }
 
 void przm_gpu_set_mock_unavailable(int32_t on) { g_mock_unavailable.store(on ? 1 : 0); }
 void przm_gpu_set_mock_error(int32_t code) { g_mock_error.store(code); }
