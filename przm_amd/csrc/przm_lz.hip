@@ -41,7 +41,8 @@
 #define QP 16384                  /* parse tile: a quarter block */
 #define SEG 32                    /* speculative-walk segment */
 #define NSEG (QP / SEG)           /* 512: one per thread */
-#define WS_CAP 2048               /* bucket elements kept in LDS per wave */
+#define WS_CAP 1024               /* bucket elements kept in LDS per wave */
+#define RING 128                  /* staged search keys per wave: previous + current chunk */
 #define NONE16 0xFFFFu
 #define NONE32 0xFFFFFFFFu
 #define XCROSS 0x80000000u
@@ -73,6 +74,8 @@ struct LzShared {
         struct {
             uint16_t pos[LZ_NW][WS_CAP];
             uint16_t srt[LZ_NW][WS_CAP];
+            uint4 ringa[LZ_NW][RING]; /* bytes p..p+15 of a staged bucket element */
+            uint4 ringb[LZ_NW][RING]; /* bytes p+16..p+27, and p */
         } ws;
         struct {
             uint16_t dist[DSZ];
@@ -363,43 +366,153 @@ struct Scr {
     uint8_t *gws; /* per-wave workspaces, GWS_STRIDE bytes each */
 };
 
-/* search elements [i0, i1) (sorted order) of a bucket whose position array is
- * P, sorted permutation SRT and per-digit group ends GE */
-template <typename PT, typename ET>
+typedef unsigned __int128 u128;
+
+/* common prefix (0..28) of two 28-byte keys given as the xor of their seven words */
+__device__ __forceinline__ uint32_t lcp28x(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, uint32_t x4,
+                                           uint32_t x5, uint32_t x6) {
+    const u128 v = ((u128)x3 << 96) | ((u128)x2 << 64) | ((u128)x1 << 32) | (u128)x0;
+    const u128 u = ((u128)1 << 96) | ((u128)x6 << 64) | ((u128)x5 << 32) | (u128)x4; /* sentinel at 28 bytes */
+    const uint32_t t0 = (uint32_t)__builtin_ctzg(v, 128), t1 = (uint32_t)__builtin_ctzg(u, 128);
+    return (v ? t0 : 128u + t1) >> 3;
+}
+
+/* byte masks of the first bt (1..28) bytes of a 28-byte key, one per word */
+__device__ __forceinline__ void prefix_masks(uint32_t bt, uint32_t mk[7]) {
+#pragma unroll
+    for (uint32_t k = 0; k < 7; k++) {
+        const uint32_t lo = 4u * k;
+        mk[k] = bt >= lo + 4u ? 0xFFFFFFFFu : bt <= lo ? 0u : 0xFFFFFFFFu >> (32u - 8u * (bt - lo));
+    }
+}
+
+/* bytes p..p+15 as four little-endian words */
+__device__ __forceinline__ uint4 ldw16(const uint8_t *in, uint32_t p) {
+    const uint32_t *wp = (const uint32_t *)(in + (p & ~3u));
+    const uint32_t sh = p & 3u, w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3], w4 = wp[4];
+    return make_uint4(__builtin_amdgcn_alignbyte(w1, w0, sh), __builtin_amdgcn_alignbyte(w2, w1, sh),
+                      __builtin_amdgcn_alignbyte(w3, w2, sh), __builtin_amdgcn_alignbyte(w4, w3, sh));
+}
+
+/* common prefix of in[p..] and in[q..], at most cap, 16 bytes per step; the
+ * first 16 bytes of p are given */
+__device__ __forceinline__ uint32_t lcp_from(const uint8_t *in, uint4 pk, uint32_t p, uint32_t q, uint32_t cap) {
+    uint32_t l = 0;
+    for (;;) {
+        const uint4 qk = ldw16(in, q + l);
+        const uint32_t x0 = pk.x ^ qk.x, x1 = pk.y ^ qk.y, x2 = pk.z ^ qk.z, x3 = pk.w ^ qk.w;
+        uint32_t e = 16u;
+        if (x0) e = (uint32_t)__builtin_ctz(x0) >> 3;
+        else if (x1) e = 4u + ((uint32_t)__builtin_ctz(x1) >> 3);
+        else if (x2) e = 8u + ((uint32_t)__builtin_ctz(x2) >> 3);
+        else if (x3) e = 12u + ((uint32_t)__builtin_ctz(x3) >> 3);
+        l += e;
+        if (e < 16u || l >= cap) return l < cap ? l : cap;
+        pk = ldw16(in, p + l);
+    }
+}
+
+/*
+ * Search elements [i0, i1) (sorted order) of a bucket whose position array is
+ * P, sorted permutation SRT and per-digit group ends GE.  Element i's chain is
+ * the up to `depth` elements before it in its group (same 16-bit hash,
+ * newest first, compress.go:681-707); the longest common prefix wins, ties
+ * keep the nearest.  Each wave stages the first 28 bytes of the current and
+ * the previous chunk of 64 elements in an LDS ring, so a candidate costs two
+ * conflict-free 16-byte reads and no compare in the block unless its first
+ * 28 bytes match.  Chains longer than the ring (depth > 64) read the rest
+ * from the block directly.
+ */
+template <bool DEEP, typename PT, typename ET>
 __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, uint32_t dgt, uint32_t n,
                                                     uint32_t depth, const PT *P, const PT *SRT, const ET *GE,
-                                                    uint32_t i0, uint32_t i1, uint32_t step) {
+                                                    uint32_t i0, uint32_t i1, uint32_t step, uint64_t *pt = nullptr) {
+    const bool prof = pt && threadIdx.x == 0;
     const uint8_t *in = S.in;
-    const uint32_t lane = __lane_id();
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    uint4 *ra = S.big.ws.ringa[w], *rb = S.big.ws.ringb[w];
     const uint32_t s0 = S.bstart[0][dgt], m0 = S.bstart[0][dgt + 1] - s0;
     const uint32_t s1 = S.bstart[1][dgt];
     for (uint32_t c0 = i0; c0 < i1; c0 += step) {
+        const uint64_t t_a = prof ? __builtin_amdgcn_s_memrealtime() : 0;
         const uint32_t i = c0 + lane;
-        if (i >= i1) continue;
-        const uint32_t j = SRT[i];
-        const uint32_t p = P[j];
-        const uint32_t low = hash4(in, p) & 0xFFu;
-        const uint32_t gs = low ? GE[low - 1] : 0u;
-        const uint32_t cap = n - p < PRZM_MAX_MATCH ? n - p : PRZM_MAX_MATCH;
-        uint32_t best = 0, bd = 0;
-        uint32_t kmax = i - gs;
-        if (kmax > depth) kmax = depth;
-        uint32_t q = kmax ? (uint32_t)P[SRT[i - 1]] : 0u;
-        for (uint32_t k = 1; k <= kmax; k++) {
-            const uint32_t qn = k < kmax ? (uint32_t)P[SRT[i - k - 1]] : 0u; /* next candidate in flight */
-            if (in[p + best] == in[q + best]) {
-                const uint32_t l = lcp(in, p, q, cap);
-                if (l > best) { best = l; bd = p - q; }
-                if (best >= cap) break;
-            }
-            q = qn;
+        const bool valid = i < i1;
+        if (c0 >= 64 && (step != 64 || c0 == i0)) { /* previous chunk not staged by this wave */
+            const uint32_t ip = c0 - 64 + lane, pp = P[SRT[ip]];
+            const uint4 b = ldw16(in, pp + 16u);
+            ra[ip & (RING - 1)] = ldw16(in, pp);
+            rb[ip & (RING - 1)] = make_uint4(b.x, b.y, b.z, pp);
         }
+        uint32_t j = 0, p = 0, kmax = 0, cap = 0;
+        uint4 ka = make_uint4(0, 0, 0, 0), kb = ka;
+        if (valid) {
+            j = SRT[i];
+            p = P[j];
+            ka = ldw16(in, p);
+            kb = ldw16(in, p + 16u);
+            ra[i & (RING - 1)] = ka;
+            rb[i & (RING - 1)] = make_uint4(kb.x, kb.y, kb.z, p);
+            const uint32_t low = hash4(in, p) & 0xFFu;
+            const uint32_t gs = low ? GE[low - 1] : 0u;
+            cap = n - p < PRZM_MAX_MATCH ? n - p : PRZM_MAX_MATCH;
+            kmax = i - gs < depth ? i - gs : depth;
+        }
+        lds_fence();
+        const uint64_t t_b = prof ? __builtin_amdgcn_s_memrealtime() : 0;
+        /* best so far and the masks of its first best + 1 bytes (at most 28): a
+         * candidate can only be longer if its key agrees on those bytes */
+        uint32_t best = 0, bd = 0, mk[7];
+        prefix_masks(1u, mk);
+        for (uint32_t k0 = 1; k0 <= depth; k0 += 4) {
+            if (!__ballot(k0 <= kmax && best < cap)) break;
+            uint4 qa[4], qb[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) { /* four candidates in flight; entries past kmax are masked below */
+                const uint32_t k = k0 + u;
+                if (DEEP && k > lane + 64u) {
+                    const uint32_t q = k <= kmax ? (uint32_t)P[SRT[i - k]] : p;
+                    const uint4 b = ldw16(in, q + 16u);
+                    qa[u] = ldw16(in, q);
+                    qb[u] = make_uint4(b.x, b.y, b.z, q);
+                } else {
+                    qa[u] = ra[(i - k) & (RING - 1)];
+                    qb[u] = rb[(i - k) & (RING - 1)];
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) {
+                const uint32_t x0 = ka.x ^ qa[u].x, x1 = ka.y ^ qa[u].y, x2 = ka.z ^ qa[u].z, x3 = ka.w ^ qa[u].w;
+                const uint32_t x4 = kb.x ^ qb[u].x, x5 = kb.y ^ qb[u].y, x6 = kb.z ^ qb[u].z;
+                const uint32_t r = (x0 & mk[0]) | (x1 & mk[1]) | (x2 & mk[2]) | (x3 & mk[3]) | (x4 & mk[4]) |
+                                   (x5 & mk[5]) | (x6 & mk[6]);
+                if (r == 0u && k0 + u <= kmax && best < cap) {
+                    const uint32_t q = qb[u].w;
+                    uint32_t l = lcp28x(x0, x1, x2, x3, x4, x5, x6);
+                    if (l == 28u && cap > 28u && (best < 28u || in[p + best] == in[q + best]))
+                        l = 28u + lcp_from(in, ldw16(in, p + 28u), p + 28u, q + 28u, cap - 28u);
+                    l = l < cap ? l : cap;
+                    if (l > best) {
+                        best = l;
+                        bd = p - q;
+                        prefix_masks(best < 28u ? best + 1u : 28u, mk);
+                    }
+                }
+            }
+        }
+        lds_fence();
+        const uint64_t t_c = prof ? __builtin_amdgcn_s_memrealtime() : 0;
+        if (valid) {
         const bool m = best >= 4;
         const uint32_t rv = p | ((m ? bd : 0u) << 16);
         const uint16_t lv = (uint16_t)(m ? best : 0u);
         if (j < m0) { sc.R0[s0 + j] = rv; sc.L0[s0 + j] = lv; }
         else { sc.R1[s1 + j - m0] = rv; sc.L1[s1 + j - m0] = lv; }
         if (p == PRZM_HALF) { S.st[ST_DISTB] = m ? bd : 0u; S.st[ST_LB] = m ? best : 0u; }
+        }
+        if (prof) {
+            const uint64_t t_d = __builtin_amdgcn_s_memrealtime();
+            pt[1] += t_b - t_a; pt[2] += t_c - t_b; pt[3] += t_d - t_c; pt[4] += 1;
+        }
     }
 }
 
@@ -407,7 +520,12 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
  * is large; returns true when the bucket was queued for the all-wave search */
 template <bool LDSWS>
 __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt, uint32_t n, uint32_t depth,
-                            bool may_queue) {
+                            bool may_queue, uint64_t *pt) {
+    /* pt (diagnostics, thread 0 only): accumulated sort, chunk staging, candidate scan and result
+     * write times, and chunks */
+    const bool prof = pt && threadIdx.x == 0;
+    const uint64_t t_a = prof ? __builtin_amdgcn_s_memrealtime() : 0;
+    uint64_t t_c = 0;
     const uint8_t *in = S.in;
     const uint32_t lane = __lane_id();
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -467,13 +585,16 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     if (!LDSWS) __threadfence_block();
     lds_fence();
     /* cnt[d] now holds the end of digit d's group */
+    if (prof) t_c = __builtin_amdgcn_s_memrealtime();
     if (!LDSWS && may_queue) {
         uint32_t *ge = (uint32_t *)(gw + 2u * PRZM_BLOCK * 2u);
         for (int k = 0; k < 4; k++) ge[lane * 4 + k] = cnt[lane * 4 + k];
         __threadfence_block();
         return true;
     }
-    bucket_search_range(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64);
+    if (prof) pt[0] += t_c - t_a;
+    if (depth > 64u) bucket_search_range<true>(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64, pt);
+    else bucket_search_range<false>(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64, pt);
     return false;
 }
 
@@ -940,9 +1061,9 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             if (dg >= 256) break;
             const uint32_t m = (S.bstart[0][dg + 1] - S.bstart[0][dg]) + (S.bstart[1][dg + 1] - S.bstart[1][dg]);
             if (m == 0) continue;
-            if (m <= WS_CAP) bucket_sort<true>(S, sc, w, dg, n, depth, false);
+            if (m <= WS_CAP) bucket_sort<true>(S, sc, w, dg, n, depth, false, ts ? ts + 27 : nullptr);
             else {
-                const bool q = bucket_sort<false>(S, sc, w, dg, n, depth, S.bigq[w] == 0);
+                const bool q = bucket_sort<false>(S, sc, w, dg, n, depth, S.bigq[w] == 0, ts ? ts + 27 : nullptr);
                 if (q && lane == 0) S.bigq[w] = dg + 1;
             }
         }
@@ -958,7 +1079,8 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             const uint16_t *P = (const uint16_t *)gw;
             const uint16_t *SRT = P + PRZM_BLOCK;
             const uint32_t *GE = (const uint32_t *)(gw + 2u * PRZM_BLOCK * 2u);
-            bucket_search_range(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
+            if (depth > 64u) bucket_search_range<true>(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
+            else bucket_search_range<false>(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
         }
         __threadfence_block();
         __syncthreads();
