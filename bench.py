@@ -16,7 +16,15 @@ generator (JSON-like records, int32 tables, motif-random, uniform random),
   serving   open-loop Poisson arrivals at 2,000 req/s, sizes log-uniform in
             [4 KiB, 16 MiB], 64 GiB HBM admission budget: p50/p99 latency.
   roofline  the dominant kernel (przm_lz_kernel) against the 8 TB/s HBM
-            peak, from a profiled run with HIP events between stages.
+            peak: its average launch time and its algorithmic bytes (block
+            bytes read once + token-stream bytes written once) come from an
+            un-graphed pass of the same batch right after the timed steps,
+            with HIP events between the launches on their stream (events
+            recorded inside a captured graph cannot be timed on this ROCm);
+            HBM traffic per block from the committed rocprofv3 --pmc summary.
+  --profile-only  one torch copy kernel of known size (the PMC scale) and one
+            batched pass over the first --profile-buffers buffers, nothing
+            else: the program the rocprofv3 --pmc passes run.
   cpu_baseline  the CPU oracle of the same mode on the host's cores over a
             time-bounded sample of the same buffers.
 """
@@ -65,7 +73,7 @@ def cpu_baseline(arena, offs, sizes, seconds):
 
 
 def pmc_traffic():
-    """HBM bytes per przm_lz_kernel dispatch from the committed rocprofv3 --pmc summary, if present."""
+    """HBM bytes per block of przm_lz_kernel from the committed rocprofv3 --pmc summary, if present."""
     p = os.path.join(ROOT, "profiles", "lz_kernel_pmc.json")
     if not os.path.exists(p):
         return None
@@ -74,6 +82,30 @@ def pmc_traffic():
         return d
     except Exception:
         return None
+
+
+def profile_only(args, g, lib, L, D, G):
+    """The program of the rocprofv3 --pmc passes: a copy kernel of known size, then one batched pass."""
+    if args.lz_grid > 0 and args.chunk > 0:
+        lib.przm_gpu_set_tuning(g.handle, args.lz_grid, args.chunk)
+    n = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for x in (a, b):
+        if lib.przm_gpu_malloc(g.handle, n, ctypes.byref(x)):
+            raise G.GPUError(-6)
+    lib.przm_gpu_copy_d2d(g.handle, b, a, n)  # warm
+    lib.przm_gpu_copy_d2d(g.handle, b, a, n)  # reads n bytes, writes n bytes
+    lib.przm_gpu_free(g.handle, a)
+    lib.przm_gpu_free(g.handle, b)
+    log(f"copy kernel: {n} bytes")
+    arena, offs, sizes = D.batch(args.profile_buffers, 1024, 8 << 20, args.seed)
+    bufs = [arena[int(o): int(o) + int(s)] for o, s in zip(offs, sizes)]
+    res, prof = g.EncodeBatch(bufs, args.level)
+    nblocks = sum(max((int(s) + 65535) // 65536, 1) for s in sizes)
+    log(f"profile pass: {len(bufs)} buffers, {int(sizes.sum())} bytes, {nblocks} blocks, "
+        f"{prof.PipelinedChunks if hasattr(prof, 'PipelinedChunks') else '?'} chunks")
+    print(json.dumps({"profile_buffers": len(bufs), "input_bytes": int(sizes.sum()), "blocks": nblocks,
+                      "copy_bytes": n}), flush=True)
 
 
 def main():
@@ -90,6 +122,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--lz-grid", type=int, default=0, help="fixed LZ workgroups (skips the autotuner with --chunk)")
+    ap.add_argument("--chunk", type=int, default=0, help="fixed blocks per launch (skips the autotuner with --lz-grid)")
+    ap.add_argument("--profile-only", action="store_true")
+    ap.add_argument("--profile-buffers", type=int, default=1500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); frames are independent "
@@ -104,6 +140,10 @@ def main():
     lib = L.lib()
     g = G.GPUCompressor(0)
     log("device", g.DeviceName, g.Arch, g.SMCount)
+
+    if args.profile_only:
+        profile_only(args, g, lib, L, D, G)
+        return
 
     t = time.perf_counter()
     arena, offs, sizes = D.batch(args.buffers, 1024, 8 << 20, args.seed)
@@ -135,7 +175,11 @@ def main():
     nf = ctypes.c_uint64()
 
     tuned = (ctypes.c_int32 * 3)()
-    if not args.no_tune:
+    if args.lz_grid > 0 and args.chunk > 0:
+        lib.przm_gpu_set_tuning(g.handle, args.lz_grid, args.chunk)
+        tuned[0], tuned[1] = args.lz_grid, args.chunk
+        log(f"fixed tuning lz_grid={args.lz_grid} chunk={args.chunk}")
+    elif not args.no_tune:
         t = time.perf_counter()
         rc = lib.przm_gpu_autotune(g.handle, d_src, spans, nbuf, args.level, tuned)
         if rc:
@@ -157,7 +201,7 @@ def main():
     for _ in range(args.steps):
         p = L.GpuProfile()
         batched(p)
-        kms.append(p.kernel_ms)
+        kms.append(p.kernel_ms)  # HIP events around the graph launch
     lib.przm_gpu_sync(g.handle)
     wall = time.perf_counter() - t0
     ms_step = wall * 1000 / args.steps
@@ -175,7 +219,7 @@ def main():
     stage_names = ["check", "lz", "hist", "rans", "frame_size", "scan", "frame_write"]
     stages = {k: round(float(v), 3) for k, v in zip(stage_names, prof.stage_ms[:7])}
     n_lz = int(prof.pipelined_chunks)
-    log(f"stages (ms, summed over {n_lz} chunks): {stages}")
+    log(f"stages of the un-graphed pass (ms, summed over {n_lz} chunks): {stages}, device {prof.kernel_ms:.1f} ms")
     bat_frames = None
 
     # plain: one buffer at a time, one launch per stage
@@ -217,16 +261,17 @@ def main():
     # written once) per launch / average launch duration, from the profiled batched pass
     roofline = None
     if prof.stage_ms[1] > 0:
-        lz_ms = prof.stage_ms[1]
-        alg = prof.lz_in_bytes + prof.lz_stream_bytes
-        ach = alg / (lz_ms * 1e6)
+        launches = n_lz
+        avg_ms = float(prof.stage_ms[1]) / launches  # events around each LZ launch, un-graphed pass
+        alg = (prof.lz_in_bytes + prof.lz_stream_bytes) / max(n_lz, 1)  # per launch, same batch every step
+        ach = alg / (avg_ms * 1e6)
         tr = pmc_traffic()
+        traffic = int(tr["hbm_bytes_per_block"] * nblocks / max(n_lz, 1)) if tr else None
         roofline = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBPS, 5),
-                    "traffic": tr.get("hbm_bytes_per_dispatch") if tr else None,
-                    "kernel": "przm_lz_kernel", "launches": n_lz,
-                    "avg_launch_ms": round(lz_ms / max(n_lz, 1), 3),
-                    "algorithmic_bytes_per_launch": int(alg / max(n_lz, 1))}
+                    "frac": round(ach / HBM_PEAK_GBPS, 5), "traffic": traffic,
+                    "kernel": "przm_lz_kernel", "launches": launches, "avg_launch_ms": round(avg_ms, 3),
+                    "algorithmic_bytes_per_launch": int(alg), "blocks_per_launch": round(nblocks / max(n_lz, 1), 1),
+                    "traffic_source": "profiles/lz_kernel_pmc.json" if tr else None}
 
     # 256 MiB Markov text, one buffer, one launch per stage
     plain256 = None
@@ -283,6 +328,7 @@ def main():
                    "level": args.level, "lz_grid": int(tuned[0]), "chunk_blocks": int(tuned[1])},
         "device_ms_per_step": round(float(np.mean(kms)), 3),
         "stages_ms": stages,
+        "ungraphed_device_ms": round(prof.kernel_ms, 3),
         "compression_ratio": round(total_in / total_frames, 4),
         "plain": plain,
         "batched_vs_plain": round(value / plain["value"], 3) if plain else None,

@@ -158,6 +158,9 @@ int32_t przm_gpu_malloc(przm_gpu *g, uint64_t bytes, void **d);
 int32_t przm_gpu_free(przm_gpu *g, void *d);
 int32_t przm_gpu_h2d(przm_gpu *g, void *d, const void *h, uint64_t bytes);
 int32_t przm_gpu_d2h(przm_gpu *g, void *h, const void *d, uint64_t bytes);
+/* Device-to-device copy of `bytes` (the przm_device_copy_kernel launches of
+ * gpu/bridge.go:382, 2628-2660 and gpu/kernel/przm_ans.cu:542-575). */
+int32_t przm_gpu_copy_d2d(przm_gpu *g, void *dst, const void *src, uint64_t bytes);
 int32_t przm_gpu_sync(przm_gpu *g);
 /* AllocPinnedHost / FreePinnedHost (gpu/memory.go:51-90). */
 int32_t przm_pinned_alloc(uint64_t bytes, void **p);

@@ -80,6 +80,7 @@ SIGNATURES = {
     "przm_gpu_free": (I32, [P, P]),
     "przm_gpu_h2d": (I32, [P, P, P, U64]),
     "przm_gpu_d2h": (I32, [P, P, P, U64]),
+    "przm_gpu_copy_d2d": (I32, [P, P, P, U64]),
     "przm_gpu_sync": (I32, [P]),
     "przm_pinned_alloc": (I32, [U64, ctypes.POINTER(P)]),
     "przm_pinned_free": (I32, [P]),

@@ -102,6 +102,18 @@ __global__ void przm_zero_kernel(uint64_t *p, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
+/* device-to-device copy: 16-byte vectors when both ends are 16-byte aligned, bytes otherwise */
+__global__ void przm_copy_kernel(const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, uint64_t n) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (uint64_t)gridDim.x * blockDim.x;
+    if ((((uintptr_t)src | (uintptr_t)dst) & 15u) == 0) {
+        const uint64_t n16 = n >> 4;
+        for (uint64_t i = t; i < n16; i += st) ((uint4 *)dst)[i] = ((const uint4 *)src)[i];
+        for (uint64_t i = (n16 << 4) + t; i < n; i += st) dst[i] = src[i];
+    } else {
+        for (uint64_t i = t; i < n; i += st) dst[i] = src[i];
+    }
+}
+
 /* desc[p] = (len << 16) | dist from the LZ kernel's dist output */
 __global__ void przm_desc_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks,
                                  uint32_t nblocks, const uint16_t *__restrict__ dist, uint32_t *__restrict__ desc,
@@ -251,6 +263,10 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         HIPCHK(hipGetLastError());
     }
     CHK(rec());
+    if (stage_ev && stage_ev->size() > evi) { /* a longer earlier sequence left spare events */
+        for (size_t k = evi; k < stage_ev->size(); k++) (void)hipEventDestroy((*stage_ev)[k]);
+        stage_ev->resize(evi);
+    }
     return 0;
 }
 
@@ -290,6 +306,7 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
     }
     std::vector<hipEvent_t> stage_ev;
     const bool want_stages = prof && !use_graph;
+    const bool want_bytes = prof && !use_graph; /* stream byte counts need a meta read-back */
     HIPCHK(hipEventRecord(c.ev[0], c.st));
     if (use_graph) {
         const bool hit = c.gexec && c.gkey.src == d_src && c.gkey.out == d_out && c.gkey.nblocks == nblocks &&
@@ -328,15 +345,18 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
         for (auto &b : blocks) in_bytes += b.len;
         prof->internal_vram_gbps = ms > 0 ? (double)(in_bytes + bo.total) / (ms * 1e6) : 0;
         for (int s = 0; s < 8; s++) prof->stage_ms[s] = 0;
-        if (want_stages && stage_ev.size() >= 2) {
+        /* per-stage times need events between the launches: un-graphed runs only (the events an
+         * event-record node of a captured graph records are not accepted by hipEventElapsedTime) */
+        const std::vector<hipEvent_t> &sev = stage_ev;
+        if (want_stages && sev.size() >= 2) {
             /* layout: [start][check+zero] then per chunk [lz][hist][rans][size][scan][write] ... [end] */
             float t;
-            HIPCHK(hipEventElapsedTime(&t, stage_ev[0], stage_ev[1]));
+            HIPCHK(hipEventElapsedTime(&t, sev[0], sev[1]));
             prof->stage_ms[0] += t;
             size_t i = 1;
-            while (i + 6 < stage_ev.size()) {
+            while (i + 6 < sev.size()) {
                 for (int s = 0; s < 6; s++) {
-                    HIPCHK(hipEventElapsedTime(&t, stage_ev[i + s], stage_ev[i + s + 1]));
+                    HIPCHK(hipEventElapsedTime(&t, sev[i + s], sev[i + s + 1]));
                     prof->stage_ms[1 + s] += t;
                 }
                 i += 6;
@@ -344,7 +364,7 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
         }
         for (auto e : stage_ev) (void)hipEventDestroy(e);
         prof->frame_bytes = bo.total;
-        if (want_stages) {
+        if (want_bytes) {
             std::vector<przm_meta_t> mt(nblocks);
             HIPCHK(hipMemcpy(mt.data(), c.meta.p, (size_t)nblocks * sizeof(przm_meta_t), hipMemcpyDeviceToHost));
             uint64_t lin = 0, lst = 0, raw = 0;
@@ -504,6 +524,20 @@ int32_t przm_gpu_h2d(przm_gpu *g, void *d, const void *h, uint64_t bytes) {
 int32_t przm_gpu_d2h(przm_gpu *g, void *h, const void *d, uint64_t bytes) {
     if (!g) return PRZM_GPU_ERR_ARG;
     HIPCHK(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+int32_t przm_gpu_copy_d2d(przm_gpu *g, void *dst, const void *src, uint64_t bytes) {
+    if (!g || ((!dst || !src) && bytes)) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPCHK(hipSetDevice(g->dev));
+    if (!bytes) return 0;
+    if (!g->c.st) HIPCHK(hipStreamCreateWithFlags(&g->c.st, hipStreamNonBlocking));
+    const uint64_t n16 = bytes >> 4;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n16 + 255) / 256 + 1, (uint64_t)g->cus * 8);
+    hipLaunchKernelGGL(przm_copy_kernel, dim3(grid), dim3(256), 0, g->c.st, (const uint8_t *)src, (uint8_t *)dst, bytes);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g->c.st));
     return 0;
 }
 int32_t przm_gpu_sync(przm_gpu *g) {

@@ -150,6 +150,30 @@ def test_full_size_markov_256mib_roundtrip(g):
     assert of == frames[: int(sizes[:64].sum())]
 
 
+def test_copy_d2d(g):
+    """przm_gpu_copy_d2d: aligned (vector) and unaligned (byte) copies of ragged sizes."""
+    import ctypes
+    from przm_amd import _lib as L
+    lib = L.lib()
+    rng = np.random.default_rng(8)
+    n = (8 << 20) + 77
+    src = rng.integers(0, 256, n, dtype=np.uint8)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert lib.przm_gpu_malloc(g.handle, n + 64, ctypes.byref(a)) == 0
+    assert lib.przm_gpu_malloc(g.handle, n + 64, ctypes.byref(b)) == 0
+    try:
+        lib.przm_gpu_h2d(g.handle, a, ctypes.c_void_p(src.ctypes.data), n)
+        for off_s, off_d, m in [(0, 0, n), (0, 0, 15), (3, 16, 1 << 20), (16, 5, 12345), (0, 0, 0)]:
+            out = np.zeros(m + 1, np.uint8)
+            assert lib.przm_gpu_copy_d2d(g.handle, ctypes.c_void_p(b.value + off_d), ctypes.c_void_p(a.value + off_s), m) == 0
+            if m:
+                lib.przm_gpu_d2h(g.handle, ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(b.value + off_d), m)
+                assert np.array_equal(out[:m], src[off_s: off_s + m]), (off_s, off_d, m)
+    finally:
+        lib.przm_gpu_free(g.handle, a)
+        lib.przm_gpu_free(g.handle, b)
+
+
 def test_mock_error_and_fallback(g):
     from przm_amd import gpu as G
     O = oracle()
