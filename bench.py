@@ -16,9 +16,10 @@ generator (JSON-like records, int32 tables, motif-random, uniform random),
   serving   open-loop Poisson arrivals at 2,000 req/s, sizes log-uniform in
             [4 KiB, 16 MiB], 64 GiB HBM admission budget: p50/p99 latency.
   roofline  the dominant kernel (przm_lz_kernel) against the 8 TB/s HBM
-            peak: its average launch time and its algorithmic bytes (block
-            bytes read once + token-stream bytes written once) come from an
-            un-graphed pass of the same batch right after the timed steps,
+            peak: algorithmic bytes per launch = (input bytes + frame bytes)
+            of the blocks one launch processes; its average launch time
+            comes from an un-graphed pass of the same batch right after the
+            timed steps,
             with HIP events between the launches on their stream (events
             recorded inside a captured graph cannot be timed on this ROCm);
             HBM traffic per block from the committed rocprofv3 --pmc summary.
@@ -48,13 +49,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(arena, offs, sizes, seconds):
-    """The C oracle (oracle/liboracle.so) on a time-bounded sample, all host cores."""
+def cpu_baseline(arena, offs, sizes, seconds, threads=None):
+    """The C oracle (oracle/liboracle.so) on a time-bounded sample, all host cores (or `threads`)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from concurrent.futures import ThreadPoolExecutor
     from oracle_lib import oracle
     O = oracle()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
     done_bytes = 0
     t0 = time.perf_counter()
     i = 0
@@ -263,7 +264,9 @@ def main():
     if prof.stage_ms[1] > 0:
         launches = n_lz
         avg_ms = float(prof.stage_ms[1]) / launches  # events around each LZ launch, un-graphed pass
-        alg = (prof.lz_in_bytes + prof.lz_stream_bytes) / max(n_lz, 1)  # per launch, same batch every step
+        # algorithmic bytes per unit: input bytes read once + frame bytes written once (intermediates
+        # stay on chip in the ideal pipeline); one launch processes nblocks / n_lz blocks
+        alg = (total_in + total_frames) / max(n_lz, 1)
         ach = alg / (avg_ms * 1e6)
         tr = pmc_traffic()
         traffic = int(tr["hbm_bytes_per_block"] * nblocks / max(n_lz, 1)) if tr else None
@@ -315,7 +318,22 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(arena, offs, sizes, args.cpu_seconds)
+        cpu1 = cpu_baseline(arena, offs, sizes, max(args.cpu_seconds / 3, 2.0), threads=1)
+        cpu["one_thread"] = {"value": cpu1["value"], "sample": cpu1["sample"]}
         log(f"cpu baseline: {cpu}")
+
+    # measured copy bandwidth (the STREAM-copy figure beside the 8 TB/s peak)
+    ncopy = 1 << 30
+    ca, cb = dmalloc(ncopy), dmalloc(ncopy)
+    lib.przm_gpu_copy_d2d(g.handle, cb, ca, ncopy)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        lib.przm_gpu_copy_d2d(g.handle, cb, ca, ncopy)
+    cw = (time.perf_counter() - t0) / 5
+    copy_gbps = round(2 * ncopy / cw / 1e9, 1)
+    lib.przm_gpu_free(g.handle, ca)
+    lib.przm_gpu_free(g.handle, cb)
+    log(f"copy kernel: {copy_gbps} GB/s (read + write)")
 
     line = {
         "metric": METRIC, "value": round(value, 4), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -335,6 +353,7 @@ def main():
         "plain_256MiB_markov": plain256,
         "serving": serving,
         "roofline": roofline,
+        "copy_kernel_gbps": copy_gbps,
         "cpu_baseline": cpu,
         "device": {"name": g.DeviceName, "arch": g.Arch, "cus": g.SMCount},
     }

@@ -266,119 +266,172 @@ __device__ __forceinline__ uint32_t u4get(const uint4 &v, uint32_t k) {
     return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
 
+#define RANS_STAGE 128u /* per-job LDS staging of emitted bytes: two 64-byte halves */
+
+/* reverse the byte order of a 16-byte word */
+__device__ __forceinline__ uint4 rev16(uint4 v) {
+    return make_uint4(__builtin_bswap32(v.w), __builtin_bswap32(v.z), __builtin_bswap32(v.y), __builtin_bswap32(v.x));
+}
+
 /*
  * The reverse 4-state pass of one stream by one lane quad (lane j = state j):
  * literals.go:305-325 / vans_meta.go:109-132.  Symbols arrive 16 at a time
  * (one 16-byte load per four steps, the next one in flight); src must be
- * 16-byte aligned and readable 16 bytes past n.  Emitted bytes go backward
- * from slot_end; at most cap bytes.  Returns the byte count (or overflow)
- * and the lane's final state.
+ * 16-byte aligned and readable 16 bytes past n.  tab is the job's table
+ * (freq | cdf << 16): V-ANS uses tab[sym], Meta8 tab[ctx * 256 + sym].
+ *
+ * Output byte k (k = 0, 1, ... in emission order) belongs at slot_end[-1-k].
+ * Bytes are first written to the job's LDS staging ring stg[k % 128]; each
+ * time a 64-byte run [64m, 64m + 64) is complete the quad stores it as four
+ * aligned 16-byte words (slot_end is 16-byte aligned), so the coder issues
+ * one global store per lane per 64 output bytes instead of byte stores.  At
+ * most cap bytes; returns the byte count (or overflow) and the lane's final
+ * state.
  */
 __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, uint32_t n, bool m8,
-                                              const uint32_t *vt, const uint32_t (*m8t)[256],
-                                              uint8_t *__restrict__ slot_end, uint32_t cap, uint32_t *state) {
+                                              const uint32_t *tab, uint8_t *stg, uint8_t *__restrict__ slot_end,
+                                              uint32_t cap, uint32_t gmax, uint32_t *state) {
+    /* every quad of the wave steps gmax times (the wave's longest job), quads
+     * with fewer steps idle through the first ones, so the wave-wide steps and
+     * barriers below run in uniform control flow */
     const uint32_t j = threadIdx.x & 3;
     const uint32_t groups = (n + 3) >> 2;
     uint32_t x = 65536u, K = 0;
     bool over = false;
     int64_t c = groups ? (int64_t)((groups - 1) >> 2) : -1;
     uint4 cur = ld16(src, c), nxt = ld16(src, c - 1);
-    for (uint32_t g = groups; g-- > 0;) {
-        if ((int64_t)(g >> 2) != c) { c--; cur = nxt; nxt = ld16(src, c - 1); }
+    for (uint32_t g = gmax; g-- > 0;) {
+        const bool live = g < groups && !over;
+        if (live && (int64_t)(g >> 2) != c) { c--; cur = nxt; nxt = ld16(src, c - 1); }
         const uint32_t i = 4 * g + j;
-        const bool v = i < n;
+        const bool v = live && i < n;
         uint32_t f = 1, cdf = 0;
         if (v) {
             const uint32_t dw = u4get(cur, g & 3u);
             const uint32_t sym = (dw >> (8 * j)) & 0xFFu;
-            uint32_t e;
+            uint32_t idx = sym;
             if (m8) {
                 uint32_t prev = 0;
-                bool has = i > 0;
                 if (j > 0) prev = (dw >> (8 * (j - 1))) & 0xFFu;
                 else if ((g & 3u) > 0) prev = u4get(cur, (g & 3u) - 1) >> 24;
                 else prev = nxt.w >> 24;
-                e = m8t[has ? ctx_meta(prev) : 0u][sym];
-            } else {
-                e = vt[sym];
+                idx += (i > 0 ? ctx_meta(prev) : 0u) << 8;
             }
+            const uint32_t e = tab[idx];
             f = e & 0xFFFFu;
             cdf = e >> 16;
         }
         const uint32_t t = 4096u * f - 1u;
-        uint32_t ne = 0, b0 = 0, b1 = 0;
-        if (v && x > t) {
-            b0 = x & 0xFFu; x >>= 8; ne = 1;
-            if (x > t) { b1 = x & 0xFFu; x >>= 8; ne = 2; }
+        uint32_t ne = 0, b0 = 0, b1 = 0, xn = x;
+        if (v && xn > t) {
+            b0 = xn & 0xFFu; xn >>= 8; ne = 1;
+            if (xn > t) { b1 = xn & 0xFFu; xn >>= 8; ne = 2; }
         }
         /* bytes of higher lanes come first: lane 3 emits before lane 2, ...
          * (quad permutes: lane j reads lanes j+1, j+2, j+3 and lane 0 of its quad) */
         const uint32_t e1 = QPERM(ne, 0xF9), e2 = QPERM(ne, 0xFE), e3 = QPERM(ne, 0xFF);
         const uint32_t after = (j < 3 ? e1 : 0u) + (j < 2 ? e2 : 0u) + (j < 1 ? e3 : 0u);
         const uint32_t tot = QPERM(ne + after, 0x00);
-        if (K + tot > cap) { over = true; break; }
-        if (ne) {
-            const uint64_t k0 = K + after;
-            slot_end[-1 - (int64_t)k0] = (uint8_t)b0;
-            if (ne == 2) slot_end[-2 - (int64_t)k0] = (uint8_t)b1;
+        if (live && K + tot > cap) over = true; /* the job is not chosen any more: stop it */
+        const bool emit = live && !over;
+        if (emit && ne) {
+            const uint32_t k0 = K + after;
+            stg[k0 & (RANS_STAGE - 1)] = (uint8_t)b0;
+            if (ne == 2) stg[(k0 + 1) & (RANS_STAGE - 1)] = (uint8_t)b1;
         }
-        K += tot;
-        if (v) {
+        const uint32_t K2 = emit ? K + tot : K;
+        __builtin_amdgcn_wave_barrier();
+        if ((K2 >> 6) != (K >> 6)) { /* the run [64m, 64m + 64) is complete: m = K >> 6 */
+            const uint32_t m = K >> 6;
+            /* lane j stores bytes k = 64m + 48 - 16j .. 64m + 63 - 16j, which sit at
+             * slot_end - 64(m + 1) + 16j + (63 - (k - 64m) - 16j) ascending */
+            const uint32_t k_lo = 64u * m + 48u - 16u * j;
+            const uint4 w = *(const uint4 *)(stg + (k_lo & (RANS_STAGE - 1)));
+            *(uint4 *)(slot_end - 64u * (m + 1u) + 16u * j) = rev16(w);
+        }
+        __builtin_amdgcn_wave_barrier();
+        K = K2;
+        if (v && emit) {
             uint32_t qq, rr;
-            divmod(x, f, &qq, &rr);
+            divmod(xn, f, &qq, &rr);
             x = qq * 4096u + cdf + rr;
         }
+    }
+    if (!over) { /* the last, partial run [64 (K >> 6), K) */
+        const uint32_t base = K & ~63u;
+        for (uint32_t k = base + j; k < K; k += 4) slot_end[-1 - (int64_t)k] = stg[k & (RANS_STAGE - 1)];
     }
     *state = x;
     return over ? PRZM_ENC_OVERFLOW : K;
 }
 
-/* one lane quad per (block, coder job); jobs kind-major so a wave's quads
- * code streams of the same kind (similar lengths) */
+/*
+ * One lane quad per (block, coder job).  Workgroups are kind-aligned: the
+ * first ceil(nblocks / 16) code the literal streams of 16 blocks each, the
+ * next the op streams with Meta8, and so on, so a workgroup's table is
+ * either sixteen per-job V-ANS tables or the one shared Meta8 table, both in
+ * LDS.  grid = PRZM_J_N * ceil(nblocks / RANS_JOBS).
+ */
 extern "C" __global__ void __launch_bounds__(RANS_NT)
 przm_rans_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const przm_ftab_t *__restrict__ ftab,
                  uint32_t nblocks) {
-    __shared__ uint32_t vtab[RANS_JOBS][256];
+    __shared__ union {
+        uint32_t vt[RANS_JOBS][256];
+        uint32_t m8[8][256];
+    } T;
+    __shared__ __align__(16) uint8_t stage[RANS_JOBS][RANS_STAGE];
     const uint32_t tid = threadIdx.x, q = tid >> 2, j = tid & 3;
-    const uint32_t job = blockIdx.x * RANS_JOBS + q;
-    const uint32_t kind = job / (nblocks ? nblocks : 1u), blk = job - kind * nblocks;
-    bool run = kind < PRZM_J_N;
+    const uint32_t per_kind = (nblocks + RANS_JOBS - 1) / RANS_JOBS;
+    const uint32_t kind = blockIdx.x / per_kind;
+    const uint32_t blk = (blockIdx.x - kind * per_kind) * RANS_JOBS + q;
+    if (kind >= PRZM_J_N) return;
+    const bool m8 = kind == PRZM_J_OP_M8 || kind == PRZM_J_META_M8 || kind == PRZM_J_VAR_M8;
+    bool run = blk < nblocks;
     const uint8_t *src = nullptr;
     uint8_t *slot_end = nullptr;
     uint32_t n = 0, tsel = 0;
-    int m8 = 0;
+    int jm8 = 0;
     uint8_t *wsb = ws + (uint64_t)blk * PRZM_WS_BYTES;
     if (run) {
-        job_layout(kind, wsb, meta + blk, &src, &n, &slot_end, &m8, &tsel);
+        job_layout(kind, wsb, meta + blk, &src, &n, &slot_end, &jm8, &tsel);
         const uint32_t minlen = kind == PRZM_J_LIT ? 1u : (m8 ? 24u : 16u);
         run = n >= minlen;
     }
-    if (run && !m8) {
+    if (m8) {
+        for (uint32_t e = tid; e < 8u * 256u; e += RANS_NT) (&T.m8[0][0])[e] = przm_m8.fc[e >> 8][e & 255u];
+    } else if (run) {
         const przm_ftab_t *ft = ftab + (uint64_t)blk * 4 + tsel;
-        for (uint32_t s = j; s < 256; s += 4) vtab[q][s] = ft->fc[s];
+        for (uint32_t s = j; s < 256; s += 4) T.vt[q][s] = ft->fc[s];
     }
     __syncthreads();
-    if (!run) {
-        if (kind < PRZM_J_N && j == 0) meta[blk].enc_len[kind] = PRZM_ENC_OVERFLOW;
-        return;
+    /* the wave's longest job sets the step count of every quad (idle quads have none) */
+    uint32_t gmax = run ? (n + 3) >> 2 : 0u;
+    for (int o = 32; o >= 4; o >>= 1) {
+        const uint32_t y = __shfl_xor(gmax, o);
+        gmax = y > gmax ? y : gmax;
     }
     uint32_t x;
-    const uint32_t K = rans_quad(src, n, m8 != 0, vtab[q], przm_m8.fc, slot_end, n, &x);
+    const uint32_t K = rans_quad(src, run ? n : 0u, m8, m8 ? &T.m8[0][0] : T.vt[q], stage[q], slot_end, n, gmax, &x);
+    if (!run) {
+        if (blk < nblocks && j == 0) meta[blk].enc_len[kind] = PRZM_ENC_OVERFLOW;
+        return;
+    }
     przm_meta_t *mt = meta + blk;
     if (j == 0) mt->enc_len[kind] = K;
     mt->enc_state[kind][j] = x;
 }
 
-/* a single V-ANS job over an arbitrary buffer (one quad) */
+/* a single V-ANS job over an arbitrary buffer (one quad); slot_end 16-byte aligned */
 extern "C" __global__ void __launch_bounds__(64)
 przm_rans1_kernel(const uint8_t *__restrict__ src, uint32_t n, const przm_ftab_t *__restrict__ ft,
                   uint8_t *__restrict__ slot_end, uint32_t cap, uint32_t *__restrict__ out) {
     __shared__ uint32_t vt[256];
+    __shared__ __align__(16) uint8_t stg[RANS_STAGE];
     for (uint32_t s = threadIdx.x; s < 256; s += 64) vt[s] = ft->fc[s];
     __syncthreads();
     if (threadIdx.x >= 4) return;
     uint32_t x;
-    const uint32_t K = rans_quad(src, n, false, vt, przm_m8.fc, slot_end, cap, &x);
+    const uint32_t K = rans_quad(src, n, false, vt, stg, slot_end, cap, (n + 3) >> 2, &x);
     if (threadIdx.x == 0) out[0] = K;
     out[1 + threadIdx.x] = x;
 }

@@ -250,7 +250,7 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         hipLaunchKernelGGL(przm_hist_kernel, dim3(nb), dim3(256), 0, st, (uint8_t *)c.ws.p, meta + c0,
                            (przm_ftab_t *)c.ftab.p, nb);
         CHK(rec());
-        hipLaunchKernelGGL(przm_rans_kernel, dim3((nb * PRZM_J_N + RANS_JOBS - 1) / RANS_JOBS), dim3(RANS_NT), 0, st,
+        hipLaunchKernelGGL(przm_rans_kernel, dim3(PRZM_J_N * ((nb + RANS_JOBS - 1) / RANS_JOBS)), dim3(RANS_NT), 0, st,
                            (uint8_t *)c.ws.p, meta + c0, (const przm_ftab_t *)c.ftab.p, nb);
         CHK(rec());
         hipLaunchKernelGGL(przm_frame_size_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, blocks + c0, meta + c0,
@@ -806,7 +806,7 @@ int32_t przm_gpu_vans_encode(przm_gpu *g, const uint8_t *src, uint64_t n, uint8_
     HIPCHK(hipSetDevice(g->dev));
     if (n == 0) { *out_len = 0; return 0; }
     Ctx &c = g->c;
-    const uint64_t slot = 2 * n + 64;
+    const uint64_t slot = (2 * n + 64 + 15) & ~15ull; /* the coder's slot end is 16-byte aligned */
     CHK(c.in.ensure(n + 64, &c.gen));
     CHK(c.out.ensure(slot, &c.gen));
     CHK(c.meta.ensure(4096 + sizeof(przm_ftab_t) + 64, &c.gen));
@@ -1002,6 +1002,7 @@ struct przm_server {
     przm_gpu *g = nullptr;
     int level = 6;
     uint64_t budget = 0, max_batch = 0;
+    uint32_t max_blocks = 0; /* blocks one launch may carry (device buffers are sized for it up front) */
     std::vector<std::unique_ptr<Slot>> slots;
     std::mutex mu;
     std::condition_variable cv_in, cv_done;
@@ -1031,7 +1032,20 @@ int32_t host_grow(uint8_t **p, uint64_t *cap, uint64_t need) {
     return 0;
 }
 
-/* launch one batch on a free slot (called with s->mu held) */
+/* device and pinned buffers of a slot for `nb` blocks over `arena` input bytes; growing only past
+ * the size fixed at creation (a single request larger than max_batch_bytes) */
+int32_t slot_reserve(przm_server *s, Slot &sl, uint64_t arena, uint32_t nb) {
+    przm_gpu *g = s->g;
+    const uint64_t bound = arena + 8ull * nb + 64;
+    CHK(host_grow(&sl.h_in, &sl.h_in_cap, arena + 64));
+    CHK(host_grow(&sl.h_out, &sl.h_out_cap, bound));
+    CHK(sl.c.in.ensure(arena + 64, &sl.c.gen));
+    CHK(sl.c.out.ensure(bound, &sl.c.gen));
+    CHK(ensure_ctx(sl.c, std::max<uint32_t>(nb, 1), std::max<uint32_t>(nb, 1), g->lz_grid, false));
+    return 0;
+}
+
+/* stage and launch the batch of a slot (worker thread, server lock not held) */
 int32_t launch_slot(przm_server *s, Slot &sl) {
     przm_gpu *g = s->g;
     uint64_t in_bytes = 0;
@@ -1045,15 +1059,11 @@ int32_t launch_slot(przm_server *s, Slot &sl) {
     build_blocks(sl.spans.data(), (uint32_t)sl.spans.size(), sl.blocks, sl.first);
     const uint32_t nb = (uint32_t)sl.blocks.size();
     const uint64_t bound = in_bytes + 8ull * nb + 64;
-    CHK(host_grow(&sl.h_in, &sl.h_in_cap, arena + 64));
-    CHK(host_grow(&sl.h_out, &sl.h_out_cap, bound));
+    CHK(slot_reserve(s, sl, arena, nb));
     for (size_t i = 0; i < sl.reqs.size(); i++)
         if (!sl.reqs[i]->src.empty()) std::memcpy(sl.h_in + sl.spans[i].offset, sl.reqs[i]->src.data(), sl.reqs[i]->src.size());
     Ctx &c = sl.c;
     const uint32_t chunk = std::max<uint32_t>(nb, 1);
-    CHK(c.in.ensure(arena + 64, &c.gen));
-    CHK(c.out.ensure(bound, &c.gen));
-    CHK(ensure_ctx(c, nb, chunk, g->lz_grid, false));
     HIPCHK(hipMemcpyAsync(c.in.p, sl.h_in, arena, hipMemcpyHostToDevice, c.st));
     HIPCHK(hipMemcpyAsync(c.blocks.p, sl.blocks.data(), nb * sizeof(przm_block_t), hipMemcpyHostToDevice, c.st));
     CHK(enqueue(g, c, (const uint8_t *)c.in.p, nb, (uint32_t)depth_of_level(s->level), (uint8_t *)c.out.p, true, false,
@@ -1067,22 +1077,30 @@ int32_t launch_slot(przm_server *s, Slot &sl) {
     return 0;
 }
 
-void finish_slot(przm_server *s, Slot &sl, int32_t rc) {
-    const double t = now_ms() - s->t0;
+/* split a finished launch's frames into its requests (worker thread, lock not held) */
+void split_slot(Slot &sl) {
     uint64_t pos = 0;
     for (size_t i = 0; i < sl.reqs.size(); i++) {
         Req &r = *sl.reqs[i];
-        if (rc) { r.status = rc; r.done_ms = t; continue; }
         uint64_t len = 0;
         r.fsz.clear();
         for (uint64_t b = sl.first[i]; b < sl.first[i + 1]; b++) { r.fsz.push_back(sl.flen[b]); len += sl.flen[b]; }
         r.out.assign(sl.h_out + pos, sl.h_out + pos + len);
         pos += len;
-        r.status = 0;
+    }
+}
+
+/* publish a finished (rc == 0) or failed launch (lock held) */
+void retire_slot(przm_server *s, Slot &sl, int32_t rc) {
+    const double t = now_ms() - s->t0;
+    for (auto &rp : sl.reqs) {
+        Req &r = *rp;
         r.done_ms = t;
+        if (rc) { r.status = rc; continue; }
+        r.status = 0;
         s->stats[1]++;
         s->stats[2] += r.src.size();
-        s->stats[3] += len;
+        s->stats[3] += r.out.size();
         std::vector<uint8_t>().swap(r.src);
     }
     s->admitted -= sl.dev_bytes;
@@ -1091,61 +1109,91 @@ void finish_slot(przm_server *s, Slot &sl, int32_t rc) {
     sl.busy = false;
 }
 
+/*
+ * The worker: retire launches whose completion event fired, then admit queued
+ * requests into a free slot (one launch per slot, up to `streams` in flight,
+ * each bounded by max_batch_bytes / max_blocks and all by the HBM budget).
+ * The lock only guards the queue and the request states: staging copies,
+ * launches and frame splitting run unlocked, so submitters never wait on them.
+ */
 void serve_loop(przm_server *s) {
     (void)hipSetDevice(s->g->dev);
-    std::unique_lock<std::mutex> lk(s->mu);
+    std::vector<Slot *> done_slots;
+    std::vector<std::pair<Slot *, int32_t>> failed;
     while (true) {
-        /* retire finished launches */
+        done_slots.clear();
         bool any_busy = false;
-        for (auto &up : s->slots) {
-            Slot &sl = *up;
-            if (!sl.busy) continue;
-            const hipError_t q = hipEventQuery(sl.done);
-            if (q == hipSuccess) finish_slot(s, sl, 0);
-            else if (q != hipErrorNotReady) finish_slot(s, sl, PRZM_GPU_ERR_HIP);
-            else any_busy = true;
-        }
-        s->cv_done.notify_all();
-        if (s->stop && s->pending.empty() && !any_busy) break;
-        /* admit: fill a free slot with queued requests under the batch and HBM bounds */
-        Slot *free_slot = nullptr;
-        for (auto &up : s->slots) if (!up->busy) { free_slot = up.get(); break; }
-        if (free_slot && !s->pending.empty()) {
-            uint64_t bytes = 0, nb = 0, cost = 0;
-            std::vector<std::shared_ptr<Req>> take;
-            while (!s->pending.empty()) {
-                auto &r = s->pending.front();
-                const uint64_t n = r->src.size();
-                const uint64_t rb = n ? (n + PRZM_BLOCK - 1) / PRZM_BLOCK : 1;
-                const uint64_t c2 = slot_cost(bytes + n, nb + rb);
-                if (!take.empty() && (bytes + n > s->max_batch || s->admitted + c2 > s->budget)) break;
-                if (take.empty() && s->admitted + c2 > s->budget && any_busy) break; /* wait for memory */
-                bytes += n; nb += rb; cost = c2;
-                take.push_back(r);
-                s->pending.pop_front();
+        Slot *launch = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            for (auto &up : s->slots) {
+                Slot &sl = *up;
+                if (!sl.busy) continue;
+                const hipError_t q = hipEventQuery(sl.done);
+                if (q == hipSuccess) done_slots.push_back(&sl);
+                else if (q != hipErrorNotReady) retire_slot(s, sl, PRZM_GPU_ERR_HIP);
+                else any_busy = true;
             }
-            if (!take.empty()) {
-                const double t = now_ms() - s->t0;
-                for (auto &r : take) r->start_ms = t;
-                free_slot->reqs = std::move(take);
-                free_slot->busy = true;
-                free_slot->dev_bytes = cost;
-                s->admitted += cost;
-                if (s->admitted > s->stats[4]) s->stats[4] = s->admitted;
-                if (free_slot->reqs.size() > s->stats[5]) s->stats[5] = free_slot->reqs.size();
-                s->stats[0]++;
-                if (any_busy) s->stats[6]++;
-                const int32_t rc = launch_slot(s, *free_slot);
-                if (rc) finish_slot(s, *free_slot, rc);
+            if (s->stop && s->pending.empty() && !any_busy && done_slots.empty()) break;
+            Slot *free_slot = nullptr;
+            for (auto &up : s->slots)
+                if (!up->busy) { free_slot = up.get(); break; }
+            if (free_slot && !s->pending.empty()) {
+                uint64_t bytes = 0, nb = 0, cost = 0;
+                std::vector<std::shared_ptr<Req>> take;
+                while (!s->pending.empty()) {
+                    auto &r = s->pending.front();
+                    const uint64_t n = r->src.size();
+                    const uint64_t rb = n ? (n + PRZM_BLOCK - 1) / PRZM_BLOCK : 1;
+                    const uint64_t c2 = slot_cost(bytes + n, nb + rb);
+                    if (!take.empty() && (bytes + n > s->max_batch || nb + rb > s->max_blocks ||
+                                          s->admitted + c2 > s->budget))
+                        break;
+                    if (take.empty() && s->admitted + c2 > s->budget && (any_busy || !done_slots.empty()))
+                        break; /* wait for memory */
+                    bytes += n;
+                    nb += rb;
+                    cost = c2;
+                    take.push_back(r);
+                    s->pending.pop_front();
+                }
+                if (!take.empty()) {
+                    const double t = now_ms() - s->t0;
+                    for (auto &r : take) r->start_ms = t;
+                    free_slot->reqs = std::move(take);
+                    free_slot->busy = true;
+                    free_slot->dev_bytes = cost;
+                    s->admitted += cost;
+                    if (s->admitted > s->stats[4]) s->stats[4] = s->admitted;
+                    if (free_slot->reqs.size() > s->stats[5]) s->stats[5] = free_slot->reqs.size();
+                    s->stats[0]++;
+                    if (any_busy) s->stats[6]++;
+                    launch = free_slot;
+                }
+            }
+            if (!launch && done_slots.empty()) {
+                if (any_busy) {
+                    lk.unlock();
+                    std::this_thread::sleep_for(std::chrono::microseconds(20));
+                } else {
+                    s->cv_in.wait_for(lk, std::chrono::milliseconds(2));
+                }
                 continue;
             }
         }
-        if (any_busy) {
-            lk.unlock();
-            std::this_thread::sleep_for(std::chrono::microseconds(50));
-            lk.lock();
-        } else {
-            s->cv_in.wait_for(lk, std::chrono::milliseconds(5));
+        if (launch) {
+            const int32_t rc = launch_slot(s, *launch);
+            if (rc) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                retire_slot(s, *launch, rc);
+                s->cv_done.notify_all();
+            }
+        }
+        if (!done_slots.empty()) {
+            for (Slot *sl : done_slots) split_slot(*sl);
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (Slot *sl : done_slots) retire_slot(s, *sl, 0);
+            s->cv_done.notify_all();
         }
     }
 }
@@ -1164,13 +1212,27 @@ int32_t przm_serve_create(przm_gpu *g, int32_t level, uint64_t hbm_budget, uint6
     s->budget = hbm_budget ? hbm_budget : (64ull << 30);
     s->max_batch = max_batch_bytes ? max_batch_bytes : (256ull << 20);
     s->t0 = now_ms();
+    /* a launch carries at most max_batch_bytes in up to max_blocks blocks (small requests take a
+     * block each); every slot is sized for that now, so the loop never allocates */
+    s->max_blocks = (uint32_t)std::min<uint64_t>(s->max_batch / PRZM_BLOCK + 1024, 65536);
+    const uint64_t arena = s->max_batch + 256ull * s->max_blocks;
     for (uint32_t i = 0; i < streams; i++) {
         auto sl = std::make_unique<Slot>();
         if (hipSetDevice(g->dev) != hipSuccess || hipStreamCreateWithFlags(&sl->c.st, hipStreamNonBlocking) != hipSuccess) {
             delete s;
             return PRZM_GPU_ERR_HIP;
         }
+        const int32_t rc = slot_reserve(s, *sl, arena, s->max_blocks);
         s->slots.push_back(std::move(sl));
+        if (rc) {
+            for (auto &x : s->slots) {
+                if (x->h_in) (void)hipHostFree(x->h_in);
+                if (x->h_out) (void)hipHostFree(x->h_out);
+                x->c.release();
+            }
+            delete s;
+            return rc;
+        }
     }
     s->worker = std::thread(serve_loop, s);
     *out = s;

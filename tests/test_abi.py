@@ -60,4 +60,14 @@ def test_host_only_entry_points():
         fo = oracle().normalize(cnt)
         assert np.array_equal(G.NormalizeFreq(cnt), fo)
         assert G.SerializeMatrix(fo) == oracle().matrix(fo)
-    assert G.PackMatch32(7, 300) and G.UnpackMatch32(G.PackMatch32(7, 300)) == (7, 300)
+
+
+def test_pack_match32_reference_kats():
+    """TestPackUnpackMatch, gpu/gpu_test.go:210-239."""
+    from przm_amd import gpu as G
+    for length, offset in [(4, 1), (10, 50), (258, 65535), (4095, 1048575)]:
+        packed = G.PackMatch32(length, offset)
+        assert G.IsMatch32(packed)
+        assert G.UnpackMatch32(packed) == (length, offset)
+    assert G.PackMatch32(3, 10) == 0
+    assert G.PackMatch32(10, 0) == 0
