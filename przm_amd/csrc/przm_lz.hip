@@ -41,7 +41,7 @@
 #define QP 16384                  /* parse tile: a quarter block */
 #define SEG 32                    /* speculative-walk segment */
 #define NSEG (QP / SEG)           /* 512: one per thread */
-#define WS_CAP 1024               /* bucket elements kept in LDS per wave */
+#define WS_CAP 1536               /* bucket elements kept in LDS per wave */
 #define RING 128                  /* staged search keys per wave: previous + current chunk */
 #define NONE16 0xFFFFu
 #define NONE32 0xFFFFFFFFu
@@ -74,8 +74,7 @@ struct LzShared {
         struct {
             uint16_t pos[LZ_NW][WS_CAP];
             uint16_t srt[LZ_NW][WS_CAP];
-            uint4 ringa[LZ_NW][RING]; /* bytes p..p+15 of a staged bucket element */
-            uint4 ringb[LZ_NW][RING]; /* bytes p+16..p+27, and p */
+            uint4 ring[LZ_NW][RING]; /* bytes p..p+11 of a staged bucket element, and p */
         } ws;
         struct {
             uint16_t dist[DSZ];
@@ -366,24 +365,15 @@ struct Scr {
     uint8_t *gws; /* per-wave workspaces, GWS_STRIDE bytes each */
 };
 
-typedef unsigned __int128 u128;
+/* index of the first set bit of a 64-bit word, 64 when zero */
+__device__ __forceinline__ uint32_t ctz64z(uint64_t x) { return x ? (uint32_t)__builtin_ctzll(x) : 64u; }
 
-/* common prefix (0..28) of two 28-byte keys given as the xor of their seven words */
-__device__ __forceinline__ uint32_t lcp28x(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, uint32_t x4,
-                                           uint32_t x5, uint32_t x6) {
-    const u128 v = ((u128)x3 << 96) | ((u128)x2 << 64) | ((u128)x1 << 32) | (u128)x0;
-    const u128 u = ((u128)1 << 96) | ((u128)x6 << 64) | ((u128)x5 << 32) | (u128)x4; /* sentinel at 28 bytes */
-    const uint32_t t0 = (uint32_t)__builtin_ctzg(v, 128), t1 = (uint32_t)__builtin_ctzg(u, 128);
-    return (v ? t0 : 128u + t1) >> 3;
-}
-
-/* byte masks of the first bt (1..28) bytes of a 28-byte key, one per word */
-__device__ __forceinline__ void prefix_masks(uint32_t bt, uint32_t mk[7]) {
-#pragma unroll
-    for (uint32_t k = 0; k < 7; k++) {
-        const uint32_t lo = 4u * k;
-        mk[k] = bt >= lo + 4u ? 0xFFFFFFFFu : bt <= lo ? 0u : 0xFFFFFFFFu >> (32u - 8u * (bt - lo));
-    }
+/* common prefix (0..12) of two 12-byte keys given as the xor of their three words, branch-free */
+__device__ __forceinline__ uint32_t lcp12x(uint32_t x0, uint32_t x1, uint32_t x2) {
+    const uint64_t lo = ((uint64_t)x1 << 32) | x0;
+    const uint32_t t_lo = ctz64z(lo);
+    const uint32_t t_hi = x2 ? (uint32_t)__builtin_ctz(x2) : 32u;
+    return (lo ? t_lo : 64u + t_hi) >> 3;
 }
 
 /* bytes p..p+15 as four little-endian words */
@@ -417,11 +407,12 @@ __device__ __forceinline__ uint32_t lcp_from(const uint8_t *in, uint4 pk, uint32
  * P, sorted permutation SRT and per-digit group ends GE.  Element i's chain is
  * the up to `depth` elements before it in its group (same 16-bit hash,
  * newest first, compress.go:681-707); the longest common prefix wins, ties
- * keep the nearest.  Each wave stages the first 28 bytes of the current and
- * the previous chunk of 64 elements in an LDS ring, so a candidate costs two
- * conflict-free 16-byte reads and no compare in the block unless its first
- * 28 bytes match.  Chains longer than the ring (depth > 64) read the rest
- * from the block directly.
+ * keep the nearest.  Each wave stages the first 12 bytes and the position of
+ * the current and the previous chunk of 64 elements in an LDS ring, so a
+ * candidate costs one conflict-free 16-byte read and a branch-free compare;
+ * only candidates whose 12 bytes all match (and whose next byte could beat
+ * the best) compare further in the block.  Chains longer than the ring
+ * (depth > 64) read the rest from the block directly.
  */
 template <bool DEEP, typename PT, typename ET>
 __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, uint32_t dgt, uint32_t n,
@@ -430,7 +421,7 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
     const bool prof = pt && threadIdx.x == 0;
     const uint8_t *in = S.in;
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-    uint4 *ra = S.big.ws.ringa[w], *rb = S.big.ws.ringb[w];
+    uint4 *ring = S.big.ws.ring[w];
     const uint32_t s0 = S.bstart[0][dgt], m0 = S.bstart[0][dgt + 1] - s0;
     const uint32_t s1 = S.bstart[1][dgt];
     for (uint32_t c0 = i0; c0 < i1; c0 += step) {
@@ -439,19 +430,17 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         const bool valid = i < i1;
         if (c0 >= 64 && (step != 64 || c0 == i0)) { /* previous chunk not staged by this wave */
             const uint32_t ip = c0 - 64 + lane, pp = P[SRT[ip]];
-            const uint4 b = ldw16(in, pp + 16u);
-            ra[ip & (RING - 1)] = ldw16(in, pp);
-            rb[ip & (RING - 1)] = make_uint4(b.x, b.y, b.z, pp);
+            const uint4 k = ldw16(in, pp);
+            ring[ip & (RING - 1)] = make_uint4(k.x, k.y, k.z, pp);
         }
         uint32_t j = 0, p = 0, kmax = 0, cap = 0;
-        uint4 ka = make_uint4(0, 0, 0, 0), kb = ka;
+        uint4 key = make_uint4(0, 0, 0, 0);
         if (valid) {
             j = SRT[i];
             p = P[j];
-            ka = ldw16(in, p);
-            kb = ldw16(in, p + 16u);
-            ra[i & (RING - 1)] = ka;
-            rb[i & (RING - 1)] = make_uint4(kb.x, kb.y, kb.z, p);
+            const uint4 k = ldw16(in, p);
+            key = make_uint4(k.x, k.y, k.z, p);
+            ring[i & (RING - 1)] = key;
             const uint32_t low = hash4(in, p) & 0xFFu;
             const uint32_t gs = low ? GE[low - 1] : 0u;
             cap = n - p < PRZM_MAX_MATCH ? n - p : PRZM_MAX_MATCH;
@@ -459,55 +448,43 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         }
         lds_fence();
         const uint64_t t_b = prof ? __builtin_amdgcn_s_memrealtime() : 0;
-        /* best so far and the masks of its first best + 1 bytes (at most 28): a
-         * candidate can only be longer if its key agrees on those bytes */
-        uint32_t best = 0, bd = 0, mk[7];
-        prefix_masks(1u, mk);
+        uint32_t best = 0, bd = 0;
         for (uint32_t k0 = 1; k0 <= depth; k0 += 4) {
             if (!__ballot(k0 <= kmax && best < cap)) break;
-            uint4 qa[4], qb[4];
+            uint4 qk[4];
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++) { /* four candidates in flight; entries past kmax are masked below */
                 const uint32_t k = k0 + u;
                 if (DEEP && k > lane + 64u) {
                     const uint32_t q = k <= kmax ? (uint32_t)P[SRT[i - k]] : p;
-                    const uint4 b = ldw16(in, q + 16u);
-                    qa[u] = ldw16(in, q);
-                    qb[u] = make_uint4(b.x, b.y, b.z, q);
+                    const uint4 b = ldw16(in, q);
+                    qk[u] = make_uint4(b.x, b.y, b.z, q);
                 } else {
-                    qa[u] = ra[(i - k) & (RING - 1)];
-                    qb[u] = rb[(i - k) & (RING - 1)];
+                    qk[u] = ring[(i - k) & (RING - 1)];
                 }
             }
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++) {
-                const uint32_t x0 = ka.x ^ qa[u].x, x1 = ka.y ^ qa[u].y, x2 = ka.z ^ qa[u].z, x3 = ka.w ^ qa[u].w;
-                const uint32_t x4 = kb.x ^ qb[u].x, x5 = kb.y ^ qb[u].y, x6 = kb.z ^ qb[u].z;
-                const uint32_t r = (x0 & mk[0]) | (x1 & mk[1]) | (x2 & mk[2]) | (x3 & mk[3]) | (x4 & mk[4]) |
-                                   (x5 & mk[5]) | (x6 & mk[6]);
-                if (r == 0u && k0 + u <= kmax && best < cap) {
-                    const uint32_t q = qb[u].w;
-                    uint32_t l = lcp28x(x0, x1, x2, x3, x4, x5, x6);
-                    if (l == 28u && cap > 28u && (best < 28u || in[p + best] == in[q + best]))
-                        l = 28u + lcp_from(in, ldw16(in, p + 28u), p + 28u, q + 28u, cap - 28u);
-                    l = l < cap ? l : cap;
-                    if (l > best) {
-                        best = l;
-                        bd = p - q;
-                        prefix_masks(best < 28u ? best + 1u : 28u, mk);
-                    }
-                }
+                const bool act = k0 + u <= kmax;
+                const uint32_t q = qk[u].w;
+                uint32_t l = lcp12x(key.x ^ qk[u].x, key.y ^ qk[u].y, key.z ^ qk[u].z);
+                if (l == 12u && act && best < cap && cap > 12u && (best < 12u || in[p + best] == in[q + best]))
+                    l = 12u + lcp_from(in, ldw16(in, p + 12u), p + 12u, q + 12u, cap - 12u);
+                l = l < cap ? l : cap;
+                const bool better = act && l > best;
+                best = better ? l : best;
+                bd = better ? p - q : bd;
             }
         }
         lds_fence();
         const uint64_t t_c = prof ? __builtin_amdgcn_s_memrealtime() : 0;
         if (valid) {
-        const bool m = best >= 4;
-        const uint32_t rv = p | ((m ? bd : 0u) << 16);
-        const uint16_t lv = (uint16_t)(m ? best : 0u);
-        if (j < m0) { sc.R0[s0 + j] = rv; sc.L0[s0 + j] = lv; }
-        else { sc.R1[s1 + j - m0] = rv; sc.L1[s1 + j - m0] = lv; }
-        if (p == PRZM_HALF) { S.st[ST_DISTB] = m ? bd : 0u; S.st[ST_LB] = m ? best : 0u; }
+            const bool m = best >= 4;
+            const uint32_t rv = p | ((m ? bd : 0u) << 16);
+            const uint16_t lv = (uint16_t)(m ? best : 0u);
+            if (j < m0) { sc.R0[s0 + j] = rv; sc.L0[s0 + j] = lv; }
+            else { sc.R1[s1 + j - m0] = rv; sc.L1[s1 + j - m0] = lv; }
+            if (p == PRZM_HALF) { S.st[ST_DISTB] = m ? bd : 0u; S.st[ST_LB] = m ? best : 0u; }
         }
         if (prof) {
             const uint64_t t_d = __builtin_amdgcn_s_memrealtime();
