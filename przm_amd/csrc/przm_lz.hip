@@ -43,6 +43,9 @@
 #define NSEG (QP / SEG)           /* 512: one per thread */
 #define WS_CAP 1536               /* bucket elements kept in LDS per wave */
 #define RING 128                  /* staged search keys per wave: previous + current chunk */
+#ifndef SEARCH_BATCH
+#define SEARCH_BATCH 4            /* chain candidates read per step of the search */
+#endif
 #define NONE16 0xFFFFu
 #define NONE32 0xFFFFFFFFu
 #define XCROSS 0x80000000u
@@ -102,6 +105,7 @@ struct LzShared {
     uint32_t wsum[LZ_NW];
     uint32_t st[ST_N + 3];
     uint32_t bigq[LZ_NW];
+    uint32_t nextb;
 };
 
 static_assert(sizeof(LzShared) <= 163840, "LDS budget");
@@ -449,11 +453,11 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         lds_fence();
         const uint64_t t_b = prof ? __builtin_amdgcn_s_memrealtime() : 0;
         uint32_t best = 0, bd = 0;
-        for (uint32_t k0 = 1; k0 <= depth; k0 += 4) {
+        for (uint32_t k0 = 1; k0 <= depth; k0 += SEARCH_BATCH) {
             if (!__ballot(k0 <= kmax && best < cap)) break;
-            uint4 qk[4];
+            uint4 qk[SEARCH_BATCH];
 #pragma unroll
-            for (uint32_t u = 0; u < 4; u++) { /* four candidates in flight; entries past kmax are masked below */
+            for (uint32_t u = 0; u < SEARCH_BATCH; u++) { /* candidates in flight; entries past kmax are masked below */
                 const uint32_t k = k0 + u;
                 if (DEEP && k > lane + 64u) {
                     const uint32_t q = k <= kmax ? (uint32_t)P[SRT[i - k]] : p;
@@ -464,7 +468,7 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
                 }
             }
 #pragma unroll
-            for (uint32_t u = 0; u < 4; u++) {
+            for (uint32_t u = 0; u < SEARCH_BATCH; u++) {
                 const bool act = k0 + u <= kmax;
                 const uint32_t q = qk[u].w;
                 uint32_t l = lcp12x(key.x ^ qk[u].x, key.y ^ qk[u].y, key.z ^ qk[u].z);
@@ -927,7 +931,7 @@ extern "C" __global__ void __launch_bounds__(LZ_NT)
 przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks, uint32_t nblocks,
                uint32_t depth, uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta,
                uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint16_t *__restrict__ dist_dbg,
-               uint64_t *__restrict__ phase_ts) {
+               uint64_t *__restrict__ phase_ts, uint32_t *__restrict__ next_block) {
     __shared__ LzShared S;
     const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
     uint8_t *my = scratch + (uint64_t)blockIdx.x * scratch_stride;
@@ -941,7 +945,14 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
     sc.gws = (uint8_t *)(sc.L1 + PRZM_HALF);
     uint16_t *T0w = (uint16_t *)my, *T1w = T0w + PRZM_HALF;
 
-    for (uint32_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+    /* blocks are handed out dynamically (next_block starts at 0 for each launch),
+     * so a workgroup that drew cheap blocks takes more of them */
+    for (;;) {
+        if (tid == 0) S.nextb = atomicAdd(next_block, 1u);
+        __syncthreads();
+        const uint32_t blk = S.nextb;
+        __syncthreads();
+        if (blk >= nblocks) break;
         const przm_block_t bd = blocks[blk];
         const uint32_t n = bd.len;
         przm_meta_t *mt = meta + blk;

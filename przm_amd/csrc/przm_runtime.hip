@@ -201,7 +201,8 @@ int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg)
     CHK(c.ftab.ensure((size_t)ch * 4 * sizeof(przm_ftab_t), &c.gen));
     CHK(c.flen.ensure((size_t)std::max<uint32_t>(nblocks, 1) * 4, &c.gen));
     CHK(c.foff.ensure((size_t)std::max<uint32_t>(nblocks, 1) * 8, &c.gen));
-    CHK(c.run.ensure(64, &c.gen));
+    /* run: the running frame offset (u64) at 0, then one block counter (u32) per chunk at 64 */
+    CHK(c.run.ensure(64 + 4ull * ((std::max<uint32_t>(nblocks, 1) + ch - 1) / ch + 2), &c.gen));
     CHK(c.scratch.ensure((size_t)grid * LZ_SCRATCH_STRIDE, &c.gen));
     if (dbg) CHK(c.dbg.ensure((size_t)std::max<uint32_t>(nblocks, 1) * PRZM_BLOCK * 2, &c.gen));
     if (!c.st) HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
@@ -234,7 +235,8 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
     };
     CHK(rec());
     if (check) hipLaunchKernelGGL(przm_check_kernel, dim3(nblocks), dim3(64), 0, st, d_src, blocks, nblocks);
-    hipLaunchKernelGGL(przm_zero_kernel, dim3(1), dim3(64), 0, st, run, 1u);
+    const uint32_t nchunks = (nblocks + chunk - 1) / chunk;
+    hipLaunchKernelGGL(przm_zero_kernel, dim3(1), dim3(64), 0, st, run, 8u + (nchunks + 2u) / 2u);
     HIPCHK(hipGetLastError());
     for (uint32_t c0 = 0; c0 < nblocks; c0 += chunk) {
         const uint32_t nb = std::min(chunk, nblocks - c0);
@@ -243,7 +245,8 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         hipLaunchKernelGGL(przm_lz_kernel, dim3(gz), dim3(LZ_NT), 0, st, d_src, blocks + c0, nb, depth,
                            (uint8_t *)c.ws.p, meta + c0, (uint8_t *)c.scratch.p, (uint64_t)LZ_SCRATCH_STRIDE,
                            dbg ? (uint16_t *)c.dbg.p + (uint64_t)c0 * PRZM_BLOCK : (uint16_t *)nullptr,
-                           g_phase_ts ? g_phase_ts + (uint64_t)c0 * 32 : (uint64_t *)nullptr);
+                           g_phase_ts ? g_phase_ts + (uint64_t)c0 * 32 : (uint64_t *)nullptr,
+                           (uint32_t *)((uint8_t *)c.run.p + 64) + c0 / chunk);
         HIPCHK(hipGetLastError());
         if (stop_after_lz) continue;
         CHK(rec());
