@@ -432,11 +432,18 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         const uint64_t t_a = prof ? __builtin_amdgcn_s_memrealtime() : 0;
         const uint32_t i = c0 + lane;
         const bool valid = i < i1;
-        if (c0 >= 64 && (step != 64 || c0 == i0)) { /* previous chunk not staged by this wave */
-            const uint32_t ip = c0 - 64 + lane, pp = P[SRT[ip]];
-            const uint4 k = ldw16(in, pp);
-            ring[ip & (RING - 1)] = make_uint4(k.x, k.y, k.z, pp);
+        /* ring[0, 64): the previous chunk, ring[64, 128): this chunk, so candidate k of
+         * lane l is entry 64 + l - k (k <= 64 + l) */
+        if (c0 >= 64) {
+            if (step == 64 && c0 != i0) {
+                ring[lane] = ring[64 + lane]; /* this wave staged it last time */
+            } else {
+                const uint32_t ip = c0 - 64 + lane, pp = P[SRT[ip]];
+                const uint4 k = ldw16(in, pp);
+                ring[lane] = make_uint4(k.x, k.y, k.z, pp);
+            }
         }
+        lds_fence();
         uint32_t j = 0, p = 0, kmax = 0, cap = 0;
         uint4 key = make_uint4(0, 0, 0, 0);
         if (valid) {
@@ -444,7 +451,7 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
             p = P[j];
             const uint4 k = ldw16(in, p);
             key = make_uint4(k.x, k.y, k.z, p);
-            ring[i & (RING - 1)] = key;
+            ring[64 + lane] = key;
             const uint32_t low = hash4(in, p) & 0xFFu;
             const uint32_t gs = low ? GE[low - 1] : 0u;
             cap = n - p < PRZM_MAX_MATCH ? n - p : PRZM_MAX_MATCH;
@@ -456,28 +463,46 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         for (uint32_t k0 = 1; k0 <= depth; k0 += SEARCH_BATCH) {
             if (!__ballot(k0 <= kmax && best < cap)) break;
             uint4 qk[SEARCH_BATCH];
+            uint32_t l[SEARCH_BATCH];
+            /* candidates k0 .. k0 + SEARCH_BATCH - 1 sit below one another in the ring */
+            const uint4 *rq = ring + (64 + (int32_t)lane - (int32_t)k0 - (int32_t)(SEARCH_BATCH - 1));
+            bool ext = false;
 #pragma unroll
-            for (uint32_t u = 0; u < SEARCH_BATCH; u++) { /* candidates in flight; entries past kmax are masked below */
+            for (uint32_t u = 0; u < SEARCH_BATCH; u++) { /* in flight together; entries past kmax are masked */
                 const uint32_t k = k0 + u;
                 if (DEEP && k > lane + 64u) {
                     const uint32_t q = k <= kmax ? (uint32_t)P[SRT[i - k]] : p;
                     const uint4 b = ldw16(in, q);
                     qk[u] = make_uint4(b.x, b.y, b.z, q);
                 } else {
-                    qk[u] = ring[(i - k) & (RING - 1)];
+                    qk[u] = rq[SEARCH_BATCH - 1 - u];
                 }
+                l[u] = lcp12x(key.x ^ qk[u].x, key.y ^ qk[u].y, key.z ^ qk[u].z);
+                ext |= l[u] == 12u && k <= kmax;
             }
+            if (__ballot(ext && cap > 12u)) {
+                /* some candidate agrees on all 12 key bytes: in order, extend those that
+                 * could still beat the best (their next byte must match) */
 #pragma unroll
-            for (uint32_t u = 0; u < SEARCH_BATCH; u++) {
-                const bool act = k0 + u <= kmax;
-                const uint32_t q = qk[u].w;
-                uint32_t l = lcp12x(key.x ^ qk[u].x, key.y ^ qk[u].y, key.z ^ qk[u].z);
-                if (l == 12u && act && best < cap && cap > 12u && (best < 12u || in[p + best] == in[q + best]))
-                    l = 12u + lcp_from(in, ldw16(in, p + 12u), p + 12u, q + 12u, cap - 12u);
-                l = l < cap ? l : cap;
-                const bool better = act && l > best;
-                best = better ? l : best;
-                bd = better ? p - q : bd;
+                for (uint32_t u = 0; u < SEARCH_BATCH; u++) {
+                    const bool act = k0 + u <= kmax;
+                    const uint32_t q = qk[u].w;
+                    uint32_t lu = l[u];
+                    if (lu == 12u && act && best < cap && cap > 12u && (best < 12u || in[p + best] == in[q + best]))
+                        lu = 12u + lcp_from(in, ldw16(in, p + 12u), p + 12u, q + 12u, cap - 12u);
+                    lu = lu < cap ? lu : cap;
+                    const bool better = act && lu > best;
+                    best = better ? lu : best;
+                    bd = better ? p - q : bd;
+                }
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < SEARCH_BATCH; u++) {
+                    const uint32_t lu = l[u] < cap ? l[u] : cap;
+                    const bool better = k0 + u <= kmax && lu > best;
+                    best = better ? lu : best;
+                    bd = better ? p - qk[u].w : bd;
+                }
             }
         }
         lds_fence();

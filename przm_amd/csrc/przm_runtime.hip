@@ -931,16 +931,16 @@ int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs,
                           int32_t tuned[3]) {
     if (!g || !bufs || !nbufs) return PRZM_GPU_ERR_ARG;
     CHK(mock_check());
-    /* sample: buffers from the front up to ~256 MiB */
+    /* sample: buffers from the front up to ~1 GiB (several chunks of the larger shapes) */
     uint32_t ns = 0;
     uint64_t bytes = 0;
-    while (ns < nbufs && bytes < (256ull << 20)) bytes += bufs[ns++].len;
+    while (ns < nbufs && bytes < (1ull << 30)) bytes += bufs[ns++].len;
     uint64_t bound = 0;
     for (uint32_t i = 0; i < ns; i++) bound += przm_gpu_frames_bound(bufs[i].len);
     void *d_out = nullptr;
     CHK(przm_gpu_malloc(g, bound + 64, &d_out));
     const int grids[] = {g->cus, 2 * g->cus};
-    const int chunks[] = {2048, 4096, 8192};
+    const int chunks[] = {4096, 8192, 16384};
     double best = 1e30;
     int bg = g->lz_grid, bc = (int)g->chunk;
     int32_t rc = 0;
