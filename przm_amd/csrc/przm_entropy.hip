@@ -291,42 +291,51 @@ __device__ __forceinline__ uint4 rev16(uint4 v) {
 __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, uint32_t n, bool m8,
                                               const uint32_t *tab, uint8_t *stg, uint8_t *__restrict__ slot_end,
                                               uint32_t cap, uint32_t gmax, uint32_t *state) {
-    /* every quad of the wave steps gmax times (the wave's longest job), quads
-     * with fewer steps idle through the first ones, so the wave-wide steps and
-     * barriers below run in uniform control flow */
+    /* Every quad of the wave takes the same G4 = gmax rounded up to four steps
+     * (gmax: the wave's longest job in 4-symbol groups), so the wave-wide steps
+     * below run in uniform control flow; a quad is live only for its own
+     * groups.  The 16-byte chunk of symbols of steps g = 4c + 3 .. 4c sits in
+     * cur with the current step's word in cur.w (the words rotate down each
+     * step), the next lower chunk in nxt. */
     const uint32_t j = threadIdx.x & 3;
     const uint32_t groups = (n + 3) >> 2;
+    const uint32_t G4 = (gmax + 3u) & ~3u;
+    const int64_t own = (int64_t)((n + 15u) >> 4); /* 16-byte chunks of this job's stream */
     uint32_t x = 65536u, K = 0;
     bool over = false;
-    int64_t c = groups ? (int64_t)((groups - 1) >> 2) : -1;
-    uint4 cur = ld16(src, c), nxt = ld16(src, c - 1);
-    for (uint32_t g = gmax; g-- > 0;) {
+    int64_t c = (int64_t)(G4 >> 2) - 1;
+    /* chunks outside the job's own stream read as zero (a job shorter than the
+     * wave's longest, or an idle quad with no stream, never loads past its end) */
+    uint4 cur = c < own ? ld16(src, c) : make_uint4(0, 0, 0, 0);
+    uint4 nxt = c - 1 < own ? ld16(src, c - 1) : make_uint4(0, 0, 0, 0);
+    uint8_t *const dummy = stg + RANS_STAGE + j; /* the target of stores that do not happen */
+    for (uint32_t g = G4; g-- > 0;) {
+        if ((g & 3u) == 3u && g != G4 - 1u) {
+            c--;
+            cur = nxt;
+            nxt = c - 1 < own ? ld16(src, c - 1) : make_uint4(0, 0, 0, 0);
+        }
+        const uint32_t dw = cur.w;
+        const uint32_t lower = (g & 3u) ? cur.z : nxt.w; /* the word of the next step down */
+        cur.w = cur.z;
+        cur.z = cur.y;
+        cur.y = cur.x;
         const bool live = g < groups && !over;
-        if (live && (int64_t)(g >> 2) != c) { c--; cur = nxt; nxt = ld16(src, c - 1); }
         const uint32_t i = 4 * g + j;
         const bool v = live && i < n;
-        uint32_t f = 1, cdf = 0;
-        if (v) {
-            const uint32_t dw = u4get(cur, g & 3u);
-            const uint32_t sym = (dw >> (8 * j)) & 0xFFu;
-            uint32_t idx = sym;
-            if (m8) {
-                uint32_t prev = 0;
-                if (j > 0) prev = (dw >> (8 * (j - 1))) & 0xFFu;
-                else if ((g & 3u) > 0) prev = u4get(cur, (g & 3u) - 1) >> 24;
-                else prev = nxt.w >> 24;
-                idx += (i > 0 ? ctx_meta(prev) : 0u) << 8;
-            }
-            const uint32_t e = tab[idx];
-            f = e & 0xFFFFu;
-            cdf = e >> 16;
+        uint32_t idx = (dw >> (8 * j)) & 0xFFu;
+        if (m8) {
+            const uint32_t prev = j ? (dw >> (8 * j - 8)) & 0xFFu : lower >> 24;
+            idx += (i > 0 ? ctx_meta(prev) : 0u) << 8;
         }
+        const uint32_t e = tab[v ? idx : 0u];
+        const uint32_t f = v ? (e & 0xFFFFu) : 1u, cdf = e >> 16;
         const uint32_t t = 4096u * f - 1u;
-        uint32_t ne = 0, b0 = 0, b1 = 0, xn = x;
-        if (v && xn > t) {
-            b0 = xn & 0xFFu; xn >>= 8; ne = 1;
-            if (xn > t) { b1 = xn & 0xFFu; xn >>= 8; ne = 2; }
-        }
+        const bool r1 = v && x > t;
+        const uint32_t x1 = r1 ? x >> 8 : x;
+        const bool r2 = r1 && x1 > t;
+        const uint32_t xn = r2 ? x1 >> 8 : x1;
+        const uint32_t ne = (uint32_t)r1 + (uint32_t)r2;
         /* bytes of higher lanes come first: lane 3 emits before lane 2, ...
          * (quad permutes: lane j reads lanes j+1, j+2, j+3 and lane 0 of its quad) */
         const uint32_t e1 = QPERM(ne, 0xF9), e2 = QPERM(ne, 0xFE), e3 = QPERM(ne, 0xFF);
@@ -334,11 +343,9 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
         const uint32_t tot = QPERM(ne + after, 0x00);
         if (live && K + tot > cap) over = true; /* the job is not chosen any more: stop it */
         const bool emit = live && !over;
-        if (emit && ne) {
-            const uint32_t k0 = K + after;
-            stg[k0 & (RANS_STAGE - 1)] = (uint8_t)b0;
-            if (ne == 2) stg[(k0 + 1) & (RANS_STAGE - 1)] = (uint8_t)b1;
-        }
+        const uint32_t k0 = K + after;
+        *(emit && r1 ? stg + (k0 & (RANS_STAGE - 1)) : dummy) = (uint8_t)x;
+        *(emit && r2 ? stg + ((k0 + 1) & (RANS_STAGE - 1)) : dummy) = (uint8_t)x1;
         const uint32_t K2 = emit ? K + tot : K;
         __builtin_amdgcn_wave_barrier();
         if ((K2 >> 6) != (K >> 6)) { /* the run [64m, 64m + 64) is complete: m = K >> 6 */
@@ -351,11 +358,9 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
         }
         __builtin_amdgcn_wave_barrier();
         K = K2;
-        if (v && emit) {
-            uint32_t qq, rr;
-            divmod(xn, f, &qq, &rr);
-            x = qq * 4096u + cdf + rr;
-        }
+        uint32_t qq, rr;
+        divmod(xn, f, &qq, &rr);
+        x = v && emit ? qq * 4096u + cdf + rr : x;
     }
     if (!over) { /* the last, partial run [64 (K >> 6), K) */
         const uint32_t base = K & ~63u;
@@ -379,7 +384,7 @@ przm_rans_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const
         uint32_t vt[RANS_JOBS][256];
         uint32_t m8[8][256];
     } T;
-    __shared__ __align__(16) uint8_t stage[RANS_JOBS][RANS_STAGE];
+    __shared__ __align__(16) uint8_t stage[RANS_JOBS][RANS_STAGE + 16]; /* + a dummy byte per lane */
     const uint32_t tid = threadIdx.x, q = tid >> 2, j = tid & 3;
     const uint32_t per_kind = (nblocks + RANS_JOBS - 1) / RANS_JOBS;
     const uint32_t kind = blockIdx.x / per_kind;
@@ -426,7 +431,7 @@ extern "C" __global__ void __launch_bounds__(64)
 przm_rans1_kernel(const uint8_t *__restrict__ src, uint32_t n, const przm_ftab_t *__restrict__ ft,
                   uint8_t *__restrict__ slot_end, uint32_t cap, uint32_t *__restrict__ out) {
     __shared__ uint32_t vt[256];
-    __shared__ __align__(16) uint8_t stg[RANS_STAGE];
+    __shared__ __align__(16) uint8_t stg[RANS_STAGE + 16];
     for (uint32_t s = threadIdx.x; s < 256; s += 64) vt[s] = ft->fc[s];
     __syncthreads();
     if (threadIdx.x >= 4) return;
