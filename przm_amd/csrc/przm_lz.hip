@@ -67,7 +67,7 @@ enum { T_LIT = 0, T_RLE = 1, T_MATCH = 2 };
 enum {
     ST_ENTRY = 0, ST_LITSTART, ST_R0, ST_R1, ST_R2,
     ST_CMD, ST_OFF, ST_LIT, ST_OP, ST_META, ST_VAR, ST_TOK,
-    ST_BUCKET, ST_DISTB, ST_LB, ST_NMATCH, ST_N
+    ST_BUCKET, ST_DISTB, ST_LB, ST_NMATCH, ST_E0, ST_BAD, ST_N
 };
 
 struct LzShared {
@@ -660,84 +660,68 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
     }
     QTS(0);
 
-    /* 2. the true path (wave 0): orbit of the entry segment under
-     *    segment -> segment-of-exit by pointer doubling; entries that are off
-     *    their segment's speculative path are re-walked (merging into the old
-     *    path where they meet) and the tokens crossing segment ends with an
-     *    unknown length are resolved, until the path is consistent. */
-    if (w == 0) {
-        uint32_t e0 = S.st[ST_ENTRY];
+    /* 2. the true path: orbit of the entry segment under segment ->
+     *    segment-of-exit, by pointer doubling with one thread per segment;
+     *    entries that are off their segment's speculative path are re-walked
+     *    (merging into the old path where they meet) and the tokens crossing
+     *    segment ends with an unknown length are resolved (wave 0, wave-wide
+     *    compares), until the path is consistent. */
+    if (tid == 0) S.st[ST_E0] = S.st[ST_ENTRY];
+    __syncthreads();
+    {
+        const uint32_t sg = s;
         uint32_t iters = 0;
         for (;;) {
             iters++;
+            const uint32_t e0 = S.st[ST_E0];
             const uint32_t s0 = (e0 - a) / SEG;
-            for (uint32_t k = 0; k < 8; k++) {
-                const uint32_t sg = lane * 8 + k;
-                if (sg < s0 || sg >= nseg) continue;
+            const bool in = sg >= s0 && sg < nseg;
+            if (in) {
                 const uint32_t x = S.sm.ps.xpos[sg];
                 const uint32_t nx = (x & XCROSS) ? NX_STOP : (x >= b ? NX_END : (x - a) / SEG);
                 S.sm.ps.nxt[sg] = (uint16_t)nx;
                 S.sm.ps.jmp[sg] = (uint16_t)nx;
                 S.sm.ps.vis[sg] = sg == s0;
             }
-            lds_fence();
-            for (int r = 0; r < 9; r++) {
-                uint32_t jn[8];
-                for (uint32_t k = 0; k < 8; k++) {
-                    const uint32_t sg = lane * 8 + k;
-                    jn[k] = NX_END;
-                    if (sg < s0 || sg >= nseg) continue;
+            if (tid == 0) S.st[ST_BAD] = NSEG;
+            __syncthreads();
+            for (int r = 0; r < 9; r++) { /* after round r every segment within 2^(r+1) - 1 steps is marked */
+                uint32_t jn = NX_END;
+                if (in) {
                     const uint32_t jm = S.sm.ps.jmp[sg];
                     if (jm < NSEG) {
                         if (S.sm.ps.vis[sg]) S.sm.ps.vis[jm] = 1;
-                        jn[k] = S.sm.ps.jmp[jm];
-                    } else jn[k] = jm;
+                        jn = S.sm.ps.jmp[jm];
+                    } else jn = jm;
                 }
-                lds_fence();
-                for (uint32_t k = 0; k < 8; k++) {
-                    const uint32_t sg = lane * 8 + k;
-                    if (sg >= s0 && sg < nseg) S.sm.ps.jmp[sg] = (uint16_t)jn[k];
+                __syncthreads();
+                if (in) S.sm.ps.jmp[sg] = (uint16_t)jn;
+                __syncthreads();
+            }
+            const bool vis = in && S.sm.ps.vis[sg];
+            const uint32_t nx = in ? S.sm.ps.nxt[sg] : NX_END;
+            if (vis && nx < NSEG) S.sm.ps.prv[nx] = (uint16_t)sg;
+            __syncthreads();
+            bool ok = false;
+            if (in) {
+                if (!vis) S.sm.ps.ent[sg] = NONE32;
+                else {
+                    const uint32_t e = sg == s0 ? e0 : S.sm.ps.xpos[S.sm.ps.prv[sg]];
+                    S.sm.ps.ent[sg] = e;
+                    ok = (S.sm.ps.bmp[sg] >> (e - (a + sg * SEG))) & 1u;
+                    if (!ok || nx == NX_STOP) atomicMin(&S.st[ST_BAD], sg);
                 }
-                lds_fence();
             }
-            for (uint32_t k = 0; k < 8; k++) {
-                const uint32_t sg = lane * 8 + k;
-                if (sg < s0 || sg >= nseg) continue;
-                const uint32_t nx = S.sm.ps.nxt[sg];
-                if (S.sm.ps.vis[sg] && nx < NSEG) S.sm.ps.prv[nx] = (uint16_t)sg;
-            }
-            lds_fence();
-            uint32_t bad = NSEG;
-            uint32_t okm = 0; /* bit k: segment lane*8+k is visited and consistent */
-            for (uint32_t k = 0; k < 8; k++) {
-                const uint32_t sg = lane * 8 + k;
-                if (sg < s0 || sg >= nseg) continue;
-                if (!S.sm.ps.vis[sg]) { S.sm.ps.ent[sg] = NONE32; continue; }
-                const uint32_t e = sg == s0 ? e0 : S.sm.ps.xpos[S.sm.ps.prv[sg]];
-                S.sm.ps.ent[sg] = e;
-                const bool ok = (S.sm.ps.bmp[sg] >> (e - (a + sg * SEG))) & 1u;
-                if (ok) okm |= 1u << k;
-                if ((!ok || S.sm.ps.nxt[sg] == NX_STOP) && sg < bad) bad = sg;
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                const uint32_t y = __shfl_xor(bad, o);
-                bad = y < bad ? y : bad;
-            }
-            lds_fence();
+            __syncthreads();
+            const uint32_t bad = S.st[ST_BAD];
             if (bad == NSEG) {
                 /* consistent: the quarter's exit is the exit of the visited segment leaving it */
-                for (uint32_t k = 0; k < 8; k++) {
-                    const uint32_t sg = lane * 8 + k;
-                    if (sg >= s0 && sg < nseg && S.sm.ps.vis[sg] && S.sm.ps.nxt[sg] == NX_END)
-                        S.st[ST_ENTRY] = S.sm.ps.xpos[sg];
-                }
-                if (pts && lane == 0) pts[10] = iters;
+                if (vis && nx == NX_END) S.st[ST_ENTRY] = S.sm.ps.xpos[sg];
+                if (pts && tid == 0) pts[10] = iters;
                 break;
             }
-            /* re-walk every visited segment whose entry is off its path */
-            for (uint32_t k = 0; k < 8; k++) {
-                const uint32_t sg = lane * 8 + k;
-                if (sg < bad || sg >= nseg || !S.sm.ps.vis[sg] || (okm >> k) & 1u) continue;
+            /* re-walk every visited segment from the first problem on whose entry is off its path */
+            if (vis && !ok && sg >= bad) {
                 const uint32_t sgs = a + sg * SEG, sge = sgs + SEG < b ? sgs + SEG : b;
                 uint32_t p = S.sm.ps.ent[sg], nb = 0;
                 const uint32_t old = S.sm.ps.bmp[sg];
@@ -752,21 +736,23 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
                 }
                 if (!done) { S.sm.ps.xpos[sg] = p; S.sm.ps.bmp[sg] = nb; }
             }
-            lds_fence();
+            __syncthreads();
             /* the first problem segment, if consistent now, may still end in an unresolved token */
-            const uint32_t eb = S.sm.ps.ent[bad];
-            const uint32_t xb = S.sm.ps.xpos[bad];
-            const bool okb = (S.sm.ps.bmp[bad] >> (eb - (a + bad * SEG))) & 1u;
-            if (okb && (xb & XCROSS)) {
-                const uint32_t tp = xb & ~XCROSS;
-                const Tok t = eval_tok<true>(S, tp, a, n, NONE32);
-                if (lane == 0) {
-                    S.sm.ps.xlen[bad] = t.len | (t.kind << 30);
-                    S.sm.ps.xpos[bad] = tp + t.len;
+            if (w == 0) {
+                const uint32_t eb = S.sm.ps.ent[bad];
+                const uint32_t xb = S.sm.ps.xpos[bad];
+                const bool okb = (S.sm.ps.bmp[bad] >> (eb - (a + bad * SEG))) & 1u;
+                if (okb && (xb & XCROSS)) {
+                    const uint32_t tp = xb & ~XCROSS;
+                    const Tok t = eval_tok<true>(S, tp, a, n, NONE32);
+                    if (lane == 0) {
+                        S.sm.ps.xlen[bad] = t.len | (t.kind << 30);
+                        S.sm.ps.xpos[bad] = tp + t.len;
+                    }
                 }
-                lds_fence();
+                if (lane == 0) S.st[ST_E0] = eb;
             }
-            e0 = eb;
+            __syncthreads();
         }
     }
     __syncthreads();
