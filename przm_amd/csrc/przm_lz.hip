@@ -286,13 +286,15 @@ template <bool COOP>
 __device__ Tok eval_tok(const LzShared &S, uint32_t p, uint32_t a, uint32_t n, uint32_t lim) {
     const uint8_t *in = S.in;
     const uint32_t i = p - a;
-    const uint32_t d = dget(S, i);
+    /* everything a position's decision reads, loaded together (one LDS round trip
+     * instead of a chain); entries i + 1 <= the tile end and bytes p..p+3 are in range */
+    const uint32_t d = dget(S, i), lc = lget(S, i), d1r = dget(S, i + 1), l1c = lget(S, i + 1);
+    const uint32_t w4 = ld32(in, p);
     if (!d) return {T_LIT, 1};
-    const uint32_t L = desc_len<COOP>(S, p, i, d, n);
+    const uint32_t L = lc < 255 ? lc : desc_len<COOP>(S, p, i, d, n);
     if (p + 1 < n) { /* lookahead, :196-216 */
-        const uint32_t d1 = dget(S, i + 1);
+        const uint32_t d1 = d1r;
         if (d1) {
-            const uint32_t l1c = lget(S, i + 1);
             uint32_t l1 = l1c;
             if (l1c == 255 && L + 1 >= 255) l1 = desc_len<COOP>(S, p + 1, i + 1, d1, n);
             if (l1 > L + 1) return {T_LIT, 1};
@@ -300,7 +302,7 @@ __device__ Tok eval_tok(const LzShared &S, uint32_t p, uint32_t a, uint32_t n, u
     }
     const uint32_t room = lim - p;
     if (p + 12 <= n) { /* run-length, :218-286 */
-        const uint32_t w4 = ld32(in, p), b0 = w4 & 0xFFu;
+        const uint32_t b0 = w4 & 0xFFu;
         if (w4 == b0 * 0x01010101u) {
             const uint32_t need = L > 12 ? L : 12;
             const uint32_t maxr = n - p < PRZM_RLE_CAP ? n - p : PRZM_RLE_CAP;
