@@ -118,6 +118,7 @@ def main():
     ap.add_argument("--level", type=int, default=6)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-batch-mib", type=int, default=64, help="serving: largest launch (MiB)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -308,7 +309,7 @@ def main():
     serving = None
     if not args.no_serve and args.serve_seconds > 0:
         pool = D.mixture(512 << 20, args.seed + 7)
-        srv = S.Server(g, level=args.level, hbm_budget=64 << 30, max_batch_bytes=256 << 20, streams=3)
+        srv = S.Server(g, level=args.level, hbm_budget=64 << 30, max_batch_bytes=args.serve_batch_mib << 20, streams=3)
         res = S.poisson_load(srv, pool, rate=2000.0, duration_s=args.serve_seconds)
         res["server"] = srv.stats()
         srv.close()

@@ -1048,6 +1048,29 @@ int32_t slot_reserve(przm_server *s, Slot &sl, uint64_t arena, uint32_t nb) {
     return 0;
 }
 
+/* host copies of a launch's requests (staging into pinned memory, splitting frames out) run on a
+ * few threads: one thread's memcpy would cap the server near the offered byte rate */
+void par_copy(const std::vector<std::pair<uint8_t *, const uint8_t *>> &dst_src, const std::vector<uint64_t> &len) {
+    uint64_t total = 0;
+    for (uint64_t l : len) total += l;
+    const unsigned nt = total < (8ull << 20) ? 1u : 4u;
+    auto work = [&](unsigned t) {
+        /* thread t copies the t-th quarter of the concatenated byte range */
+        const uint64_t lo = total * t / nt, hi = total * (t + 1) / nt;
+        uint64_t pos = 0;
+        for (size_t i = 0; i < len.size(); i++) {
+            const uint64_t a = std::max(pos, lo), b = std::min(pos + len[i], hi);
+            if (a < b) std::memcpy(dst_src[i].first + (a - pos), dst_src[i].second + (a - pos), b - a);
+            pos += len[i];
+        }
+    };
+    if (nt == 1) { work(0); return; }
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+}
+
 /* stage and launch the batch of a slot (worker thread, server lock not held) */
 int32_t launch_slot(przm_server *s, Slot &sl) {
     przm_gpu *g = s->g;
@@ -1063,8 +1086,15 @@ int32_t launch_slot(przm_server *s, Slot &sl) {
     const uint32_t nb = (uint32_t)sl.blocks.size();
     const uint64_t bound = in_bytes + 8ull * nb + 64;
     CHK(slot_reserve(s, sl, arena, nb));
-    for (size_t i = 0; i < sl.reqs.size(); i++)
-        if (!sl.reqs[i]->src.empty()) std::memcpy(sl.h_in + sl.spans[i].offset, sl.reqs[i]->src.data(), sl.reqs[i]->src.size());
+    {
+        std::vector<std::pair<uint8_t *, const uint8_t *>> ds;
+        std::vector<uint64_t> ln;
+        for (size_t i = 0; i < sl.reqs.size(); i++) {
+            ds.push_back({sl.h_in + sl.spans[i].offset, sl.reqs[i]->src.data()});
+            ln.push_back(sl.reqs[i]->src.size());
+        }
+        par_copy(ds, ln);
+    }
     Ctx &c = sl.c;
     const uint32_t chunk = std::max<uint32_t>(nb, 1);
     HIPCHK(hipMemcpyAsync(c.in.p, sl.h_in, arena, hipMemcpyHostToDevice, c.st));
@@ -1083,14 +1113,19 @@ int32_t launch_slot(przm_server *s, Slot &sl) {
 /* split a finished launch's frames into its requests (worker thread, lock not held) */
 void split_slot(Slot &sl) {
     uint64_t pos = 0;
+    std::vector<std::pair<uint8_t *, const uint8_t *>> ds;
+    std::vector<uint64_t> ln;
     for (size_t i = 0; i < sl.reqs.size(); i++) {
         Req &r = *sl.reqs[i];
         uint64_t len = 0;
         r.fsz.clear();
         for (uint64_t b = sl.first[i]; b < sl.first[i + 1]; b++) { r.fsz.push_back(sl.flen[b]); len += sl.flen[b]; }
-        r.out.assign(sl.h_out + pos, sl.h_out + pos + len);
+        r.out.resize(len);
+        ds.push_back({r.out.data(), sl.h_out + pos});
+        ln.push_back(len);
         pos += len;
     }
+    par_copy(ds, ln);
 }
 
 /* publish a finished (rc == 0) or failed launch (lock held) */
