@@ -366,8 +366,7 @@ __device__ __forceinline__ void lds_fence() {
 
 struct Scr {
     const uint16_t *T0, *T1;
-    uint32_t *R0, *R1;
-    uint16_t *L0, *L1;
+    uint32_t *D;      /* per position: match distance | min(length, 255) << 16 */
     uint8_t *gws; /* per-wave workspaces, GWS_STRIDE bytes each */
 };
 
@@ -428,8 +427,6 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
     const uint8_t *in = S.in;
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     uint4 *ring = S.big.ws.ring[w];
-    const uint32_t s0 = S.bstart[0][dgt], m0 = S.bstart[0][dgt + 1] - s0;
-    const uint32_t s1 = S.bstart[1][dgt];
     for (uint32_t c0 = i0; c0 < i1; c0 += step) {
         const uint64_t t_a = prof ? __builtin_amdgcn_s_memrealtime() : 0;
         const uint32_t i = c0 + lane;
@@ -509,13 +506,9 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         }
         lds_fence();
         const uint64_t t_c = prof ? __builtin_amdgcn_s_memrealtime() : 0;
-        if (valid) {
+        if (valid) { /* by position: dist | min(len, 255) << 16, 0 for no match */
             const bool m = best >= 4;
-            const uint32_t rv = p | ((m ? bd : 0u) << 16);
-            const uint16_t lv = (uint16_t)(m ? best : 0u);
-            if (j < m0) { sc.R0[s0 + j] = rv; sc.L0[s0 + j] = lv; }
-            else { sc.R1[s1 + j - m0] = rv; sc.L1[s1 + j - m0] = lv; }
-            if (p == PRZM_HALF) { S.st[ST_DISTB] = m ? bd : 0u; S.st[ST_LB] = m ? best : 0u; }
+            sc.D[p] = m ? bd | ((best < 255u ? best : 255u) << 16) : 0u;
         }
         if (prof) {
             const uint64_t t_d = __builtin_amdgcn_s_memrealtime();
@@ -951,11 +944,8 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
     Scr sc;
     sc.T0 = (const uint16_t *)my;
     sc.T1 = sc.T0 + PRZM_HALF;
-    sc.R0 = (uint32_t *)(my + 2u * PRZM_HALF * 2u);
-    sc.R1 = sc.R0 + PRZM_HALF;
-    sc.L0 = (uint16_t *)(sc.R1 + PRZM_HALF);
-    sc.L1 = sc.L0 + PRZM_HALF;
-    sc.gws = (uint8_t *)(sc.L1 + PRZM_HALF);
+    sc.D = (uint32_t *)(my + 2u * PRZM_HALF * 2u);
+    sc.gws = (uint8_t *)(my + 2u * PRZM_HALF * 2u + 2u * PRZM_HALF * 4u + 2u * PRZM_HALF * 2u);
     uint16_t *T0w = (uint16_t *)my, *T1w = T0w + PRZM_HALF;
 
     /* blocks are handed out dynamically (next_block starts at 0 for each launch),
@@ -1092,25 +1082,12 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             const uint32_t a = Q * QP;
             if (a >= n) break;
             const uint32_t b = n < a + QP ? n : a + QP;
-            const uint32_t H = Q >> 1;
-            for (uint32_t i = tid; i <= b - a; i += LZ_NT) { S.big.q.dist[DIX(i)] = 0; S.big.q.lc[LIX(i)] = 0; }
-            __syncthreads();
-            const uint32_t hs = S.bstart[H][256];
-            const uint32_t *R = H ? sc.R1 : sc.R0;
-            const uint16_t *RL = H ? sc.L1 : sc.L0;
-            for (uint32_t i = tid; i < hs; i += LZ_NT) {
-                const uint32_t rv = R[i];
-                const uint32_t p = rv & 0xFFFFu;
-                if (p >= a && p <= b) {
-                    const uint32_t l = RL[i];
-                    S.big.q.dist[DIX(p - a)] = (uint16_t)(rv >> 16);
-                    S.big.q.lc[LIX(p - a)] = (uint8_t)(l < 255 ? l : 255);
-                }
-            }
-            if (b == PRZM_HALF && b < n && tid == 0) {
-                const uint32_t l = S.st[ST_LB];
-                S.big.q.dist[DIX(b - a)] = (uint16_t)S.st[ST_DISTB];
-                S.big.q.lc[LIX(b - a)] = (uint8_t)(l < 255 ? l : 255);
+            /* the tile's match results [a, b] from the position-ordered results (positions
+             * past n - 4 have none) */
+            for (uint32_t i = tid; i <= b - a; i += LZ_NT) {
+                const uint32_t v = a + i < nh ? sc.D[a + i] : 0u;
+                S.big.q.dist[DIX(i)] = (uint16_t)v;
+                S.big.q.lc[LIX(i)] = (uint8_t)(v >> 16);
             }
             __syncthreads();
             if (dist_dbg)
