@@ -196,15 +196,20 @@ __device__ __forceinline__ uint32_t RUN(const uint8_t *in, uint32_t p, uint32_t 
 
 /* ---------------- wave / workgroup helpers ---------------- */
 
+/* lanes of the wave whose 8-bit digit d equals this lane's (valid lanes only): per bit, keep
+ * the lanes that agree with this lane's bit, i.e. m &= ~(ballot ^ t) with t = 0 or ~0 the bit
+ * sign-extended, one v_bitop3 per 32-bit half */
 __device__ __forceinline__ uint64_t peers8(uint32_t d, bool valid) {
-    uint64_t m = __ballot(valid);
+    const uint64_t v = __ballot(valid);
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bb = __ballot(bit);
-        m &= bit ? bb : ~bb;
+        const uint32_t t = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+        const uint64_t bb = __ballot(t != 0u);
+        lo &= ~((uint32_t)bb ^ t);
+        hi &= ~((uint32_t)(bb >> 32) ^ t);
     }
-    return m;
+    return ((uint64_t)hi << 32) | lo;
 }
 
 /* exclusive prefix sum over the workgroup; every thread calls it */
@@ -370,15 +375,24 @@ struct Scr {
     uint8_t *gws; /* per-wave workspaces, GWS_STRIDE bytes each */
 };
 
-/* index of the first set bit of a 64-bit word, 64 when zero */
-__device__ __forceinline__ uint32_t ctz64z(uint64_t x) { return x ? (uint32_t)__builtin_ctzll(x) : 64u; }
+/* index of the lowest set bit, 0xFFFFFFFF when x == 0: v_ffbl_b32 itself, so that the compiler
+ * does not add a zero test around it (the C form is for host-side compilation of this file) */
+__device__ __forceinline__ uint32_t ffbl(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    __asm__("v_ffbl_b32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+#else
+    return x ? (uint32_t)__builtin_ctz(x) : 0xFFFFFFFFu;
+#endif
+}
+__device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
-/* common prefix (0..12) of two 12-byte keys given as the xor of their three words, branch-free */
-__device__ __forceinline__ uint32_t lcp12x(uint32_t x0, uint32_t x1, uint32_t x2) {
-    const uint64_t lo = ((uint64_t)x1 << 32) | x0;
-    const uint32_t t_lo = ctz64z(lo);
-    const uint32_t t_hi = x2 ? (uint32_t)__builtin_ctz(x2) : 32u;
-    return (lo ? t_lo : 64u + t_hi) >> 3;
+/* Bit index of the first difference of two 12-byte keys given as the xor of their words,
+ * 0xFFFFFFFF when they are equal: the length of their common prefix is that index / 8.
+ * ffbl(x) < 32 for x != 0, so "| 32" adds 32 and keeps 0xFFFFFFFF for equal words. */
+__device__ __forceinline__ uint32_t diff12(uint32_t x0, uint32_t x1, uint32_t x2) {
+    return umin(umin(ffbl(x0), ffbl(x1) | 32u), ffbl(x2) | 64u);
 }
 
 /* bytes p..p+15 as four little-endian words */
@@ -389,22 +403,31 @@ __device__ __forceinline__ uint4 ldw16(const uint8_t *in, uint32_t p) {
                       __builtin_amdgcn_alignbyte(w3, w2, sh), __builtin_amdgcn_alignbyte(w4, w3, sh));
 }
 
-/* common prefix of in[p..] and in[q..], at most cap, 16 bytes per step; the
- * first 16 bytes of p are given */
-__device__ __forceinline__ uint32_t lcp_from(const uint8_t *in, uint4 pk, uint32_t p, uint32_t q, uint32_t cap) {
-    uint32_t l = 0;
-    for (;;) {
-        const uint4 qk = ldw16(in, q + l);
-        const uint32_t x0 = pk.x ^ qk.x, x1 = pk.y ^ qk.y, x2 = pk.z ^ qk.z, x3 = pk.w ^ qk.w;
-        uint32_t e = 16u;
-        if (x0) e = (uint32_t)__builtin_ctz(x0) >> 3;
-        else if (x1) e = 4u + ((uint32_t)__builtin_ctz(x1) >> 3);
-        else if (x2) e = 8u + ((uint32_t)__builtin_ctz(x2) >> 3);
-        else if (x3) e = 12u + ((uint32_t)__builtin_ctz(x3) >> 3);
+/* number of equal leading bytes (0..16) of two 16-byte strings, branch-free */
+__device__ __forceinline__ uint32_t eq16(uint4 a, uint4 b) {
+    const uint32_t t = umin(diff12(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z), ffbl(a.w ^ b.w) | 96u);
+    return umin(t >> 3, 16u);
+}
+
+/* Common prefix of in[p..] and in[q..] (q < p) known to agree on 12 bytes, at most cap (> 12).
+ * One LDS round trip per 16 bytes: the ten words of both sides are loaded before any is used. */
+__device__ __forceinline__ uint32_t lcp_ext(const uint8_t *in, uint32_t p, uint32_t q, uint32_t cap) {
+    uint32_t l = 12u, e;
+    do {
+        const uint32_t pa = p + l, pb = q + l;
+        const uint32_t *wa = (const uint32_t *)(in + (pa & ~3u)), *wb = (const uint32_t *)(in + (pb & ~3u));
+        const uint32_t a0 = wa[0], a1 = wa[1], a2 = wa[2], a3 = wa[3], a4 = wa[4];
+        const uint32_t b0 = wb[0], b1 = wb[1], b2 = wb[2], b3 = wb[3], b4 = wb[4];
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t sa = pa & 3u, sb = pb & 3u;
+        const uint4 a = make_uint4(__builtin_amdgcn_alignbyte(a1, a0, sa), __builtin_amdgcn_alignbyte(a2, a1, sa),
+                                   __builtin_amdgcn_alignbyte(a3, a2, sa), __builtin_amdgcn_alignbyte(a4, a3, sa));
+        const uint4 b = make_uint4(__builtin_amdgcn_alignbyte(b1, b0, sb), __builtin_amdgcn_alignbyte(b2, b1, sb),
+                                   __builtin_amdgcn_alignbyte(b3, b2, sb), __builtin_amdgcn_alignbyte(b4, b3, sb));
+        e = eq16(a, b);
         l += e;
-        if (e < 16u || l >= cap) return l < cap ? l : cap;
-        pk = ldw16(in, p + l);
-    }
+    } while (e == 16u && l < cap);
+    return umin(l, cap);
 }
 
 /*
@@ -412,14 +435,22 @@ __device__ __forceinline__ uint32_t lcp_from(const uint8_t *in, uint4 pk, uint32
  * P, sorted permutation SRT and per-digit group ends GE.  Element i's chain is
  * the up to `depth` elements before it in its group (same 16-bit hash,
  * newest first, compress.go:681-707); the longest common prefix wins, ties
- * keep the nearest.  Each wave stages the first 12 bytes and the position of
- * the current and the previous chunk of 64 elements in an LDS ring, so a
- * candidate costs one conflict-free 16-byte read and a branch-free compare;
- * only candidates whose 12 bytes all match (and whose next byte could beat
- * the best) compare further in the block.  Chains longer than the ring
- * (depth > 64) read the rest from the block directly.
+ * keep the nearest, and only lengths >= 4 count.
+ *
+ * Each wave stages the first 12 bytes and the position of the current and the
+ * previous chunk of 64 elements in an LDS ring, so a candidate costs one
+ * conflict-free 16-byte read and a branch-free compare.  Lengths are kept as
+ * bit indices of the first difference (8 * length + 0..7), so "longer" is one
+ * unsigned compare and the best so far is the threshold 8 * best + 7.
+ *
+ * No per-candidate range test: a ring entry before the element's group start
+ * is another element of the same bucket with a different 16-bit hash, whose
+ * first 4 bytes therefore differ (length <= 3, never taken), and the ring slots
+ * of the previous chunk hold, on the first chunk, a key whose hash lies in
+ * another bucket.  Chains longer than the ring (depth > 64) read the rest from
+ * the block, with the same filler key past the group start.
  */
-template <bool DEEP, typename PT, typename ET>
+template <bool DEEP, uint32_t B, typename PT, typename ET>
 __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, uint32_t dgt, uint32_t n,
                                                     uint32_t depth, const PT *P, const PT *SRT, const ET *GE,
                                                     uint32_t i0, uint32_t i1, uint32_t step, uint64_t *pt = nullptr) {
@@ -427,27 +458,27 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
     const uint8_t *in = S.in;
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     uint4 *ring = S.big.ws.ring[w];
+    /* a 4-byte word whose hash4 high byte is not dgt: hash4(0) = 0, hash4(1) = 0x1E35 */
+    const uint32_t fz = dgt == 0 ? 1u : 0u;
+    const uint4 filler = make_uint4(fz, fz, fz, 0u);
     for (uint32_t c0 = i0; c0 < i1; c0 += step) {
         const uint64_t t_a = prof ? __builtin_amdgcn_s_memrealtime() : 0;
         const uint32_t i = c0 + lane;
         const bool valid = i < i1;
         /* ring[0, 64): the previous chunk, ring[64, 128): this chunk, so candidate k of
          * lane l is entry 64 + l - k (k <= 64 + l) */
-        if (c0 >= 64) {
-            if (step == 64 && c0 != i0) {
-                ring[lane] = ring[64 + lane]; /* this wave staged it last time */
-            } else {
-                const uint32_t ip = c0 - 64 + lane, pp = P[SRT[ip]];
-                const uint4 k = ldw16(in, pp);
-                ring[lane] = make_uint4(k.x, k.y, k.z, pp);
-            }
+        if (c0 < 64) ring[lane] = filler;
+        else if (step == 64 && c0 != i0) ring[lane] = ring[64 + lane]; /* this wave staged it last time */
+        else {
+            const uint32_t ip = c0 - 64 + lane, pp = P[SRT[ip]];
+            const uint4 k = ldw16(in, pp);
+            ring[lane] = make_uint4(k.x, k.y, k.z, pp);
         }
         lds_fence();
-        uint32_t j = 0, p = 0, kmax = 0, cap = 0;
+        uint32_t p = 0, kmax = 0, cap = 0;
         uint4 key = make_uint4(0, 0, 0, 0);
         if (valid) {
-            j = SRT[i];
-            p = P[j];
+            p = P[SRT[i]];
             const uint4 k = ldw16(in, p);
             key = make_uint4(k.x, k.y, k.z, p);
             ring[64 + lane] = key;
@@ -458,57 +489,63 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         }
         lds_fence();
         const uint64_t t_b = prof ? __builtin_amdgcn_s_memrealtime() : 0;
-        uint32_t best = 0, bd = 0;
-        for (uint32_t k0 = 1; k0 <= depth; k0 += SEARCH_BATCH) {
-            if (!__ballot(k0 <= kmax && best < cap)) break;
-            uint4 qk[SEARCH_BATCH];
-            uint32_t l[SEARCH_BATCH];
-            /* candidates k0 .. k0 + SEARCH_BATCH - 1 sit below one another in the ring */
-            const uint4 *rq = ring + (64 + (int32_t)lane - (int32_t)k0 - (int32_t)(SEARCH_BATCH - 1));
-            bool ext = false;
+        /* thr = 8 * best + 7 with best = 3 (lengths below 4 do not count); capb caps a
+         * key compare at min(cap, 12) bytes; bit indices >= 96 mean all 12 key bytes agree */
+        uint32_t thr = 31u, bq = 0u;
+        const uint32_t capb = 8u * umin(cap, 12u) + 7u, capt = 8u * cap + 7u;
+        for (uint32_t k0 = 1; k0 <= depth; k0 += B) { /* depth is a multiple of B */
+            if (!__ballot(k0 <= kmax && thr < capt)) break;
+            uint4 qk[B];
+            uint32_t lb[B];
+            /* candidates k0 .. k0 + B - 1 sit below one another in the ring */
+            const uint4 *rq = ring + (64 + (int32_t)lane - (int32_t)k0 - (int32_t)(B - 1));
+            uint32_t mx = 0;
 #pragma unroll
-            for (uint32_t u = 0; u < SEARCH_BATCH; u++) { /* in flight together; entries past kmax are masked */
+            for (uint32_t u = 0; u < B; u++) { /* in flight together */
                 const uint32_t k = k0 + u;
                 if (DEEP && k > lane + 64u) {
-                    const uint32_t q = k <= kmax ? (uint32_t)P[SRT[i - k]] : p;
-                    const uint4 b = ldw16(in, q);
-                    qk[u] = make_uint4(b.x, b.y, b.z, q);
+                    if (k <= kmax) {
+                        const uint32_t q = (uint32_t)P[SRT[i - k]];
+                        const uint4 b = ldw16(in, q);
+                        qk[u] = make_uint4(b.x, b.y, b.z, q);
+                    } else {
+                        qk[u] = filler;
+                    }
                 } else {
-                    qk[u] = rq[SEARCH_BATCH - 1 - u];
+                    qk[u] = rq[B - 1 - u];
                 }
-                l[u] = lcp12x(key.x ^ qk[u].x, key.y ^ qk[u].y, key.z ^ qk[u].z);
-                ext |= l[u] == 12u && k <= kmax;
+                lb[u] = umin(diff12(key.x ^ qk[u].x, key.y ^ qk[u].y, key.z ^ qk[u].z), capb);
+                mx = mx > lb[u] ? mx : lb[u];
             }
-            if (__ballot(ext && cap > 12u)) {
-                /* some candidate agrees on all 12 key bytes: in order, extend those that
-                 * could still beat the best (their next byte must match) */
+            if (__ballot(mx >= 96u && cap > 12u)) {
+                /* some candidate agrees on all 12 key bytes: its exact length decides.  Once
+                 * the best reaches 28 bytes a candidate can only beat it if it agrees at byte
+                 * `best` (one byte compare before the full one). */
 #pragma unroll
-                for (uint32_t u = 0; u < SEARCH_BATCH; u++) {
-                    const bool act = k0 + u <= kmax;
-                    const uint32_t q = qk[u].w;
-                    uint32_t lu = l[u];
-                    if (lu == 12u && act && best < cap && cap > 12u && (best < 12u || in[p + best] == in[q + best]))
-                        lu = 12u + lcp_from(in, ldw16(in, p + 12u), p + 12u, q + 12u, cap - 12u);
-                    lu = lu < cap ? lu : cap;
-                    const bool better = act && lu > best;
-                    best = better ? lu : best;
-                    bd = better ? p - q : bd;
+                for (uint32_t u = 0; u < B; u++) {
+                    if (lb[u] >= 96u && cap > 12u) {
+                        const uint32_t q = qk[u].w, best = thr >> 3;
+                        if (best < cap && (best < 28u || in[p + best] == in[q + best]))
+                            lb[u] = 8u * lcp_ext(in, p, q, cap);
+                    }
+                    const bool better = lb[u] > thr;
+                    thr = better ? (lb[u] | 7u) : thr;
+                    bq = better ? qk[u].w : bq;
                 }
             } else {
 #pragma unroll
-                for (uint32_t u = 0; u < SEARCH_BATCH; u++) {
-                    const uint32_t lu = l[u] < cap ? l[u] : cap;
-                    const bool better = k0 + u <= kmax && lu > best;
-                    best = better ? lu : best;
-                    bd = better ? p - qk[u].w : bd;
+                for (uint32_t u = 0; u < B; u++) {
+                    const bool better = lb[u] > thr;
+                    thr = better ? (lb[u] | 7u) : thr;
+                    bq = better ? qk[u].w : bq;
                 }
             }
         }
         lds_fence();
         const uint64_t t_c = prof ? __builtin_amdgcn_s_memrealtime() : 0;
         if (valid) { /* by position: dist | min(len, 255) << 16, 0 for no match */
-            const bool m = best >= 4;
-            sc.D[p] = m ? bd | ((best < 255u ? best : 255u) << 16) : 0u;
+            const uint32_t best = thr >> 3;
+            sc.D[p] = best >= 4u ? (p - bq) | (umin(best, 255u) << 16) : 0u;
         }
         if (prof) {
             const uint64_t t_d = __builtin_amdgcn_s_memrealtime();
@@ -557,6 +594,7 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     }
     if (!LDSWS) __threadfence_block();
     lds_fence();
+    if (prof) pt[-2] += __builtin_amdgcn_s_memrealtime() - t_a;
     /* exclusive scan of the 256 digit counts (4 per lane) */
     {
         uint32_t c4[4], t = 0;
@@ -594,8 +632,9 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
         return true;
     }
     if (prof) pt[0] += t_c - t_a;
-    if (depth > 64u) bucket_search_range<true>(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64, pt);
-    else bucket_search_range<false>(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64, pt);
+    if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64, pt);
+    else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64, pt);
+    else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64, pt);
     return false;
 }
 
@@ -1000,6 +1039,7 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
                 if (p < end) atomicAdd(&S.sm.tab[w][hash4(S.in, p) >> 8], 1u);
             }
             __syncthreads();
+            if (H == 0) PTS(14);
             {
                 const uint32_t dg = tid < 256 ? tid : 0;
                 uint32_t tot = 0;
@@ -1019,6 +1059,7 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
                 if (tid == 0) S.bstart[H][256] = all;
             }
             __syncthreads();
+            if (H == 0) PTS(15);
             const uint64_t lt = (1ull << lane) - 1ull;
             for (uint32_t c = 0; c < PRZM_HALF / LZ_NW; c += 64) {
                 const uint32_t p = wb + c + lane;
@@ -1034,10 +1075,12 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
                 lds_fence();
             }
             __syncthreads();
+            if (H == 0) PTS(23);
             const uint32_t cnt = S.bstart[H][256];
             uint16_t *T = H ? T1w : T0w;
             for (uint32_t i = tid; i < cnt; i += LZ_NT) T[i] = S.big.stage[i];
             __syncthreads();
+            if (H == 0) PTS(24);
         }
         PTS(2);
 
@@ -1070,8 +1113,9 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             const uint16_t *P = (const uint16_t *)gw;
             const uint16_t *SRT = P + PRZM_BLOCK;
             const uint32_t *GE = (const uint32_t *)(gw + 2u * PRZM_BLOCK * 2u);
-            if (depth > 64u) bucket_search_range<true>(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
-            else bucket_search_range<false>(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
+            if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
+            else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
+            else bucket_search_range<false, SEARCH_BATCH>(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
         }
         __threadfence_block();
         __syncthreads();
