@@ -56,7 +56,7 @@
  * lanes of a wave walking consecutive segments hit different banks */
 #define DIX(i) ((i) + ((i) >> 5))
 #define LIX(i) ((i) + (((i) >> 5) << 2))
-#define DSZ (DIX(QP + 1) + 2)
+#define DSZ ((DIX(QP + 1) + 3) & ~1) /* even, so that lc below starts 4-byte aligned */
 #define LSZ (LIX(QP + 1) + 8)
 #define GWS_SLOT (2u * PRZM_BLOCK * 2u + 256u * 4u) /* one large bucket: positions, order, group ends */
 #define GWS_STRIDE (2u * GWS_SLOT)                   /* per wave: the queued bucket + one worked in place */
@@ -99,6 +99,7 @@ struct LzShared {
             uint8_t vis[NSEG];
             uint8_t ntk[NSEG];
             uint16_t tdist[QP / 4];
+            uint32_t hm[NSEG + 1]; /* per segment: positions with a match (bit k: position 32 s + k) */
         } ps;
     } sm;
     uint32_t bstart[2][257];
@@ -640,6 +641,27 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
 
 /* ---------------- parse of one tile ---------------- */
 
+/* bits [x, y) of a 32-bit mask, 0 <= x <= y <= 32 */
+__device__ __forceinline__ uint32_t bits_range(uint32_t x, uint32_t y) {
+    return (uint32_t)((1ull << y) - (1ull << x));
+}
+
+/* the first position at or after p (ss <= p < se) that is not a certain literal, or se */
+__device__ __forceinline__ uint32_t lit_run_end(uint32_t litm, uint32_t ss, uint32_t p, uint32_t se) {
+    const uint32_t nl = ~litm & (0xFFFFFFFFu << (p - ss));
+    const uint32_t q = nl ? ss + (uint32_t)__builtin_ctz(nl) : ss + 32u;
+    return q < se ? q : se;
+}
+
+/* the token at p, a position of segment ss that is not a certain literal: eval_tok where the
+ * class mask says it must decide, else a match of the stored length */
+__device__ __forceinline__ Tok seg_tok(const LzShared &S, uint32_t cxm, uint32_t ss, uint32_t p, uint32_t a,
+                                       uint32_t n, uint32_t lim) {
+    if ((cxm >> (p - ss)) & 1u) return eval_tok<false>(S, p, a, n, lim);
+    return {T_MATCH, lget(S, p - a)};
+}
+
+
 __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint8_t *__restrict__ wsb,
                            uint64_t *__restrict__ pts) {
 #define QTS(i) do { if (pts && threadIdx.x == 0) pts[i] = __builtin_amdgcn_s_memrealtime(); } while (0)
@@ -652,13 +674,54 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
     const uint32_t se = segv ? (ss + SEG < b ? ss + SEG : b) : ss;
     const uint32_t nseg = (b - a + SEG - 1) / SEG;
 
+    /* 0. classes of the segment's positions, from the decision inputs alone
+     *    (przm_tokenizer.cuh:171-320 as in eval_tok): litm = certainly a literal (no match,
+     *    or the lookahead prefers the next position); cxm = only eval_tok can decide (a
+     *    length of 255 or more, a lookahead that needs one, a possible byte run); any other
+     *    position is a match of its stored length.  The walks below then take a literal run
+     *    in one step. */
+    uint32_t litm = 0, cxm = 0;
+    if (segv) {
+        const uint32_t h0 = S.sm.ps.hm[s], h1 = S.sm.ps.hm[s + 1];
+        const uint32_t *lwp = (const uint32_t *)(S.big.q.lc + LIX(s * SEG));
+        const uint32_t *iwp = (const uint32_t *)(in + ss);
+        uint32_t lw[10], iw[9];
+#pragma unroll
+        for (int j = 0; j < 10; j++) lw[j] = lwp[j];
+#pragma unroll
+        for (int j = 0; j < 9; j++) iw[j] = iwp[j];
+#pragma unroll
+        for (uint32_t k = 0; k < SEG; k++) {
+            const uint32_t p = ss + k;
+            const uint32_t lc = (lw[k >> 2] >> (8u * (k & 3u))) & 0xFFu;
+            const uint32_t l1c = k + 1 < SEG ? (lw[(k + 1) >> 2] >> (8u * ((k + 1) & 3u))) & 0xFFu : lw[9] & 0xFFu;
+            const bool dk = (h0 >> k) & 1u;
+            const bool d1 = k + 1 < SEG ? (bool)((h0 >> (k + 1)) & 1u) : (bool)(h1 & 1u);
+            const uint32_t w4 = (k & 3u) ? __builtin_amdgcn_alignbyte(iw[(k >> 2) + 1], iw[k >> 2], k & 3u) : iw[k >> 2];
+            const bool la = p + 1 < n && d1;
+            const bool la_cx = la && l1c == 255u && lc >= 254u;
+            const bool la_lit = la && !la_cx && l1c > lc + 1u;
+            const bool run = p + 12 <= n && w4 == (w4 & 0xFFu) * 0x01010101u;
+            const bool lit = !dk || (lc != 255u && la_lit);
+            const bool cx = dk && (lc == 255u || (!la_lit && (la_cx || run)));
+            if (p < se) {
+                litm |= (uint32_t)lit << k;
+                cxm |= (uint32_t)cx << k;
+            }
+        }
+    }
+
     /* 1. speculative walk of every segment from its start */
     if (segv) {
         uint32_t bm = 0, p = ss, xp = 0;
         bool cross = false;
         while (p < se) {
+            const uint32_t q = lit_run_end(litm, ss, p, se);
+            bm |= bits_range(p - ss, q - ss);
+            p = q;
+            if (p >= se) break;
             bm |= 1u << (p - ss);
-            const Tok t = eval_tok<false>(S, p, a, n, se);
+            const Tok t = seg_tok(S, cxm, ss, p, a, n, se);
             if (t.kind == T_LIT) { p++; continue; }
             if (t.len == LEN_CROSS) { cross = true; xp = p; break; }
             p += t.len;
@@ -680,9 +743,18 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
                 uint32_t p = xp, nb = 0;
                 bool done = false;
                 while (p < se) {
+                    const uint32_t q = lit_run_end(litm, ss, p, se);
+                    const uint32_t run = bits_range(p - ss, q - ss), hit = old & run;
+                    if (hit) {
+                        nb |= bits_range(p - ss, (uint32_t)__builtin_ctz(hit));
+                        S.sm.ps.bmp[s] = old | nb; done = true; break;
+                    }
+                    nb |= run;
+                    p = q;
+                    if (p >= se) break;
                     if ((old >> (p - ss)) & 1u) { S.sm.ps.bmp[s] = old | nb; done = true; break; }
                     nb |= 1u << (p - ss);
-                    const Tok t = eval_tok<false>(S, p, a, n, se);
+                    const Tok t = seg_tok(S, cxm, ss, p, a, n, se);
                     if (t.kind == T_LIT) { p++; continue; }
                     if (t.len == LEN_CROSS) { S.sm.ps.xpos[s] = p | XCROSS; S.sm.ps.bmp[s] = nb; done = true; break; }
                     p += t.len;
@@ -761,9 +833,18 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
                 const uint32_t old = S.sm.ps.bmp[sg];
                 bool done = false;
                 while (p < sge) {
+                    const uint32_t q = lit_run_end(litm, sgs, p, sge);
+                    const uint32_t run = bits_range(p - sgs, q - sgs), hit = old & run;
+                    if (hit) {
+                        nb |= bits_range(p - sgs, (uint32_t)__builtin_ctz(hit));
+                        S.sm.ps.bmp[sg] = old | nb; done = true; break;
+                    }
+                    nb |= run;
+                    p = q;
+                    if (p >= sge) break;
                     if ((old >> (p - sgs)) & 1u) { S.sm.ps.bmp[sg] = old | nb; done = true; break; }
                     nb |= 1u << (p - sgs);
-                    const Tok t = eval_tok<false>(S, p, a, n, sge);
+                    const Tok t = seg_tok(S, cxm, sgs, p, a, n, sge);
                     if (t.kind == T_LIT) { p++; continue; }
                     if (t.len == LEN_CROSS) { S.sm.ps.xpos[sg] = p | XCROSS; S.sm.ps.bmp[sg] = nb; done = true; break; }
                     p += t.len;
@@ -799,7 +880,11 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
     if (live) {
         uint32_t p = t0;
         while (p < se) {
-            Tok t = eval_tok<false>(S, p, a, n, se);
+            const uint32_t q = lit_run_end(litm, ss, p, se);
+            nl += q - p;
+            p = q;
+            if (p >= se) break;
+            Tok t = seg_tok(S, cxm, ss, p, a, n, se);
             if (t.kind == T_LIT) { nl++; p++; continue; }
             if (t.len == LEN_CROSS) t.len = S.sm.ps.xlen[s] & 0x3FFFFFFFu;
             if (t.kind == T_RLE) nr++;
@@ -1128,10 +1213,17 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             const uint32_t b = n < a + QP ? n : a + QP;
             /* the tile's match results [a, b] from the position-ordered results (positions
              * past n - 4 have none) */
-            for (uint32_t i = tid; i <= b - a; i += LZ_NT) {
-                const uint32_t v = a + i < nh ? sc.D[a + i] : 0u;
-                S.big.q.dist[DIX(i)] = (uint16_t)v;
-                S.big.q.lc[LIX(i)] = (uint8_t)(v >> 16);
+            for (uint32_t i0 = 0; i0 <= b - a; i0 += LZ_NT) { /* the same trip count in every lane */
+                const uint32_t i = i0 + tid;
+                const bool act = i <= b - a;
+                const uint32_t v = act && a + i < nh ? sc.D[a + i] : 0u;
+                if (act) {
+                    S.big.q.dist[DIX(i)] = (uint16_t)v;
+                    S.big.q.lc[LIX(i)] = (uint8_t)(v >> 16);
+                }
+                /* lanes 0..31 and 32..63 of a wave cover one 32-position segment each */
+                const uint64_t hb = __ballot((v & 0xFFFFu) != 0u);
+                if (act && (lane & 31u) == 0) S.sm.ps.hm[i >> 5] = (uint32_t)(hb >> (lane & 32u));
             }
             __syncthreads();
             if (dist_dbg)
