@@ -46,6 +46,7 @@
 #ifndef SEARCH_BATCH
 #define SEARCH_BATCH 4            /* chain candidates read per step of the search */
 #endif
+#define WS_MAXV 5                 /* values per workgroup scan */
 #define NONE16 0xFFFFu
 #define NONE32 0xFFFFFFFFu
 #define XCROSS 0x80000000u
@@ -103,7 +104,7 @@ struct LzShared {
         } ps;
     } sm;
     uint32_t bstart[2][257];
-    uint32_t wsum[LZ_NW];
+    uint32_t wsum[LZ_NW][WS_MAXV];
     uint32_t st[ST_N + 3];
     uint32_t bigq[LZ_NW];
     uint32_t nextb;
@@ -213,46 +214,87 @@ __device__ __forceinline__ uint64_t peers8(uint32_t d, bool valid) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-/* exclusive prefix sum over the workgroup; every thread calls it */
-__device__ uint32_t wg_scan(LzShared &S, uint32_t v, uint32_t *total) {
-    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    if (lane == 63) S.wsum[w] = x;
-    __syncthreads();
-    uint32_t pre = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < LZ_NW; i++) {
-        const uint32_t t = S.wsum[i];
-        pre += (uint32_t)i < w ? t : 0u;
-        tot += t;
-    }
-    __syncthreads();
-    *total = tot;
-    return pre + x - v;
+/* DPP move of x with control CTRL on rows ROWMASK; lanes whose source is out of range, and
+ * rows outside the mask, read 0 */
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dppz(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWMASK, 0xF, false);
 }
 
-/* exclusive running maximum over the workgroup */
-__device__ uint32_t wg_scan_max(LzShared &S, uint32_t v) {
+/* inclusive wave scans: row_shr 1, 2, 4, 8 inside each row of 16 lanes, then row 0's last lane
+ * into row 1 and row 2's into row 3 (row_bcast:15), then lane 31 into rows 2 and 3
+ * (row_bcast:31) */
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x) {
+    x += dppz<0x111, 0xF>(x);
+    x += dppz<0x112, 0xF>(x);
+    x += dppz<0x114, 0xF>(x);
+    x += dppz<0x118, 0xF>(x);
+    x += dppz<0x142, 0xA>(x);
+    x += dppz<0x143, 0xC>(x);
+    return x;
+}
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t x) {
+    x = umax(x, dppz<0x111, 0xF>(x));
+    x = umax(x, dppz<0x112, 0xF>(x));
+    x = umax(x, dppz<0x114, 0xF>(x));
+    x = umax(x, dppz<0x118, 0xF>(x));
+    x = umax(x, dppz<0x142, 0xA>(x));
+    x = umax(x, dppz<0x143, 0xC>(x));
+    return x;
+}
+/* the value of the lane below (0 in lane 0): wave_shr:1 */
+__device__ __forceinline__ uint32_t wave_prev(uint32_t x) { return dppz<0x138, 0xF>(x); }
+
+/*
+ * Exclusive workgroup scans of NS sums (sv: value in, exclusive prefix out; stot: totals)
+ * and NM running maxima (mv: value in, exclusive running maximum out) in one pass: one
+ * barrier to publish the per-wave totals, one to release them.  Every thread calls it.
+ */
+template <int NS, int NM>
+__device__ __forceinline__ void wg_scan_multi(LzShared &S, uint32_t *sv, uint32_t *stot, uint32_t *mv) {
+    static_assert(NS + NM <= WS_MAXV, "wsum size");
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-    uint32_t x = v;
+    uint32_t xs[NS > 0 ? NS : 1], xm[NM > 0 ? NM : 1];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= (uint32_t)o) x = x > y ? x : y;
+    for (int k = 0; k < NS; k++) xs[k] = wave_incl_sum(sv[k]);
+#pragma unroll
+    for (int k = 0; k < NM; k++) xm[k] = wave_incl_max(mv[k]);
+    if (lane == 63) {
+#pragma unroll
+        for (int k = 0; k < NS; k++) S.wsum[w][k] = xs[k];
+#pragma unroll
+        for (int k = 0; k < NM; k++) S.wsum[w][NS + k] = xm[k];
     }
-    uint32_t ex = __shfl_up(x, 1);
-    if (lane == 0) ex = 0;
-    if (lane == 63) S.wsum[w] = x;
     __syncthreads();
-    uint32_t pre = 0;
-    for (uint32_t i = 0; i < w; i++) pre = pre > S.wsum[i] ? pre : S.wsum[i];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < LZ_NW; i++) {
+            const uint32_t t = S.wsum[i][k];
+            pre += (uint32_t)i < w ? t : 0u;
+            tot += t;
+        }
+        sv[k] = pre + xs[k] - sv[k];
+        stot[k] = tot;
+    }
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+        uint32_t pre = 0;
+#pragma unroll
+        for (int i = 0; i < LZ_NW; i++) pre = (uint32_t)i < w ? umax(pre, S.wsum[i][NS + k]) : pre;
+        mv[k] = umax(pre, wave_prev(xm[k]));
+    }
     __syncthreads();
-    return pre > ex ? pre : ex;
+}
+
+/* exclusive prefix sum over the workgroup; every thread calls it */
+__device__ __forceinline__ uint32_t wg_scan(LzShared &S, uint32_t v, uint32_t *total) {
+    uint32_t sv[1] = {v}, st[1];
+    wg_scan_multi<1, 0>(S, sv, st, nullptr);
+    *total = st[0];
+    return sv[0];
 }
 
 __device__ __forceinline__ uint32_t leb_len(uint32_t v) {
@@ -600,12 +642,7 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     {
         uint32_t c4[4], t = 0;
         for (int k = 0; k < 4; k++) { c4[k] = cnt[lane * 4 + k]; t += c4[k]; }
-        uint32_t x = t;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o);
-            if (lane >= (uint32_t)o) x += y;
-        }
-        uint32_t run = x - t;
+        uint32_t run = wave_incl_sum(t) - t;
         for (int k = 0; k < 4; k++) { cnt[lane * 4 + k] = run; run += c4[k]; }
     }
     lds_fence();
@@ -898,10 +935,12 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
         }
     }
     if (segv) S.sm.ps.ntk[s] = (uint8_t)nt;
-    uint32_t tot_l, tot_m;
-    const uint32_t lit_base = wg_scan(S, nl + nr, &tot_l) + S.st[ST_LIT];
-    const uint32_t m_base = wg_scan(S, nm, &tot_m);
-    uint32_t ls_in = wg_scan_max(S, last_end);
+    uint32_t sv3[2] = {nl + nr, nm}, st3[2], mv3[1] = {last_end};
+    wg_scan_multi<2, 1>(S, sv3, st3, mv3);
+    const uint32_t tot_l = st3[0], tot_m = st3[1];
+    const uint32_t lit_base = sv3[0] + S.st[ST_LIT];
+    const uint32_t m_base = sv3[1];
+    uint32_t ls_in = mv3[0];
     const uint32_t carried_ls = S.st[ST_LITSTART];
     if (ls_in < carried_ls) ls_in = carried_ls;
     QTS(2);
@@ -949,12 +988,14 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
             ls = p + len;
         }
     }
-    uint32_t t_cmd, t_off, t_op, t_meta, t_var;
-    uint32_t b_cmd = wg_scan(S, ccmd, &t_cmd) + S.st[ST_CMD];
-    uint32_t b_off = wg_scan(S, coff, &t_off) + S.st[ST_OFF];
-    uint32_t b_op = wg_scan(S, cop, &t_op) + S.st[ST_OP];
-    uint32_t b_meta = wg_scan(S, cmeta, &t_meta) + S.st[ST_META];
-    uint32_t b_var = wg_scan(S, cvar, &t_var) + S.st[ST_VAR];
+    uint32_t sv5[5] = {ccmd, coff, cop, cmeta, cvar}, st5[5];
+    wg_scan_multi<5, 0>(S, sv5, st5, nullptr);
+    const uint32_t t_cmd = st5[0], t_off = st5[1], t_op = st5[2], t_meta = st5[3], t_var = st5[4];
+    uint32_t b_cmd = sv5[0] + S.st[ST_CMD];
+    uint32_t b_off = sv5[1] + S.st[ST_OFF];
+    uint32_t b_op = sv5[2] + S.st[ST_OP];
+    uint32_t b_meta = sv5[3] + S.st[ST_META];
+    uint32_t b_var = sv5[4] + S.st[ST_VAR];
     QTS(4);
 
     /* 6. write bytes: literal gaps, then each token */
@@ -1040,16 +1081,15 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
     }
     {
         const uint32_t v = live ? last_end : 0u;
-        const uint32_t ex = wg_scan_max(S, v);
-        const uint32_t incl = ex > v ? ex : v;
+        uint32_t sv7[1] = {nr + nm}, st7[1], mv7[1] = {v};
+        wg_scan_multi<1, 1>(S, sv7, st7, mv7);
+        const uint32_t incl = mv7[0] > v ? mv7[0] : v;
         if (tid == LZ_NT - 1) {
             const uint32_t c = S.st[ST_LITSTART];
             S.st[ST_LITSTART] = incl > c ? incl : c;
         }
+        if (tid == 0) S.st[ST_TOK] += st7[0];
     }
-    uint32_t ntok;
-    (void)wg_scan(S, nr + nm, &ntok);
-    if (tid == 0) S.st[ST_TOK] += ntok;
     __syncthreads();
     QTS(6);
     (void)lane;
