@@ -51,6 +51,7 @@
 #define NONE32 0xFFFFFFFFu
 #define XCROSS 0x80000000u
 #define LEN_CROSS 0xFFFFFFFFu
+#define XROOM 128u                /* bytes past a segment end a thread may look to settle a token */
 #define NX_STOP 0xFFFEu
 #define NX_END 0xFFFFu
 /* padded LDS indices: one spare u16 / four spare bytes per 32 entries, so the
@@ -760,7 +761,13 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
             bm |= 1u << (p - ss);
             const Tok t = seg_tok(S, cxm, ss, p, a, n, se);
             if (t.kind == T_LIT) { p++; continue; }
-            if (t.len == LEN_CROSS) { cross = true; xp = p; break; }
+            if (t.len == LEN_CROSS) {
+                const Tok t2 = eval_tok<false>(S, p, a, n, se + XROOM);
+                if (t2.len == LEN_CROSS) { cross = true; xp = p; break; }
+                S.sm.ps.xlen[s] = t2.len | (t2.kind << 30);
+                p += t2.len;
+                break;
+            }
             p += t.len;
         }
         S.sm.ps.bmp[s] = bm;
@@ -793,7 +800,13 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
                     nb |= 1u << (p - ss);
                     const Tok t = seg_tok(S, cxm, ss, p, a, n, se);
                     if (t.kind == T_LIT) { p++; continue; }
-                    if (t.len == LEN_CROSS) { S.sm.ps.xpos[s] = p | XCROSS; S.sm.ps.bmp[s] = nb; done = true; break; }
+                    if (t.len == LEN_CROSS) {
+                        const Tok t2 = eval_tok<false>(S, p, a, n, se + XROOM);
+                        if (t2.len == LEN_CROSS) { S.sm.ps.xpos[s] = p | XCROSS; S.sm.ps.bmp[s] = nb; done = true; break; }
+                        S.sm.ps.xlen[s] = t2.len | (t2.kind << 30);
+                        p += t2.len;
+                        break;
+                    }
                     p += t.len;
                 }
                 if (!done) { S.sm.ps.xpos[s] = p; S.sm.ps.bmp[s] = nb; }
@@ -883,7 +896,13 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
                     nb |= 1u << (p - sgs);
                     const Tok t = seg_tok(S, cxm, sgs, p, a, n, sge);
                     if (t.kind == T_LIT) { p++; continue; }
-                    if (t.len == LEN_CROSS) { S.sm.ps.xpos[sg] = p | XCROSS; S.sm.ps.bmp[sg] = nb; done = true; break; }
+                    if (t.len == LEN_CROSS) {
+                        const Tok t2 = eval_tok<false>(S, p, a, n, sge + XROOM);
+                        if (t2.len == LEN_CROSS) { S.sm.ps.xpos[sg] = p | XCROSS; S.sm.ps.bmp[sg] = nb; done = true; break; }
+                        S.sm.ps.xlen[sg] = t2.len | (t2.kind << 30);
+                        p += t2.len;
+                        break;
+                    }
                     p += t.len;
                 }
                 if (!done) { S.sm.ps.xpos[sg] = p; S.sm.ps.bmp[sg] = nb; }
