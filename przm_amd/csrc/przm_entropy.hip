@@ -259,12 +259,6 @@ __constant__ M8Table przm_m8 = M8Table();
 /* value of lane sel[j] of the quad for quad lane j; ctrl = sel0 | sel1 << 2 | sel2 << 4 | sel3 << 6 */
 #define QPERM(v, ctrl) ((uint32_t)__builtin_amdgcn_mov_dpp((int)(v), (ctrl), 0xF, 0xF, false))
 
-__device__ __forceinline__ uint4 ld16(const uint8_t *p, int64_t c) {
-    return c >= 0 ? *(const uint4 *)(p + 16 * c) : make_uint4(0, 0, 0, 0);
-}
-__device__ __forceinline__ uint32_t u4get(const uint4 &v, uint32_t k) {
-    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
-}
 
 #define RANS_STAGE 128u /* per-job LDS staging of emitted bytes: two 64-byte halves */
 
@@ -273,94 +267,113 @@ __device__ __forceinline__ uint4 rev16(uint4 v) {
     return make_uint4(__builtin_bswap32(v.w), __builtin_bswap32(v.z), __builtin_bswap32(v.y), __builtin_bswap32(v.x));
 }
 
+#define RANS_PF 8u /* 16-byte symbol chunks in flight per quad (4 steps each) */
+
+/* a readable zero chunk: the load address of chunks outside a job's stream */
+__device__ __align__(16) uint4 przm_zero16;
+
+/* One coder step of quad lane j for step g (symbols 4g .. 4g + 3; this lane's is byte j of
+ * dw, lower is the word of step g - 1).  Updates the state x, the emitted byte count K and
+ * the overflow flag. */
+__device__ __forceinline__ void rans_step(uint32_t g, uint32_t dw, uint32_t lower, uint32_t j, uint32_t n,
+                                          uint32_t groups, bool m8, const uint32_t *tab, uint8_t *stg,
+                                          uint8_t *dummy, uint8_t *__restrict__ slot_end, uint32_t cap,
+                                          uint32_t &x, uint32_t &K, bool &over) {
+    const bool live = g < groups && !over;
+    const uint32_t i = 4 * g + j;
+    const bool v = live && i < n;
+    uint32_t idx = (dw >> (8 * j)) & 0xFFu;
+    if (m8) {
+        const uint32_t prev = j ? (dw >> (8 * j - 8)) & 0xFFu : lower >> 24;
+        idx += (i > 0 ? ctx_meta(prev) : 0u) << 8;
+    }
+    const uint32_t e = tab[v ? idx : 0u];
+    const uint32_t f = v ? (e & 0xFFFFu) : 1u, cdf = e >> 16;
+    const uint32_t t = 4096u * f - 1u;
+    const bool r1 = v && x > t;
+    const uint32_t x1 = r1 ? x >> 8 : x;
+    const bool r2 = r1 && x1 > t;
+    const uint32_t xn = r2 ? x1 >> 8 : x1;
+    const uint32_t ne = (uint32_t)r1 + (uint32_t)r2;
+    /* bytes of higher lanes come first: lane 3 emits before lane 2, ...
+     * (quad permutes: lane j reads lanes j+1, j+2, j+3 and lane 0 of its quad) */
+    const uint32_t e1 = QPERM(ne, 0xF9), e2 = QPERM(ne, 0xFE), e3 = QPERM(ne, 0xFF);
+    const uint32_t after = (j < 3 ? e1 : 0u) + (j < 2 ? e2 : 0u) + (j < 1 ? e3 : 0u);
+    const uint32_t tot = QPERM(ne + after, 0x00);
+    if (live && K + tot > cap) over = true; /* the job is not chosen any more: stop it */
+    const bool emit = live && !over;
+    const uint32_t k0 = K + after;
+    *(emit && r1 ? stg + (k0 & (RANS_STAGE - 1)) : dummy) = (uint8_t)x;
+    *(emit && r2 ? stg + ((k0 + 1) & (RANS_STAGE - 1)) : dummy) = (uint8_t)x1;
+    const uint32_t K2 = emit ? K + tot : K;
+    __builtin_amdgcn_wave_barrier();
+    if ((K2 >> 6) != (K >> 6)) { /* the run [64m, 64m + 64) is complete: m = K >> 6 */
+        const uint32_t m = K >> 6;
+        /* lane j stores bytes k = 64m + 48 - 16j .. 64m + 63 - 16j, which sit at
+         * slot_end - 64(m + 1) + 16j + (63 - (k - 64m) - 16j) ascending */
+        const uint32_t k_lo = 64u * m + 48u - 16u * j;
+        const uint4 w = *(const uint4 *)(stg + (k_lo & (RANS_STAGE - 1)));
+        *(uint4 *)(slot_end - 64u * (m + 1u) + 16u * j) = rev16(w);
+    }
+    __builtin_amdgcn_wave_barrier();
+    K = K2;
+    uint32_t qq, rr;
+    divmod(xn, f, &qq, &rr);
+    x = v && emit ? qq * 4096u + cdf + rr : x;
+}
+
 /*
  * The reverse 4-state pass of one stream by one lane quad (lane j = state j):
- * literals.go:305-325 / vans_meta.go:109-132.  Symbols arrive 16 at a time
- * (one 16-byte load per four steps, the next one in flight); src must be
- * 16-byte aligned and readable 16 bytes past n.  tab is the job's table
- * (freq | cdf << 16): V-ANS uses tab[sym], Meta8 tab[ctx * 256 + sym].
+ * literals.go:305-325 / vans_meta.go:109-132.  tab is the job's table
+ * (freq | cdf << 16): V-ANS uses tab[sym], Meta8 tab[ctx * 256 + sym].  src
+ * must be 16-byte aligned.
+ *
+ * Symbols arrive 16 at a time: chunk c holds the symbols of steps 4c .. 4c + 3.
+ * RANS_PF chunks are in flight in a register ring whose slots are named
+ * statically (the loop over one turn of the ring is unrolled), and every load
+ * is issued unconditionally (chunks outside the stream read przm_zero16), so
+ * the wait for a chunk waits for that load only, RANS_PF chunks after it was
+ * issued, not for the loads and stores issued since.
+ *
+ * Every quad of the wave takes the same number of steps, 4 * RANS_PF * the
+ * wave's longest job in ring turns (gmax: the longest job in 4-symbol groups),
+ * so the steps run in uniform control flow; a quad is live only for its own
+ * groups.
  *
  * Output byte k (k = 0, 1, ... in emission order) belongs at slot_end[-1-k].
  * Bytes are first written to the job's LDS staging ring stg[k % 128]; each
  * time a 64-byte run [64m, 64m + 64) is complete the quad stores it as four
- * aligned 16-byte words (slot_end is 16-byte aligned), so the coder issues
- * one global store per lane per 64 output bytes instead of byte stores.  At
- * most cap bytes; returns the byte count (or overflow) and the lane's final
- * state.
+ * aligned 16-byte words (slot_end is 16-byte aligned).  At most cap bytes;
+ * returns the byte count (or overflow) and the lane's final state.
  */
 __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, uint32_t n, bool m8,
                                               const uint32_t *tab, uint8_t *stg, uint8_t *__restrict__ slot_end,
                                               uint32_t cap, uint32_t gmax, uint32_t *state) {
-    /* Every quad of the wave takes the same G4 = gmax rounded up to four steps
-     * (gmax: the wave's longest job in 4-symbol groups), so the wave-wide steps
-     * below run in uniform control flow; a quad is live only for its own
-     * groups.  The 16-byte chunk of symbols of steps g = 4c + 3 .. 4c sits in
-     * cur with the current step's word in cur.w (the words rotate down each
-     * step), the next lower chunk in nxt. */
     const uint32_t j = threadIdx.x & 3;
     const uint32_t groups = (n + 3) >> 2;
-    const uint32_t G4 = (gmax + 3u) & ~3u;
-    const int64_t own = (int64_t)((n + 15u) >> 4); /* 16-byte chunks of this job's stream */
+    const int32_t own = (int32_t)((n + 15u) >> 4); /* 16-byte chunks of this job's stream */
+    const int32_t NC = (int32_t)((((gmax + 3u) >> 2) + RANS_PF - 1u) / RANS_PF * RANS_PF);
+    const uint4 *src4 = (const uint4 *)src;
+    auto chunk = [&](int32_t c) -> const uint4 * { return c >= 0 && c < own ? src4 + c : &przm_zero16; };
     uint32_t x = 65536u, K = 0;
     bool over = false;
-    int64_t c = (int64_t)(G4 >> 2) - 1;
-    /* chunks outside the job's own stream read as zero (a job shorter than the
-     * wave's longest, or an idle quad with no stream, never loads past its end) */
-    uint4 cur = c < own ? ld16(src, c) : make_uint4(0, 0, 0, 0);
-    uint4 nxt = c - 1 < own ? ld16(src, c - 1) : make_uint4(0, 0, 0, 0);
     uint8_t *const dummy = stg + RANS_STAGE + j; /* the target of stores that do not happen */
-    for (uint32_t g = G4; g-- > 0;) {
-        if ((g & 3u) == 3u && g != G4 - 1u) {
-            c--;
-            cur = nxt;
-            nxt = c - 1 < own ? ld16(src, c - 1) : make_uint4(0, 0, 0, 0);
+    uint4 buf[RANS_PF];
+#pragma unroll
+    for (uint32_t d = 0; d < RANS_PF; d++) buf[d] = *chunk(NC - 1 - (int32_t)d);
+    for (int32_t cb = NC - 1; cb >= 0; cb -= (int32_t)RANS_PF) {
+#pragma unroll
+        for (uint32_t d = 0; d < RANS_PF; d++) {
+            const int32_t c = cb - (int32_t)d;
+            /* the chunk below c: the next slot, or for the last slot the one just reloaded */
+            const uint32_t nw = d + 1 < RANS_PF ? buf[d + 1].w : buf[0].w;
+            const uint32_t g0 = 4u * (uint32_t)c;
+            rans_step(g0 + 3, buf[d].w, buf[d].z, j, n, groups, m8, tab, stg, dummy, slot_end, cap, x, K, over);
+            rans_step(g0 + 2, buf[d].z, buf[d].y, j, n, groups, m8, tab, stg, dummy, slot_end, cap, x, K, over);
+            rans_step(g0 + 1, buf[d].y, buf[d].x, j, n, groups, m8, tab, stg, dummy, slot_end, cap, x, K, over);
+            rans_step(g0, buf[d].x, nw, j, n, groups, m8, tab, stg, dummy, slot_end, cap, x, K, over);
+            buf[d] = *chunk(c - (int32_t)RANS_PF);
         }
-        const uint32_t dw = cur.w;
-        const uint32_t lower = (g & 3u) ? cur.z : nxt.w; /* the word of the next step down */
-        cur.w = cur.z;
-        cur.z = cur.y;
-        cur.y = cur.x;
-        const bool live = g < groups && !over;
-        const uint32_t i = 4 * g + j;
-        const bool v = live && i < n;
-        uint32_t idx = (dw >> (8 * j)) & 0xFFu;
-        if (m8) {
-            const uint32_t prev = j ? (dw >> (8 * j - 8)) & 0xFFu : lower >> 24;
-            idx += (i > 0 ? ctx_meta(prev) : 0u) << 8;
-        }
-        const uint32_t e = tab[v ? idx : 0u];
-        const uint32_t f = v ? (e & 0xFFFFu) : 1u, cdf = e >> 16;
-        const uint32_t t = 4096u * f - 1u;
-        const bool r1 = v && x > t;
-        const uint32_t x1 = r1 ? x >> 8 : x;
-        const bool r2 = r1 && x1 > t;
-        const uint32_t xn = r2 ? x1 >> 8 : x1;
-        const uint32_t ne = (uint32_t)r1 + (uint32_t)r2;
-        /* bytes of higher lanes come first: lane 3 emits before lane 2, ...
-         * (quad permutes: lane j reads lanes j+1, j+2, j+3 and lane 0 of its quad) */
-        const uint32_t e1 = QPERM(ne, 0xF9), e2 = QPERM(ne, 0xFE), e3 = QPERM(ne, 0xFF);
-        const uint32_t after = (j < 3 ? e1 : 0u) + (j < 2 ? e2 : 0u) + (j < 1 ? e3 : 0u);
-        const uint32_t tot = QPERM(ne + after, 0x00);
-        if (live && K + tot > cap) over = true; /* the job is not chosen any more: stop it */
-        const bool emit = live && !over;
-        const uint32_t k0 = K + after;
-        *(emit && r1 ? stg + (k0 & (RANS_STAGE - 1)) : dummy) = (uint8_t)x;
-        *(emit && r2 ? stg + ((k0 + 1) & (RANS_STAGE - 1)) : dummy) = (uint8_t)x1;
-        const uint32_t K2 = emit ? K + tot : K;
-        __builtin_amdgcn_wave_barrier();
-        if ((K2 >> 6) != (K >> 6)) { /* the run [64m, 64m + 64) is complete: m = K >> 6 */
-            const uint32_t m = K >> 6;
-            /* lane j stores bytes k = 64m + 48 - 16j .. 64m + 63 - 16j, which sit at
-             * slot_end - 64(m + 1) + 16j + (63 - (k - 64m) - 16j) ascending */
-            const uint32_t k_lo = 64u * m + 48u - 16u * j;
-            const uint4 w = *(const uint4 *)(stg + (k_lo & (RANS_STAGE - 1)));
-            *(uint4 *)(slot_end - 64u * (m + 1u) + 16u * j) = rev16(w);
-        }
-        __builtin_amdgcn_wave_barrier();
-        K = K2;
-        uint32_t qq, rr;
-        divmod(xn, f, &qq, &rr);
-        x = v && emit ? qq * 4096u + cdf + rr : x;
     }
     if (!over) { /* the last, partial run [64 (K >> 6), K) */
         const uint32_t base = K & ~63u;
@@ -370,16 +383,61 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
     return over ? PRZM_ENC_OVERFLOW : K;
 }
 
+/* the stream a coder job reads */
+__device__ __forceinline__ uint32_t job_stream(uint32_t kind) {
+    return kind == PRZM_J_LIT ? PRZM_S_LIT : kind <= PRZM_J_OP_V ? PRZM_S_OP : kind <= PRZM_J_META_V ? PRZM_S_META : PRZM_S_VAR;
+}
+
+/*
+ * Order of the coder jobs of one kind (one workgroup per kind): the blocks by
+ * the length of the job's stream in 256-byte classes, longest first, into
+ * perm[kind * nblocks ..].  A wave of przm_rans_kernel steps as often as its
+ * longest job, so jobs of like length share a wave, and the longest waves
+ * start first.  The order inside a class is whatever the atomics give: every
+ * job writes only its own outputs, so the results do not depend on it.
+ */
+extern "C" __global__ void __launch_bounds__(1024)
+przm_rans_order_kernel(const przm_meta_t *__restrict__ meta, uint32_t nblocks, uint32_t *__restrict__ perm) {
+    __shared__ uint32_t cnt[256];
+    const uint32_t kind = blockIdx.x, tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t sx = job_stream(kind);
+    if (tid < 256) cnt[tid] = 0;
+    __syncthreads();
+    for (uint32_t b = tid; b < nblocks; b += blockDim.x) {
+        const uint32_t c = meta[b].len[sx] >> 8;
+        atomicAdd(&cnt[255u - (c < 255u ? c : 255u)], 1u);
+    }
+    __syncthreads();
+    if (tid < 64) { /* exclusive scan of the 256 counts, four per lane */
+        uint32_t c4[4], t = 0;
+        for (int k = 0; k < 4; k++) { c4[k] = cnt[lane * 4 + k]; t += c4[k]; }
+        uint32_t x = t;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        uint32_t run = x - t;
+        for (int k = 0; k < 4; k++) { cnt[lane * 4 + k] = run; run += c4[k]; }
+    }
+    __syncthreads();
+    for (uint32_t b = tid; b < nblocks; b += blockDim.x) {
+        const uint32_t c = meta[b].len[sx] >> 8;
+        const uint32_t pos = atomicAdd(&cnt[255u - (c < 255u ? c : 255u)], 1u);
+        perm[(uint64_t)kind * nblocks + pos] = b;
+    }
+}
+
 /*
  * One lane quad per (block, coder job).  Workgroups are kind-aligned: the
  * first ceil(nblocks / 16) code the literal streams of 16 blocks each, the
  * next the op streams with Meta8, and so on, so a workgroup's table is
  * either sixteen per-job V-ANS tables or the one shared Meta8 table, both in
- * LDS.  grid = PRZM_J_N * ceil(nblocks / RANS_JOBS).
+ * LDS.  Inside a kind the blocks come in the order perm gives
+ * (przm_rans_order_kernel).  grid = PRZM_J_N * ceil(nblocks / RANS_JOBS).
  */
 extern "C" __global__ void __launch_bounds__(RANS_NT)
 przm_rans_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const przm_ftab_t *__restrict__ ftab,
-                 uint32_t nblocks) {
+                 uint32_t nblocks, const uint32_t *__restrict__ perm) {
     __shared__ union {
         uint32_t vt[RANS_JOBS][256];
         uint32_t m8[8][256];
@@ -388,8 +446,9 @@ przm_rans_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const
     const uint32_t tid = threadIdx.x, q = tid >> 2, j = tid & 3;
     const uint32_t per_kind = (nblocks + RANS_JOBS - 1) / RANS_JOBS;
     const uint32_t kind = blockIdx.x / per_kind;
-    const uint32_t blk = (blockIdx.x - kind * per_kind) * RANS_JOBS + q;
     if (kind >= PRZM_J_N) return;
+    const uint32_t slot = (blockIdx.x - kind * per_kind) * RANS_JOBS + q;
+    const uint32_t blk = slot < nblocks ? perm[(uint64_t)kind * nblocks + slot] : nblocks;
     const bool m8 = kind == PRZM_J_OP_M8 || kind == PRZM_J_META_M8 || kind == PRZM_J_VAR_M8;
     bool run = blk < nblocks;
     const uint8_t *src = nullptr;

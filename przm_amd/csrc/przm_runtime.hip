@@ -79,7 +79,7 @@ struct DevBuf {
 /* One set of device buffers + stream: the handle owns one, the server one per slot. */
 struct Ctx {
     hipStream_t st = nullptr;
-    DevBuf in, out, blocks, ws, meta, ftab, flen, foff, run, scratch, dbg;
+    DevBuf in, out, blocks, ws, meta, ftab, flen, foff, run, scratch, dbg, perm;
     uint64_t gen = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -90,7 +90,7 @@ struct Ctx {
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
-        for (DevBuf *b : {&in, &out, &blocks, &ws, &meta, &ftab, &flen, &foff, &run, &scratch, &dbg}) b->release();
+        for (DevBuf *b : {&in, &out, &blocks, &ws, &meta, &ftab, &flen, &foff, &run, &scratch, &dbg, &perm}) b->release();
         for (auto e : ev) (void)hipEventDestroy(e);
         ev.clear();
         if (st) (void)hipStreamDestroy(st);
@@ -199,6 +199,7 @@ int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg)
     CHK(c.ws.ensure((size_t)ch * PRZM_WS_BYTES, &c.gen));
     CHK(c.meta.ensure((size_t)std::max<uint32_t>(nblocks, 1) * sizeof(przm_meta_t), &c.gen));
     CHK(c.ftab.ensure((size_t)ch * 4 * sizeof(przm_ftab_t), &c.gen));
+    CHK(c.perm.ensure((size_t)ch * PRZM_J_N * 4, &c.gen));
     CHK(c.flen.ensure((size_t)std::max<uint32_t>(nblocks, 1) * 4, &c.gen));
     CHK(c.foff.ensure((size_t)std::max<uint32_t>(nblocks, 1) * 8, &c.gen));
     /* run: the running frame offset (u64) at 0, then one block counter (u32) per chunk at 64 */
@@ -253,8 +254,10 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         hipLaunchKernelGGL(przm_hist_kernel, dim3(nb), dim3(256), 0, st, (uint8_t *)c.ws.p, meta + c0,
                            (przm_ftab_t *)c.ftab.p, nb);
         CHK(rec());
+        hipLaunchKernelGGL(przm_rans_order_kernel, dim3(PRZM_J_N), dim3(1024), 0, st, (const przm_meta_t *)(meta + c0), nb,
+                           (uint32_t *)c.perm.p);
         hipLaunchKernelGGL(przm_rans_kernel, dim3(PRZM_J_N * ((nb + RANS_JOBS - 1) / RANS_JOBS)), dim3(RANS_NT), 0, st,
-                           (uint8_t *)c.ws.p, meta + c0, (const przm_ftab_t *)c.ftab.p, nb);
+                           (uint8_t *)c.ws.p, meta + c0, (const przm_ftab_t *)c.ftab.p, nb, (const uint32_t *)c.perm.p);
         CHK(rec());
         hipLaunchKernelGGL(przm_frame_size_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, blocks + c0, meta + c0,
                            (const uint8_t *)c.ws.p, nb, flen + c0);
