@@ -209,16 +209,27 @@ przm_hist1_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__restr
 #define RANS_NT 64
 #define RANS_JOBS (RANS_NT / 4)
 
-__device__ __forceinline__ uint32_t ctx_meta(uint32_t p) { /* vans_meta.go:45-68 */
-    return p == 0 ? 0u : p < 4 ? 1u : p < 16 ? 2u : p < 32 ? 3u : p < 64 ? 4u : p < 128 ? 5u : p < 192 ? 6u : 7u;
+/* vans_meta.go:45-68: 0 for 0, 1 for 1..3, 2 for 4..15, 3 for 16..31, 4 for 32..63, 5 for
+ * 64..127, 6 for 128..191, 7 for 192..255.  Branch-free from the bit length b of p (b = 0 for
+ * p = 0): max((b + 1) / 2, b - 2), plus one from 192 on. */
+__device__ __forceinline__ uint32_t ctx_meta(uint32_t p) {
+    const int32_t b = 31 - (int32_t)__builtin_clz(2u * p + 1u);
+    const int32_t lo = (b + 1) >> 1, hi = b - 2;
+    return (uint32_t)(lo > hi ? lo : hi) + (p >= 192u ? 1u : 0u);
 }
 
-/* x / f and x % f for x < 2^24, f in [1, 4096] */
+/*
+ * x / f and x % f for f in [1, 4096] and a quotient below 4096 (x <= 4096 f - 1, which the
+ * renormalisation guarantees; f = 1 with any x < 2^24 is exact too).  x * rcp(f) is within
+ * 1.5 * 2^-23 relative, i.e. 2^-10 absolute, of x / f; adding 2^-9 before truncating gives
+ * floor(x / f) or one more, so a single correction, without branches, finishes it.
+ */
 __device__ __forceinline__ void divmod(uint32_t x, uint32_t f, uint32_t *q, uint32_t *r) {
-    uint32_t qq = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)f));
+    uint32_t qq = (uint32_t)__builtin_fmaf((float)x, __builtin_amdgcn_rcpf((float)f), 0x1p-9f);
     int32_t rr = (int32_t)(x - qq * f);
-    if (rr < 0) { qq -= 1; rr += (int32_t)f; }
-    else if (rr >= (int32_t)f) { qq += 1; rr -= (int32_t)f; }
+    const bool lo = rr < 0;
+    qq = lo ? qq - 1u : qq;
+    rr = lo ? rr + (int32_t)f : rr;
     *q = qq;
     *r = (uint32_t)rr;
 }
@@ -275,15 +286,16 @@ __device__ __align__(16) uint4 przm_zero16;
 /* One coder step of quad lane j for step g (symbols 4g .. 4g + 3; this lane's is byte j of
  * dw, lower is the word of step g - 1).  Updates the state x, the emitted byte count K and
  * the overflow flag. */
+template <bool M8>
 __device__ __forceinline__ void rans_step(uint32_t g, uint32_t dw, uint32_t lower, uint32_t j, uint32_t n,
-                                          uint32_t groups, bool m8, const uint32_t *tab, uint8_t *stg,
-                                          uint8_t *dummy, uint8_t *__restrict__ slot_end, uint32_t cap,
-                                          uint32_t &x, uint32_t &K, bool &over) {
+                                          uint32_t groups, const uint32_t *__restrict__ tab,
+                                          uint8_t *__restrict__ stg, uint8_t *dummy, uint8_t *__restrict__ slot_end,
+                                          uint32_t cap, uint32_t &x, uint32_t &K, bool &over) {
     const bool live = g < groups && !over;
     const uint32_t i = 4 * g + j;
     const bool v = live && i < n;
     uint32_t idx = (dw >> (8 * j)) & 0xFFu;
-    if (m8) {
+    if (M8) {
         const uint32_t prev = j ? (dw >> (8 * j - 8)) & 0xFFu : lower >> 24;
         idx += (i > 0 ? ctx_meta(prev) : 0u) << 8;
     }
@@ -346,9 +358,11 @@ __device__ __forceinline__ void rans_step(uint32_t g, uint32_t dw, uint32_t lowe
  * aligned 16-byte words (slot_end is 16-byte aligned).  At most cap bytes;
  * returns the byte count (or overflow) and the lane's final state.
  */
-__device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, uint32_t n, bool m8,
-                                              const uint32_t *tab, uint8_t *stg, uint8_t *__restrict__ slot_end,
-                                              uint32_t cap, uint32_t gmax, uint32_t *state) {
+template <bool M8>
+__device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, uint32_t n,
+                                              const uint32_t *__restrict__ tab, uint8_t *__restrict__ stg,
+                                              uint8_t *__restrict__ slot_end, uint32_t cap, uint32_t gmax,
+                                              uint32_t *state) {
     const uint32_t j = threadIdx.x & 3;
     const uint32_t groups = (n + 3) >> 2;
     const int32_t own = (int32_t)((n + 15u) >> 4); /* 16-byte chunks of this job's stream */
@@ -368,10 +382,10 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
             /* the chunk below c: the next slot, or for the last slot the one just reloaded */
             const uint32_t nw = d + 1 < RANS_PF ? buf[d + 1].w : buf[0].w;
             const uint32_t g0 = 4u * (uint32_t)c;
-            rans_step(g0 + 3, buf[d].w, buf[d].z, j, n, groups, m8, tab, stg, dummy, slot_end, cap, x, K, over);
-            rans_step(g0 + 2, buf[d].z, buf[d].y, j, n, groups, m8, tab, stg, dummy, slot_end, cap, x, K, over);
-            rans_step(g0 + 1, buf[d].y, buf[d].x, j, n, groups, m8, tab, stg, dummy, slot_end, cap, x, K, over);
-            rans_step(g0, buf[d].x, nw, j, n, groups, m8, tab, stg, dummy, slot_end, cap, x, K, over);
+            rans_step<M8>(g0 + 3, buf[d].w, buf[d].z, j, n, groups, tab, stg, dummy, slot_end, cap, x, K, over);
+            rans_step<M8>(g0 + 2, buf[d].z, buf[d].y, j, n, groups, tab, stg, dummy, slot_end, cap, x, K, over);
+            rans_step<M8>(g0 + 1, buf[d].y, buf[d].x, j, n, groups, tab, stg, dummy, slot_end, cap, x, K, over);
+            rans_step<M8>(g0, buf[d].x, nw, j, n, groups, tab, stg, dummy, slot_end, cap, x, K, over);
             buf[d] = *chunk(c - (int32_t)RANS_PF);
         }
     }
@@ -475,7 +489,8 @@ przm_rans_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const
         gmax = y > gmax ? y : gmax;
     }
     uint32_t x;
-    const uint32_t K = rans_quad(src, run ? n : 0u, m8, m8 ? &T.m8[0][0] : T.vt[q], stage[q], slot_end, n, gmax, &x);
+    const uint32_t K = m8 ? rans_quad<true>(src, run ? n : 0u, &T.m8[0][0], stage[q], slot_end, n, gmax, &x)
+                          : rans_quad<false>(src, run ? n : 0u, T.vt[q], stage[q], slot_end, n, gmax, &x);
     if (!run) {
         if (blk < nblocks && j == 0) meta[blk].enc_len[kind] = PRZM_ENC_OVERFLOW;
         return;
@@ -495,7 +510,7 @@ przm_rans1_kernel(const uint8_t *__restrict__ src, uint32_t n, const przm_ftab_t
     __syncthreads();
     if (threadIdx.x >= 4) return;
     uint32_t x;
-    const uint32_t K = rans_quad(src, n, false, vt, stg, slot_end, cap, (n + 3) >> 2, &x);
+    const uint32_t K = rans_quad<false>(src, n, vt, stg, slot_end, cap, (n + 3) >> 2, &x);
     if (threadIdx.x == 0) out[0] = K;
     out[1 + threadIdx.x] = x;
 }
