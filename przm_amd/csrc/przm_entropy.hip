@@ -226,7 +226,7 @@ __device__ __forceinline__ uint32_t ctx_meta(uint32_t p) {
  */
 __device__ __forceinline__ void divmod(uint32_t x, uint32_t f, uint32_t *q, uint32_t *r) {
     uint32_t qq = (uint32_t)__builtin_fmaf((float)x, __builtin_amdgcn_rcpf((float)f), 0x1p-9f);
-    int32_t rr = (int32_t)(x - qq * f);
+    int32_t rr = (int32_t)(x - __umul24(qq, f)); /* qq and f both fit in 24 bits */
     const bool lo = rr < 0;
     qq = lo ? qq - 1u : qq;
     rr = lo ? rr + (int32_t)f : rr;
@@ -261,12 +261,6 @@ struct M8Table {
 };
 __constant__ M8Table przm_m8 = M8Table();
 
-/*
- * The reverse 4-state pass of one stream by one lane quad (lane j = state j):
- * literals.go:305-325 / vans_meta.go:109-132.  Emitted bytes go backward from
- * slot_end; at most cap bytes.  Returns the byte count (or overflow) and the
- * lane's final state.
- */
 /* value of lane sel[j] of the quad for quad lane j; ctrl = sel0 | sel1 << 2 | sel2 << 4 | sel3 << 6 */
 #define QPERM(v, ctrl) ((uint32_t)__builtin_amdgcn_mov_dpp((int)(v), (ctrl), 0xF, 0xF, false))
 
@@ -284,40 +278,41 @@ __device__ __forceinline__ uint4 rev16(uint4 v) {
 __device__ __align__(16) uint4 przm_zero16;
 
 /* One coder step of quad lane j for step g (symbols 4g .. 4g + 3; this lane's is byte j of
- * dw, lower is the word of step g - 1).  Updates the state x, the emitted byte count K and
- * the overflow flag. */
+ * dw, lower is the word of step g - 1), live for symbols below nl.  Updates the state x and
+ * the emitted byte count K. */
 template <bool M8>
-__device__ __forceinline__ void rans_step(uint32_t g, uint32_t dw, uint32_t lower, uint32_t j, uint32_t n,
-                                          uint32_t groups, const uint32_t *__restrict__ tab,
-                                          uint8_t *__restrict__ stg, uint8_t *dummy, uint8_t *__restrict__ slot_end,
-                                          uint32_t cap, uint32_t &x, uint32_t &K, bool &over) {
-    const bool live = g < groups && !over;
+__device__ __forceinline__ void rans_step(uint32_t g, uint32_t dw, uint32_t lower, uint32_t j, uint32_t nl,
+                                          const uint32_t *__restrict__ tab, uint8_t *__restrict__ stg,
+                                          uint8_t *dummy, uint8_t *__restrict__ slot_end, uint32_t &x,
+                                          uint32_t &K) {
     const uint32_t i = 4 * g + j;
-    const bool v = live && i < n;
+    const bool v = i < nl;
     uint32_t idx = (dw >> (8 * j)) & 0xFFu;
     if (M8) {
         const uint32_t prev = j ? (dw >> (8 * j - 8)) & 0xFFu : lower >> 24;
         idx += (i > 0 ? ctx_meta(prev) : 0u) << 8;
     }
-    const uint32_t e = tab[v ? idx : 0u];
-    const uint32_t f = v ? (e & 0xFFFFu) : 1u, cdf = e >> 16;
+    /* a lane that is not live reads some entry of its table and changes nothing */
+    const uint32_t e = tab[idx];
+    const uint32_t f = e & 0xFFFFu, cdf = e >> 16;
     const uint32_t t = 4096u * f - 1u;
     const bool r1 = v && x > t;
     const uint32_t x1 = r1 ? x >> 8 : x;
     const bool r2 = r1 && x1 > t;
     const uint32_t xn = r2 ? x1 >> 8 : x1;
     const uint32_t ne = (uint32_t)r1 + (uint32_t)r2;
-    /* bytes of higher lanes come first: lane 3 emits before lane 2, ...
-     * (quad permutes: lane j reads lanes j+1, j+2, j+3 and lane 0 of its quad) */
-    const uint32_t e1 = QPERM(ne, 0xF9), e2 = QPERM(ne, 0xFE), e3 = QPERM(ne, 0xFF);
-    const uint32_t after = (j < 3 ? e1 : 0u) + (j < 2 ? e2 : 0u) + (j < 1 ? e3 : 0u);
-    const uint32_t tot = QPERM(ne + after, 0x00);
-    if (live && K + tot > cap) over = true; /* the job is not chosen any more: stop it */
-    const bool emit = live && !over;
+    /* bytes of higher lanes come first: lane 3 emits before lane 2, ...  The quad's counts
+     * (0..2 each) as nibbles, summed over the quad by two quad permutes; this lane's offset
+     * is the sum of the nibbles above its own, the step's total the sum of all four. */
+    const uint32_t pn = ne << (4u * j);
+    const uint32_t qn = pn + QPERM(pn, 0xB1);  /* lanes [1, 0, 3, 2] */
+    const uint32_t rn = qn + QPERM(qn, 0x4E);  /* lanes [2, 3, 0, 1] */
+    const uint32_t after = (__umul24(rn >> (4u * j + 4u), 0x111u) >> 8) & 0xFu;
+    const uint32_t tot = (__umul24(rn, 0x1111u) >> 12) & 0xFu;
     const uint32_t k0 = K + after;
-    *(emit && r1 ? stg + (k0 & (RANS_STAGE - 1)) : dummy) = (uint8_t)x;
-    *(emit && r2 ? stg + ((k0 + 1) & (RANS_STAGE - 1)) : dummy) = (uint8_t)x1;
-    const uint32_t K2 = emit ? K + tot : K;
+    *(r1 ? stg + (k0 & (RANS_STAGE - 1)) : dummy) = (uint8_t)x;
+    *(r2 ? stg + ((k0 + 1) & (RANS_STAGE - 1)) : dummy) = (uint8_t)x1;
+    const uint32_t K2 = K + tot;
     __builtin_amdgcn_wave_barrier();
     if ((K2 >> 6) != (K >> 6)) { /* the run [64m, 64m + 64) is complete: m = K >> 6 */
         const uint32_t m = K >> 6;
@@ -331,7 +326,7 @@ __device__ __forceinline__ void rans_step(uint32_t g, uint32_t dw, uint32_t lowe
     K = K2;
     uint32_t qq, rr;
     divmod(xn, f, &qq, &rr);
-    x = v && emit ? qq * 4096u + cdf + rr : x;
+    x = v ? qq * 4096u + cdf + rr : x;
 }
 
 /*
@@ -349,8 +344,10 @@ __device__ __forceinline__ void rans_step(uint32_t g, uint32_t dw, uint32_t lowe
  *
  * Every quad of the wave takes the same number of steps, 4 * RANS_PF * the
  * wave's longest job in ring turns (gmax: the longest job in 4-symbol groups),
- * so the steps run in uniform control flow; a quad is live only for its own
- * groups.
+ * so the steps run in uniform control flow; a lane is live only for its own
+ * symbols.  The output bound is checked once per chunk: a job past cap
+ * (never chosen) stops there, having written at most 32 bytes past cap, which
+ * every slot has room for.
  *
  * Output byte k (k = 0, 1, ... in emission order) belongs at slot_end[-1-k].
  * Bytes are first written to the job's LDS staging ring stg[k % 128]; each
@@ -364,12 +361,11 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
                                               uint8_t *__restrict__ slot_end, uint32_t cap, uint32_t gmax,
                                               uint32_t *state) {
     const uint32_t j = threadIdx.x & 3;
-    const uint32_t groups = (n + 3) >> 2;
     const int32_t own = (int32_t)((n + 15u) >> 4); /* 16-byte chunks of this job's stream */
     const int32_t NC = (int32_t)((((gmax + 3u) >> 2) + RANS_PF - 1u) / RANS_PF * RANS_PF);
     const uint4 *src4 = (const uint4 *)src;
     auto chunk = [&](int32_t c) -> const uint4 * { return c >= 0 && c < own ? src4 + c : &przm_zero16; };
-    uint32_t x = 65536u, K = 0;
+    uint32_t x = 65536u, K = 0, nl = n;
     bool over = false;
     uint8_t *const dummy = stg + RANS_STAGE + j; /* the target of stores that do not happen */
     uint4 buf[RANS_PF];
@@ -382,11 +378,15 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
             /* the chunk below c: the next slot, or for the last slot the one just reloaded */
             const uint32_t nw = d + 1 < RANS_PF ? buf[d + 1].w : buf[0].w;
             const uint32_t g0 = 4u * (uint32_t)c;
-            rans_step<M8>(g0 + 3, buf[d].w, buf[d].z, j, n, groups, tab, stg, dummy, slot_end, cap, x, K, over);
-            rans_step<M8>(g0 + 2, buf[d].z, buf[d].y, j, n, groups, tab, stg, dummy, slot_end, cap, x, K, over);
-            rans_step<M8>(g0 + 1, buf[d].y, buf[d].x, j, n, groups, tab, stg, dummy, slot_end, cap, x, K, over);
-            rans_step<M8>(g0, buf[d].x, nw, j, n, groups, tab, stg, dummy, slot_end, cap, x, K, over);
+            rans_step<M8>(g0 + 3, buf[d].w, buf[d].z, j, nl, tab, stg, dummy, slot_end, x, K);
+            rans_step<M8>(g0 + 2, buf[d].z, buf[d].y, j, nl, tab, stg, dummy, slot_end, x, K);
+            rans_step<M8>(g0 + 1, buf[d].y, buf[d].x, j, nl, tab, stg, dummy, slot_end, x, K);
+            rans_step<M8>(g0, buf[d].x, nw, j, nl, tab, stg, dummy, slot_end, x, K);
             buf[d] = *chunk(c - (int32_t)RANS_PF);
+            if (K > cap) { /* the job is not chosen any more: stop it */
+                over = true;
+                nl = 0;
+            }
         }
     }
     if (!over) { /* the last, partial run [64 (K >> 6), K) */
