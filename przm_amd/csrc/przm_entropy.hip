@@ -176,7 +176,33 @@ przm_hist_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, przm_
     for (int i = 0; i < 4; i++) cnt[w][lane * 4 + i] = 0;
     __builtin_amdgcn_wave_barrier();
     if (run) {
-        for (uint32_t i = lane; i < n; i += 64) atomicAdd(&cnt[w][p[i]], 1u);
+        /* streams start 16-byte aligned: 16 bytes per lane and load, two loads in flight */
+        const uint4 *p4 = (const uint4 *)p;
+        const uint32_t n16 = n >> 4;
+        uint32_t i = lane;
+        for (; i + 64 < n16; i += 128) {
+            const uint4 a = p4[i], b = p4[i + 64];
+            const uint32_t wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                atomicAdd(&cnt[w][wd[k] & 0xFFu], 1u);
+                atomicAdd(&cnt[w][(wd[k] >> 8) & 0xFFu], 1u);
+                atomicAdd(&cnt[w][(wd[k] >> 16) & 0xFFu], 1u);
+                atomicAdd(&cnt[w][wd[k] >> 24], 1u);
+            }
+        }
+        for (; i < n16; i += 64) {
+            const uint4 a = p4[i];
+            const uint32_t wd[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                atomicAdd(&cnt[w][wd[k] & 0xFFu], 1u);
+                atomicAdd(&cnt[w][(wd[k] >> 8) & 0xFFu], 1u);
+                atomicAdd(&cnt[w][(wd[k] >> 16) & 0xFFu], 1u);
+                atomicAdd(&cnt[w][wd[k] >> 24], 1u);
+            }
+        }
+        for (uint32_t t = (n16 << 4) + lane; t < n; t += 64) atomicAdd(&cnt[w][p[t]], 1u);
     }
     __syncthreads();
     if (!run) return;

@@ -475,8 +475,8 @@ __device__ __forceinline__ uint32_t lcp_ext(const uint8_t *in, uint32_t p, uint3
 }
 
 /*
- * Search elements [i0, i1) (sorted order) of a bucket whose position array is
- * P, sorted permutation SRT and per-digit group ends GE.  Element i's chain is
+ * Search elements [i0, i1) (sorted order) of a bucket whose positions in group
+ * order are SRT and per-digit group ends GE.  Element i's chain is
  * the up to `depth` elements before it in its group (same 16-bit hash,
  * newest first, compress.go:681-707); the longest common prefix wins, ties
  * keep the nearest, and only lengths >= 4 count.
@@ -496,7 +496,7 @@ __device__ __forceinline__ uint32_t lcp_ext(const uint8_t *in, uint32_t p, uint3
  */
 template <bool DEEP, uint32_t B, typename PT, typename ET>
 __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, uint32_t dgt, uint32_t n,
-                                                    uint32_t depth, const PT *P, const PT *SRT, const ET *GE,
+                                                    uint32_t depth, const PT *SRT, const ET *GE,
                                                     uint32_t i0, uint32_t i1, uint32_t step, uint64_t *pt = nullptr) {
     const bool prof = pt && threadIdx.x == 0;
     const uint8_t *in = S.in;
@@ -514,7 +514,7 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         if (c0 < 64) ring[lane] = filler;
         else if (step == 64 && c0 != i0) ring[lane] = ring[64 + lane]; /* this wave staged it last time */
         else {
-            const uint32_t ip = c0 - 64 + lane, pp = P[SRT[ip]];
+            const uint32_t ip = c0 - 64 + lane, pp = SRT[ip];
             const uint4 k = ldw16(in, pp);
             ring[lane] = make_uint4(k.x, k.y, k.z, pp);
         }
@@ -522,7 +522,7 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         uint32_t p = 0, kmax = 0, cap = 0;
         uint4 key = make_uint4(0, 0, 0, 0);
         if (valid) {
-            p = P[SRT[i]];
+            p = SRT[i];
             const uint4 k = ldw16(in, p);
             key = make_uint4(k.x, k.y, k.z, p);
             ring[64 + lane] = key;
@@ -549,7 +549,7 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
                 const uint32_t k = k0 + u;
                 if (DEEP && k > lane + 64u) {
                     if (k <= kmax) {
-                        const uint32_t q = (uint32_t)P[SRT[i - k]];
+                        const uint32_t q = (uint32_t)SRT[i - k];
                         const uint4 b = ldw16(in, q);
                         qk[u] = make_uint4(b.x, b.y, b.z, q);
                     } else {
@@ -650,12 +650,13 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     for (uint32_t c0 = 0; c0 < m; c0 += 64) {
         const uint32_t j = c0 + lane;
         const bool valid = j < m;
-        const uint32_t low = valid ? (hash4(in, P[j]) & 0xFFu) : 0u;
+        const uint32_t pj = valid ? P[j] : 0u;
+        const uint32_t low = valid ? (hash4(in, pj) & 0xFFu) : 0u;
         const uint64_t pm = peers8(low, valid);
         const uint32_t base = valid ? cnt[low] : 0u;
         lds_fence();
         if (valid) {
-            SRT[base + (uint32_t)__popcll(pm & lt)] = (uint16_t)j;
+            SRT[base + (uint32_t)__popcll(pm & lt)] = (uint16_t)pj; /* the position itself, in group order */
             if ((pm & lt) == 0) cnt[low] = base + (uint32_t)__popcll(pm);
         }
         lds_fence();
@@ -671,9 +672,9 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
         return true;
     }
     if (prof) pt[0] += t_c - t_a;
-    if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64, pt);
-    else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64, pt);
-    else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, P, SRT, cnt, 0, m, 64, pt);
+    if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pt);
+    else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pt);
+    else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pt);
     return false;
 }
 
@@ -1254,12 +1255,11 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             const uint32_t dg = dq - 1;
             const uint32_t m = (S.bstart[0][dg + 1] - S.bstart[0][dg]) + (S.bstart[1][dg + 1] - S.bstart[1][dg]);
             const uint8_t *gw = sc.gws + (uint64_t)ow * GWS_STRIDE;
-            const uint16_t *P = (const uint16_t *)gw;
-            const uint16_t *SRT = P + PRZM_BLOCK;
+            const uint16_t *SRT = (const uint16_t *)gw + PRZM_BLOCK;
             const uint32_t *GE = (const uint32_t *)(gw + 2u * PRZM_BLOCK * 2u);
-            if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
-            else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
-            else bucket_search_range<false, SEARCH_BATCH>(S, sc, dg, n, depth, P, SRT, GE, w * 64, m, LZ_NW * 64);
+            if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64);
+            else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64);
+            else bucket_search_range<false, SEARCH_BATCH>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64);
         }
         __threadfence_block();
         __syncthreads();
