@@ -85,7 +85,7 @@ struct LzShared {
             uint16_t dist[DSZ];
             uint8_t lc[LSZ];
             uint16_t tlen[NSEG][8]; /* cached non-literal tokens of each segment (swizzled) */
-            uint8_t toff[NSEG][8];  /* start offset in segment | kind << 5 */
+            uint8_t toff[NSEG][8];  /* start offset in segment | kind << 5 | (ring index + 1) << 6 */
         } q;
     } big;
     union {
@@ -993,6 +993,7 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
             } else {
                 const uint32_t d = dget(S, p - a);
                 const int kk = lru_index(S, mi, d);
+                S.big.q.toff[s][sk] = (uint8_t)(o | ((uint32_t)(kk + 1) << 6)); /* kept for the write pass */
                 mi++;
                 const uint32_t keep = kk >= 0 ? (len <= 7 ? 7u : 15u) : 3u;
                 if (ll > keep) {
@@ -1042,7 +1043,7 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
                 lit[lb++] = in[p];
             } else {
                 const uint32_t d = dget(S, p - a);
-                const int kk = lru_index(S, mi, d);
+                const int kk = (int)(o >> 6) - 1; /* the ring index the size pass found */
                 mi++;
                 const uint32_t keep = kk >= 0 ? (len <= 7 ? 7u : 15u) : 3u;
                 uint32_t l2 = ll;
@@ -1184,7 +1185,6 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
                 if (p < end) atomicAdd(&S.sm.tab[w][hash4(S.in, p) >> 8], 1u);
             }
             __syncthreads();
-            if (H == 0) PTS(14);
             {
                 const uint32_t dg = tid < 256 ? tid : 0;
                 uint32_t tot = 0;
@@ -1204,7 +1204,6 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
                 if (tid == 0) S.bstart[H][256] = all;
             }
             __syncthreads();
-            if (H == 0) PTS(15);
             const uint64_t lt = (1ull << lane) - 1ull;
             for (uint32_t c = 0; c < PRZM_HALF / LZ_NW; c += 64) {
                 const uint32_t p = wb + c + lane;
@@ -1246,6 +1245,7 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
                 if (q && lane == 0) S.bigq[w] = dg + 1;
             }
         }
+        if (ts && lane == 0 && (w == 0 || w == LZ_NW - 1)) ts[w == 0 ? 14 : 15] = __builtin_amdgcn_s_memrealtime();
         __threadfence_block();
         __syncthreads();
         PTS(13);
