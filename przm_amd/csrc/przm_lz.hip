@@ -84,8 +84,8 @@ struct LzShared {
         struct {
             uint16_t dist[DSZ];
             uint8_t lc[LSZ];
-            uint16_t tlen[NSEG][8]; /* cached non-literal tokens of each segment (swizzled) */
-            uint8_t toff[NSEG][8];  /* start offset in segment | kind << 5 | (ring index + 1) << 6 */
+            __align__(16) uint16_t tlen[NSEG][8]; /* cached non-literal tokens of each segment (swizzled) */
+            uint8_t toff[NSEG][8];  /* start offset in segment | kind << 5 */
         } q;
     } big;
     union {
@@ -376,31 +376,6 @@ __device__ Tok eval_tok(const LzShared &S, uint32_t p, uint32_t a, uint32_t n, u
     return {T_MATCH, LEN_CROSS};
 }
 
-/* ring-register index of match number i (0..2) or -1 for an explicit offset */
-__device__ int lru_index(const LzShared &S, uint32_t i, uint32_t d) {
-    uint32_t seen0 = 0, seen1 = 0;
-    int ns = 0;
-    for (uint32_t j = i; j-- > 0;) {
-        const uint32_t v = S.sm.ps.tdist[j];
-        if (v == d) return ns;
-        if (ns == 0) { seen0 = v; ns = 1; }
-        else if (v != seen0 && (ns == 1 || v != seen1)) {
-            if (ns == 1) { seen1 = v; ns = 2; }
-            else return -1;
-        }
-    }
-    const uint32_t r[3] = {S.st[ST_R0], S.st[ST_R1], S.st[ST_R2]};
-    for (int j = 0; j < 3; j++) {
-        const uint32_t v = r[j];
-        if (v == d) return ns;
-        if (ns == 0) { seen0 = v; ns = 1; }
-        else if (v != seen0 && (ns == 1 || v != seen1)) {
-            if (ns == 1) { seen1 = v; ns = 2; }
-            else return -1;
-        }
-    }
-    return -1;
-}
 
 /* Orders LDS accesses of one wave for the compiler only: the LDS executes a
  * wave's DS instructions in issue order, so no hardware wait is needed.  It
@@ -676,6 +651,185 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pt);
     else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pt);
     return false;
+}
+
+/*
+ * A segment's cached tokens (at most 8) in registers: toff and tlen rows as they lie in LDS
+ * (entry k of the segment at index k ^ (s & 7)), and the distances of its matches in order
+ * (tdist[m_base ..]).
+ */
+struct SegTok {
+    uint64_t off;      /* toff row: byte i */
+    uint64_t len[2];   /* tlen row: u16 i in len[i >> 2] */
+    uint64_t dist[2];  /* u16 r: distance of the segment's match number r */
+};
+__device__ __forceinline__ uint32_t u16_of(const uint64_t (&v)[2], uint32_t i) {
+    return (uint32_t)(((i < 4u ? v[0] : v[1]) >> (16u * (i & 3u))) & 0xFFFFu);
+}
+__device__ __forceinline__ SegTok seg_tok_load(const LzShared &S, uint32_t s, uint32_t m_base) {
+    SegTok t;
+    t.off = *(const uint64_t *)&S.big.q.toff[s][0];
+    const uint4 l = *(const uint4 *)&S.big.q.tlen[s][0];
+    t.len[0] = ((uint64_t)l.y << 32) | l.x;
+    t.len[1] = ((uint64_t)l.w << 32) | l.z;
+    uint32_t d[8];
+#pragma unroll
+    for (uint32_t r = 0; r < 8; r++) d[r] = m_base + r < QP / 4 ? S.sm.ps.tdist[m_base + r] : 0u;
+    t.dist[0] = ((uint64_t)(d[3] << 16 | d[2]) << 32) | (d[1] << 16 | d[0]);
+    t.dist[1] = ((uint64_t)(d[7] << 16 | d[6]) << 32) | (d[5] << 16 | d[4]);
+    return t;
+}
+
+/* The three most recent distinct match distances before match number i (most recent first),
+ * from tdist[i - 1], tdist[i - 2], ... and then the ring carried into the tile
+ * (przm_tokenizer.cuh ring registers). */
+__device__ __forceinline__ void ring_before(const LzShared &S, uint32_t i, uint32_t &r0, uint32_t &r1,
+                                            uint32_t &r2) {
+    uint32_t g0 = 0, g1 = 0, g2 = 0, ns = 0;
+    for (uint32_t j = i; j-- > 0 && ns < 3;) {
+        const uint32_t v = S.sm.ps.tdist[j];
+        if (ns == 0) { g0 = v; ns = 1; }
+        else if (v != g0 && (ns == 1 || v != g1)) {
+            if (ns == 1) { g1 = v; ns = 2; }
+            else { g2 = v; ns = 3; }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const uint32_t v = j == 0 ? S.st[ST_R0] : j == 1 ? S.st[ST_R1] : S.st[ST_R2];
+        if (ns == 0) { g0 = v; ns = 1; }
+        else if (ns < 3 && v != g0 && (ns == 1 || v != g1)) {
+            if (ns == 1) { g1 = v; ns = 2; }
+            else { g2 = v; ns = 3; }
+        }
+    }
+    r0 = g0; r1 = g1; r2 = g2;
+}
+
+/* ring index of distance d (0..2, or -1 for an explicit offset), and the ring after it */
+__device__ __forceinline__ int ring_use(uint32_t d, uint32_t &r0, uint32_t &r1, uint32_t &r2) {
+    const int kk = d == r0 ? 0 : d == r1 ? 1 : d == r2 ? 2 : -1;
+    r2 = kk == 0 || kk == 1 ? r2 : r1;
+    r1 = kk == 0 ? r1 : r0;
+    r0 = d;
+    return kk;
+}
+
+/* bytes of the staging areas in the dist / lc arrays (dead after the tokens are listed):
+ * literals (at most a quarter's bytes), cmd and op, each with 32 bytes of slack for the
+ * 16-byte reads of the copy-out */
+#define STG_LIT_CAP (QP + 32u)
+#ifndef STG_CMD_CAP
+#define STG_CMD_CAP 17408u /* a build with 0 here writes every quarter directly */
+#endif
+#define STG_OP_CAP 17408u
+static_assert(STG_LIT_CAP + STG_CMD_CAP + STG_OP_CAP + 32u <= DSZ * 2u + LSZ, "staging fits in dist + lc");
+
+/* the literal bytes in[cur .. p) (at most 32, inside one segment; none when the previous token
+ * ends at or past p) to lit[lb ..]: the 36 bytes around them are loaded at once, then stored
+ * one by one */
+__device__ __forceinline__ void copy_run(uint8_t *lit, uint32_t &lb, const uint8_t *in, uint32_t cur, uint32_t p) {
+    if (p <= cur) return;
+    const uint32_t L = p - cur;
+    const uint32_t *w = (const uint32_t *)(in + (cur & ~3u));
+    const uint32_t sh = cur & 3u;
+    uint32_t wd[9], aw[8];
+#pragma unroll
+    for (int k = 0; k < 9; k++) wd[k] = w[k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) aw[k] = __builtin_amdgcn_alignbyte(wd[k + 1], wd[k], sh);
+#pragma unroll
+    for (uint32_t t = 0; t < 32; t++) {
+        if (t >= L) break;
+        lit[lb + t] = (uint8_t)(aw[t >> 2] >> (8u * (t & 3u)));
+    }
+    lb += L;
+}
+
+/*
+ * The byte writes of one segment's tokens (przm_tokenizer.cuh:330-437 emission): literal gaps,
+ * then each token.  STG: literal, cmd and op bytes go to the LDS staging areas at offsets
+ * relative to the quarter's start (lb, b_cmd, b_op given so), the other streams to global
+ * memory; otherwise every stream goes to global memory at absolute offsets.
+ */
+template <bool STG>
+__device__ __forceinline__ void write_tokens(LzShared &S, uint32_t s, uint32_t ss, uint32_t se, uint32_t t0,
+                                             uint32_t nt, const SegTok &tk, uint32_t kkp, uint32_t ls_in,
+                                             uint32_t lb, uint32_t b_cmd, uint32_t b_op, uint32_t b_meta,
+                                             uint32_t b_var, uint32_t b_off, uint8_t *glit, uint8_t *gcmd,
+                                             uint8_t *gop, uint8_t *meta, uint8_t *var, uint8_t *off) {
+    const uint8_t *in = S.in;
+    uint8_t *const LB = (uint8_t *)S.big.q.dist;
+    uint8_t *const lit = STG ? LB : glit;
+    uint8_t *const cmd = STG ? LB + STG_LIT_CAP : gcmd;
+    uint8_t *const op = STG ? LB + STG_LIT_CAP + STG_CMD_CAP : gop;
+    uint32_t mr = 0, ls = ls_in, cur = t0;
+    for (uint32_t k = 0; k <= nt; k++) {
+        const uint32_t sk = k ^ (s & 7u);
+        const bool tok = k < nt;
+        const uint32_t o = tok ? (uint32_t)(tk.off >> (8u * (sk & 7u))) & 0xFFu : 0u;
+        const uint32_t len = tok ? u16_of(tk.len, sk & 7u) : 0u;
+        const uint32_t p = tok ? ss + (o & 31u) : se;
+        copy_run(lit, lb, in, cur, p);
+        if (!tok) break;
+        const uint32_t ll = p - ls;
+        if (o & 0x20u) {
+            if (ll) {
+                if (ll <= 8) { cmd[b_cmd++] = (uint8_t)(0xF2 + ll); op[b_op++] = (uint8_t)(0xF2 + ll); }
+                else {
+                    cmd[b_cmd++] = 0xF0; op[b_op++] = 0xF0;
+                    b_cmd = put_leb(cmd, b_cmd, ll); b_var = put_leb(var, b_var, ll);
+                }
+            }
+            cmd[b_cmd++] = 0xF1; op[b_op++] = 0xF1;
+            b_cmd = put_leb(cmd, b_cmd, len); b_var = put_leb(var, b_var, len);
+            lit[lb++] = in[p];
+        } else {
+            const uint32_t d = u16_of(tk.dist, mr++);         /* this match's distance */
+            const int kk = (int)((kkp >> (2u * k)) & 3u) - 1; /* the ring index the size pass found */
+            const uint32_t keep = kk >= 0 ? (len <= 7 ? 7u : 15u) : 3u;
+            uint32_t l2 = ll;
+            if (ll > keep) {
+                const uint32_t f = ll - keep;
+                if (f <= 8) { cmd[b_cmd++] = (uint8_t)(0xF2 + f); op[b_op++] = (uint8_t)(0xF2 + f); }
+                else {
+                    cmd[b_cmd++] = 0xF0; op[b_op++] = 0xF0;
+                    b_cmd = put_leb(cmd, b_cmd, f); b_var = put_leb(var, b_var, f);
+                }
+                l2 = keep;
+            }
+            const uint32_t base = len - 4 > 14 ? 15u : len - 4;
+            if (kk >= 0 && len <= 7) {
+                const uint8_t oc = (uint8_t)((l2 << 4) | ((len - 4) << 2) | (uint32_t)kk);
+                cmd[b_cmd++] = oc; op[b_op++] = oc;
+            } else if (kk >= 0) {
+                const uint8_t tok = (uint8_t)((l2 << 4) | base);
+                cmd[b_cmd++] = (uint8_t)(0xC0 | kk); cmd[b_cmd++] = tok;
+                op[b_op++] = (uint8_t)(0xC0 | kk); meta[b_meta++] = tok;
+                if (base == 15) { b_cmd = put_leb(cmd, b_cmd, len - 19); b_var = put_leb(var, b_var, len - 19); }
+            } else {
+                const uint8_t oc = (uint8_t)(0x80 | (l2 << 4) | base);
+                cmd[b_cmd++] = oc; op[b_op++] = oc;
+                if (base == 15) { b_cmd = put_leb(cmd, b_cmd, len - 19); b_var = put_leb(var, b_var, len - 19); }
+                b_off = put_leb(off, b_off, d);
+            }
+        }
+        ls = p + len;
+        cur = p + len;
+    }
+}
+
+/* n bytes from the LDS area src (4-byte aligned, readable 20 bytes past n) to dst: the
+ * workgroup stores the 16-byte aligned body of dst as uint4s, the ends as bytes */
+__device__ __forceinline__ void lds_copy_out(uint8_t *__restrict__ dst, const uint8_t *src, uint32_t n) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t mis = (uint32_t)((uintptr_t)dst & 15u);
+    const uint32_t head = umin(n, (16u - mis) & 15u);
+    if (tid < head) dst[tid] = src[tid];
+    const uint32_t body = (n - head) & ~15u;
+    for (uint32_t o = head + tid * 16u; o < head + body; o += LZ_NT * 16u) *(uint4 *)(dst + o) = ldw16(src, o);
+    const uint32_t t0 = head + body;
+    if (tid < n - t0) dst[t0 + tid] = src[t0 + tid];
 }
 
 /* ---------------- parse of one tile ---------------- */
@@ -977,13 +1131,18 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
     __syncthreads();
     QTS(3);
 
-    /* 5. byte counts per segment */
-    uint32_t ccmd = 0, coff = 0, cop = 0, cmeta = 0, cvar = 0;
+    /* 5. byte counts per segment, with the segment's tokens and match distances in registers
+     *    and the ring of the last three distances kept up to date from the segment's start */
+    uint32_t ccmd = 0, coff = 0, cop = 0, cmeta = 0, cvar = 0, kkp = 0;
+    SegTok tk = {};
     if (live) {
-        uint32_t mi = m_base, ls = ls_in;
+        tk = seg_tok_load(S, s, m_base);
+        uint32_t r0, r1, r2;
+        ring_before(S, m_base, r0, r1, r2);
+        uint32_t mr = 0, ls = ls_in;
         for (uint32_t k = 0; k < nt; k++) {
             const uint32_t sk = k ^ (s & 7u);
-            const uint32_t o = S.big.q.toff[s][sk], len = S.big.q.tlen[s][sk];
+            const uint32_t o = (uint32_t)(tk.off >> (8u * sk)) & 0xFFu, len = u16_of(tk.len, sk);
             const uint32_t p = ss + (o & 31u);
             const uint32_t ll = p - ls;
             if (o & 0x20u) {
@@ -991,10 +1150,9 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
                 const uint32_t q = leb_len(len);
                 ccmd += 1 + q; cop += 1; cvar += q;
             } else {
-                const uint32_t d = dget(S, p - a);
-                const int kk = lru_index(S, mi, d);
-                S.big.q.toff[s][sk] = (uint8_t)(o | ((uint32_t)(kk + 1) << 6)); /* kept for the write pass */
-                mi++;
+                const uint32_t d = u16_of(tk.dist, mr++);
+                const int kk = ring_use(d, r0, r1, r2);
+                kkp |= (uint32_t)(kk + 1) << (2u * k); /* kept for the write pass */
                 const uint32_t keep = kk >= 0 ? (len <= 7 ? 7u : 15u) : 3u;
                 if (ll > keep) {
                     const uint32_t f = ll - keep;
@@ -1019,62 +1177,25 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
     uint32_t b_var = sv5[4] + S.st[ST_VAR];
     QTS(4);
 
-    /* 6. write bytes: literal gaps, then each token */
+    /* 6. write bytes: literal gaps, then each token.  dist and lc are dead after step 4,
+     *    so when the quarter's cmd and op bytes fit, its literal, cmd and op bytes are staged
+     *    there and copied out with aligned 16-byte stores (scattered byte stores otherwise) */
+    const uint32_t lit0 = S.st[ST_LIT], cmd0 = S.st[ST_CMD], op0 = S.st[ST_OP];
+    const bool stage = t_cmd <= STG_CMD_CAP && t_op <= STG_OP_CAP;
     if (live) {
-        uint32_t mi = m_base, ls = ls_in, lb = lit_base, cur = t0;
-        for (uint32_t k = 0; k <= nt; k++) {
-            const uint32_t sk = k ^ (s & 7u);
-            const uint32_t o = k < nt ? S.big.q.toff[s][sk] : 0u;
-            const uint32_t len = k < nt ? S.big.q.tlen[s][sk] : 0u;
-            const uint32_t p = k < nt ? ss + (o & 31u) : se;
-            for (uint32_t q = cur; q < p; q++) lit[lb++] = in[q];
-            if (k == nt) break;
-            const uint32_t ll = p - ls;
-            if (o & 0x20u) {
-                if (ll) {
-                    if (ll <= 8) { cmd[b_cmd++] = (uint8_t)(0xF2 + ll); op[b_op++] = (uint8_t)(0xF2 + ll); }
-                    else {
-                        cmd[b_cmd++] = 0xF0; op[b_op++] = 0xF0;
-                        b_cmd = put_leb(cmd, b_cmd, ll); b_var = put_leb(var, b_var, ll);
-                    }
-                }
-                cmd[b_cmd++] = 0xF1; op[b_op++] = 0xF1;
-                b_cmd = put_leb(cmd, b_cmd, len); b_var = put_leb(var, b_var, len);
-                lit[lb++] = in[p];
-            } else {
-                const uint32_t d = dget(S, p - a);
-                const int kk = (int)(o >> 6) - 1; /* the ring index the size pass found */
-                mi++;
-                const uint32_t keep = kk >= 0 ? (len <= 7 ? 7u : 15u) : 3u;
-                uint32_t l2 = ll;
-                if (ll > keep) {
-                    const uint32_t f = ll - keep;
-                    if (f <= 8) { cmd[b_cmd++] = (uint8_t)(0xF2 + f); op[b_op++] = (uint8_t)(0xF2 + f); }
-                    else {
-                        cmd[b_cmd++] = 0xF0; op[b_op++] = 0xF0;
-                        b_cmd = put_leb(cmd, b_cmd, f); b_var = put_leb(var, b_var, f);
-                    }
-                    l2 = keep;
-                }
-                const uint32_t base = len - 4 > 14 ? 15u : len - 4;
-                if (kk >= 0 && len <= 7) {
-                    const uint8_t oc = (uint8_t)((l2 << 4) | ((len - 4) << 2) | (uint32_t)kk);
-                    cmd[b_cmd++] = oc; op[b_op++] = oc;
-                } else if (kk >= 0) {
-                    const uint8_t tok = (uint8_t)((l2 << 4) | base);
-                    cmd[b_cmd++] = (uint8_t)(0xC0 | kk); cmd[b_cmd++] = tok;
-                    op[b_op++] = (uint8_t)(0xC0 | kk); meta[b_meta++] = tok;
-                    if (base == 15) { b_cmd = put_leb(cmd, b_cmd, len - 19); b_var = put_leb(var, b_var, len - 19); }
-                } else {
-                    const uint8_t oc = (uint8_t)(0x80 | (l2 << 4) | base);
-                    cmd[b_cmd++] = oc; op[b_op++] = oc;
-                    if (base == 15) { b_cmd = put_leb(cmd, b_cmd, len - 19); b_var = put_leb(var, b_var, len - 19); }
-                    b_off = put_leb(off, b_off, d);
-                }
-            }
-            ls = p + len;
-            cur = p + len;
-        }
+        if (stage)
+            write_tokens<true>(S, s, ss, se, t0, nt, tk, kkp, ls_in, lit_base - lit0, b_cmd - cmd0, b_op - op0,
+                               b_meta, b_var, b_off, lit, cmd, op, meta, var, off);
+        else
+            write_tokens<false>(S, s, ss, se, t0, nt, tk, kkp, ls_in, lit_base, b_cmd, b_op, b_meta, b_var, b_off,
+                                lit, cmd, op, meta, var, off);
+    }
+    __syncthreads();
+    if (stage) {
+        uint8_t *const LB = (uint8_t *)S.big.q.dist;
+        lds_copy_out(lit + lit0, LB, tot_l);
+        lds_copy_out(cmd + cmd0, LB + STG_LIT_CAP, t_cmd);
+        lds_copy_out(op + op0, LB + STG_LIT_CAP + STG_CMD_CAP, t_op);
     }
     __syncthreads();
     QTS(5);
