@@ -1393,10 +1393,25 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             const uint32_t b = n < a + QP ? n : a + QP;
             /* the tile's match results [a, b] from the position-ordered results (positions
              * past n - 4 have none) */
-            for (uint32_t i0 = 0; i0 <= b - a; i0 += LZ_NT) { /* the same trip count in every lane */
-                const uint32_t i = i0 + tid;
+            /* eight loads at a time (unconditional, clamped into the array, so that they are in
+             * flight together), then their LDS writes; the same trip count in every lane */
+            constexpr uint32_t FR = (QP + 1 + LZ_NT - 1) / LZ_NT, FB = 8;
+            const uint32_t dlast = nh ? nh - 1u : 0u;
+#pragma unroll 1
+            for (uint32_t r0 = 0; r0 < FR; r0 += FB) {
+            uint32_t dv[FB];
+#pragma unroll
+            for (uint32_t u = 0; u < FB; u++) {
+                const uint32_t e = a + tid + (r0 + u) * LZ_NT;
+                dv[u] = sc.D[e < dlast ? e : dlast];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < FB; u++) {
+                const uint32_t r = r0 + u;
+                if (r >= FR) break;
+                const uint32_t i = tid + r * LZ_NT;
                 const bool act = i <= b - a;
-                const uint32_t v = act && a + i < nh ? sc.D[a + i] : 0u;
+                const uint32_t v = act && a + i < nh ? dv[u] : 0u;
                 if (act) {
                     S.big.q.dist[DIX(i)] = (uint16_t)v;
                     S.big.q.lc[LIX(i)] = (uint8_t)(v >> 16);
@@ -1404,6 +1419,7 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
                 /* lanes 0..31 and 32..63 of a wave cover one 32-position segment each */
                 const uint64_t hb = __ballot((v & 0xFFFFu) != 0u);
                 if (act && (lane & 31u) == 0) S.sm.ps.hm[i >> 5] = (uint32_t)(hb >> (lane & 32u));
+            }
             }
             __syncthreads();
             if (dist_dbg)
