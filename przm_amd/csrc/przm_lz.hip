@@ -36,6 +36,9 @@
 
 #include "przm_common.h"
 
+/* a byte in global memory (address space 1), for pointers the compiler cannot trace */
+typedef __attribute__((address_space(1))) uint8_t gu8;
+
 #define LZ_NT 512
 #define LZ_NW 8
 #define QP 16384                  /* parse tile: a quarter block */
@@ -301,7 +304,8 @@ __device__ __forceinline__ uint32_t wg_scan(LzShared &S, uint32_t v, uint32_t *t
 __device__ __forceinline__ uint32_t leb_len(uint32_t v) {
     return v < 0x80u ? 1u : v < 0x4000u ? 2u : v < 0x200000u ? 3u : v < 0x10000000u ? 4u : 5u;
 }
-__device__ __forceinline__ uint32_t put_leb(uint8_t *b, uint32_t n, uint32_t v) {
+template <typename P>
+__device__ __forceinline__ uint32_t put_leb(P b, uint32_t n, uint32_t v) {
     while (v >= 0x80u) { b[n++] = (uint8_t)(v | 0x80u); v >>= 7; }
     b[n++] = (uint8_t)v;
     return n;
@@ -715,54 +719,62 @@ __device__ __forceinline__ int ring_use(uint32_t d, uint32_t &r0, uint32_t &r1, 
     return kk;
 }
 
-/* bytes of the staging areas in the dist / lc arrays (dead after the tokens are listed):
- * literals (at most a quarter's bytes), cmd and op, each with 32 bytes of slack for the
- * 16-byte reads of the copy-out */
-#define STG_LIT_CAP (QP + 32u)
-#ifndef STG_CMD_CAP
-#define STG_CMD_CAP 17408u /* a build with 0 here writes every quarter directly */
+/* the byte staging area of the write pass: the dist and lc arrays, dead once the tokens
+ * are listed; each stream takes its quarter's bytes plus 32 of slack for the 16-byte reads
+ * of the copy-out, rounded up to 16 */
+#define STG_BYTES ((DSZ * 2u + LSZ) & ~15u)
+#ifndef STG_LIMIT
+#define STG_LIMIT STG_BYTES /* a build with 0 here writes every quarter directly */
 #endif
-#define STG_OP_CAP 17408u
-static_assert(STG_LIT_CAP + STG_CMD_CAP + STG_OP_CAP + 32u <= DSZ * 2u + LSZ, "staging fits in dist + lc");
+__device__ __forceinline__ uint32_t stg_room(uint32_t n) { return (n + 32u + 15u) & ~15u; }
 
 /* the literal bytes in[cur .. p) (at most 32, inside one segment; none when the previous token
  * ends at or past p) to lit[lb ..]: the 36 bytes around them are loaded at once, then stored
  * one by one */
-__device__ __forceinline__ void copy_run(uint8_t *lit, uint32_t &lb, const uint8_t *in, uint32_t cur, uint32_t p) {
+template <typename P>
+__device__ __forceinline__ void copy_run(P lit, uint32_t &lb, const uint8_t *in, uint32_t cur, uint32_t p) {
     if (p <= cur) return;
     const uint32_t L = p - cur;
     const uint32_t *w = (const uint32_t *)(in + (cur & ~3u));
     const uint32_t sh = cur & 3u;
-    uint32_t wd[9], aw[8];
+    if (L <= 4) { /* the common short gap between tokens: two words */
+        const uint32_t a4 = __builtin_amdgcn_alignbyte(w[1], w[0], sh);
+        lit[lb] = (uint8_t)a4;
+        if (L > 1) lit[lb + 1] = (uint8_t)(a4 >> 8);
+        if (L > 2) lit[lb + 2] = (uint8_t)(a4 >> 16);
+        if (L > 3) lit[lb + 3] = (uint8_t)(a4 >> 24);
+        lb += L;
+        return;
+    }
+    uint32_t wd[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) wd[k] = w[k];
 #pragma unroll
-    for (int k = 0; k < 8; k++) aw[k] = __builtin_amdgcn_alignbyte(wd[k + 1], wd[k], sh);
-#pragma unroll
-    for (uint32_t t = 0; t < 32; t++) {
-        if (t >= L) break;
-        lit[lb + t] = (uint8_t)(aw[t >> 2] >> (8u * (t & 3u)));
+    for (uint32_t g = 0; g < 8; g++) { /* four bytes per step */
+        if (4u * g >= L) break;
+        const uint32_t v = __builtin_amdgcn_alignbyte(wd[g + 1], wd[g], sh);
+        const uint32_t o = lb + 4u * g, r = L - 4u * g;
+        lit[o] = (uint8_t)v;
+        if (r > 1) lit[o + 1] = (uint8_t)(v >> 8);
+        if (r > 2) lit[o + 2] = (uint8_t)(v >> 16);
+        if (r > 3) lit[o + 3] = (uint8_t)(v >> 24);
     }
     lb += L;
 }
 
 /*
  * The byte writes of one segment's tokens (przm_tokenizer.cuh:330-437 emission): literal gaps,
- * then each token.  STG: literal, cmd and op bytes go to the LDS staging areas at offsets
- * relative to the quarter's start (lb, b_cmd, b_op given so), the other streams to global
- * memory; otherwise every stream goes to global memory at absolute offsets.
+ * then each token.  Each stream is a base pointer and this segment's first index in it: the
+ * LDS staging areas with offsets inside the quarter, or the streams themselves with absolute
+ * offsets (the function is inlined into both, so each keeps its address space).
  */
-template <bool STG>
+template <typename LP>
 __device__ __forceinline__ void write_tokens(LzShared &S, uint32_t s, uint32_t ss, uint32_t se, uint32_t t0,
                                              uint32_t nt, const SegTok &tk, uint32_t kkp, uint32_t ls_in,
-                                             uint32_t lb, uint32_t b_cmd, uint32_t b_op, uint32_t b_meta,
-                                             uint32_t b_var, uint32_t b_off, uint8_t *glit, uint8_t *gcmd,
-                                             uint8_t *gop, uint8_t *meta, uint8_t *var, uint8_t *off) {
+                                             LP lit, uint32_t lb, LP cmd, uint32_t b_cmd, LP op, uint32_t b_op,
+                                             gu8 *meta, uint32_t b_meta, gu8 *var, uint32_t b_var, gu8 *off,
+                                             uint32_t b_off) {
     const uint8_t *in = S.in;
-    uint8_t *const LB = (uint8_t *)S.big.q.dist;
-    uint8_t *const lit = STG ? LB : glit;
-    uint8_t *const cmd = STG ? LB + STG_LIT_CAP : gcmd;
-    uint8_t *const op = STG ? LB + STG_LIT_CAP + STG_CMD_CAP : gop;
     uint32_t mr = 0, ls = ls_in, cur = t0;
     for (uint32_t k = 0; k <= nt; k++) {
         const uint32_t sk = k ^ (s & 7u);
@@ -821,13 +833,18 @@ __device__ __forceinline__ void write_tokens(LzShared &S, uint32_t s, uint32_t s
 
 /* n bytes from the LDS area src (4-byte aligned, readable 20 bytes past n) to dst: the
  * workgroup stores the 16-byte aligned body of dst as uint4s, the ends as bytes */
-__device__ __forceinline__ void lds_copy_out(uint8_t *__restrict__ dst, const uint8_t *src, uint32_t n) {
+__device__ __forceinline__ void lds_copy_out(gu8 *__restrict__ dst, const uint8_t *src, uint32_t n) {
     const uint32_t tid = threadIdx.x;
-    const uint32_t mis = (uint32_t)((uintptr_t)dst & 15u);
+    const uint32_t mis = (uint32_t)((uintptr_t)(const void *)(const uint8_t *)dst & 15u);
     const uint32_t head = umin(n, (16u - mis) & 15u);
     if (tid < head) dst[tid] = src[tid];
     const uint32_t body = (n - head) & ~15u;
-    for (uint32_t o = head + tid * 16u; o < head + body; o += LZ_NT * 16u) *(uint4 *)(dst + o) = ldw16(src, o);
+    typedef __attribute__((address_space(1))) uint32_t gu32;
+    for (uint32_t o = head + tid * 16u; o < head + body; o += LZ_NT * 16u) {
+        const uint4 v = ldw16(src, o);
+        gu32 *d4 = (gu32 *)(dst + o);
+        d4[0] = v.x; d4[1] = v.y; d4[2] = v.z; d4[3] = v.w;
+    }
     const uint32_t t0 = head + body;
     if (tid < n - t0) dst[t0 + tid] = src[t0 + tid];
 }
@@ -857,10 +874,14 @@ __device__ __forceinline__ Tok seg_tok(const LzShared &S, uint32_t cxm, uint32_t
 
 __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint8_t *__restrict__ wsb,
                            uint64_t *__restrict__ pts) {
-#define QTS(i) do { if (pts && threadIdx.x == 0) pts[i] = __builtin_amdgcn_s_memrealtime(); } while (0)
+#define QTS(i) do { if (pts && threadIdx.x == 0) ((__attribute__((address_space(1))) uint64_t *)pts)[i] = __builtin_amdgcn_s_memrealtime(); } while (0)
     const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
-    uint8_t *cmd = wsb + PRZM_WS_CMD, *off = wsb + PRZM_WS_OFF, *lit = wsb + PRZM_WS_LIT;
-    uint8_t *op = wsb + PRZM_WS_OP, *meta = wsb + PRZM_WS_META, *var = wsb + PRZM_WS_VAR;
+    /* the streams are in global memory: said in their type, since this function is called
+     * (not inlined) and would otherwise store through flat addresses, whose completion every
+     * later LDS wait would also wait for */
+    gu8 *const wsg = (gu8 *)wsb;
+    gu8 *cmd = wsg + PRZM_WS_CMD, *off = wsg + PRZM_WS_OFF, *lit = wsg + PRZM_WS_LIT;
+    gu8 *op = wsg + PRZM_WS_OP, *meta = wsg + PRZM_WS_META, *var = wsg + PRZM_WS_VAR;
     const uint8_t *in = S.in;
     const uint32_t s = tid, ss = a + s * SEG;
     const bool segv = ss < b;
@@ -1177,25 +1198,28 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
     uint32_t b_var = sv5[4] + S.st[ST_VAR];
     QTS(4);
 
-    /* 6. write bytes: literal gaps, then each token.  dist and lc are dead after step 4,
-     *    so when the quarter's cmd and op bytes fit, its literal, cmd and op bytes are staged
-     *    there and copied out with aligned 16-byte stores (scattered byte stores otherwise) */
-    const uint32_t lit0 = S.st[ST_LIT], cmd0 = S.st[ST_CMD], op0 = S.st[ST_OP];
-    const bool stage = t_cmd <= STG_CMD_CAP && t_op <= STG_OP_CAP;
+    /* 6. write bytes: literal gaps, then each token.  dist and lc are dead after step 4, so
+     *    when the quarter's literal, cmd and op bytes fit there they are staged in LDS and
+     *    copied out with aligned 16-byte stores (scattered byte stores otherwise); the small
+     *    meta, var and off streams are written directly */
+    const uint32_t lit0 = S.st[ST_LIT];
+    const uint32_t o_cmd = stg_room(tot_l), o_op = o_cmd + stg_room(t_cmd);
+    const bool stage = o_op + stg_room(t_op) <= STG_LIMIT; /* uniform */
+    uint8_t *const LB = (uint8_t *)S.big.q.dist;
     if (live) {
         if (stage)
-            write_tokens<true>(S, s, ss, se, t0, nt, tk, kkp, ls_in, lit_base - lit0, b_cmd - cmd0, b_op - op0,
-                               b_meta, b_var, b_off, lit, cmd, op, meta, var, off);
+            write_tokens(S, s, ss, se, t0, nt, tk, kkp, ls_in, LB, lit_base - lit0, LB + o_cmd, sv5[0], LB + o_op,
+                         sv5[2], meta, b_meta, var, b_var, off, b_off);
         else
-            write_tokens<false>(S, s, ss, se, t0, nt, tk, kkp, ls_in, lit_base, b_cmd, b_op, b_meta, b_var, b_off,
-                                lit, cmd, op, meta, var, off);
+            write_tokens(S, s, ss, se, t0, nt, tk, kkp, ls_in, lit, lit_base, cmd, b_cmd, op, b_op, meta, b_meta,
+                         var, b_var, off, b_off);
     }
     __syncthreads();
+    QTS(7);
     if (stage) {
-        uint8_t *const LB = (uint8_t *)S.big.q.dist;
         lds_copy_out(lit + lit0, LB, tot_l);
-        lds_copy_out(cmd + cmd0, LB + STG_LIT_CAP, t_cmd);
-        lds_copy_out(op + op0, LB + STG_LIT_CAP + STG_CMD_CAP, t_op);
+        lds_copy_out(cmd + (b_cmd - sv5[0]), LB + o_cmd, t_cmd);
+        lds_copy_out(op + (b_op - sv5[2]), LB + o_op, t_op);
     }
     __syncthreads();
     QTS(5);
@@ -1340,12 +1364,10 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
                 lds_fence();
             }
             __syncthreads();
-            if (H == 0) PTS(23);
             const uint32_t cnt = S.bstart[H][256];
             uint16_t *T = H ? T1w : T0w;
             for (uint32_t i = tid; i < cnt; i += LZ_NT) T[i] = S.big.stage[i];
             __syncthreads();
-            if (H == 0) PTS(24);
         }
         PTS(2);
 
@@ -1393,9 +1415,9 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             const uint32_t b = n < a + QP ? n : a + QP;
             /* the tile's match results [a, b] from the position-ordered results (positions
              * past n - 4 have none) */
-            /* eight loads at a time (unconditional, clamped into the array, so that they are in
+            /* twelve loads at a time (unconditional, clamped into the array, so that they are in
              * flight together), then their LDS writes; the same trip count in every lane */
-            constexpr uint32_t FR = (QP + 1 + LZ_NT - 1) / LZ_NT, FB = 8;
+            constexpr uint32_t FR = (QP + 1 + LZ_NT - 1) / LZ_NT, FB = 12;
             const uint32_t dlast = nh ? nh - 1u : 0u;
 #pragma unroll 1
             for (uint32_t r0 = 0; r0 < FR; r0 += FB) {
