@@ -392,6 +392,13 @@ __device__ __forceinline__ void lds_fence() {
 
 /* ---------------- phase C: one bucket ---------------- */
 
+/* a match result whose store is held back (one per lane): stores count in the same counter as
+ * loads, so the positions loaded at the next bucket's start would otherwise wait for the last
+ * chunk's stores; they are written once those loads are used.  a == NONE32: none */
+struct Pend {
+    uint32_t v, a;
+};
+
 struct Scr {
     const uint16_t *T0, *T1;
     uint32_t *D;      /* per position: match distance | min(length, 255) << 16 */
@@ -476,7 +483,8 @@ __device__ __forceinline__ uint32_t lcp_ext(const uint8_t *in, uint32_t p, uint3
 template <bool DEEP, uint32_t B, typename PT, typename ET>
 __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, uint32_t dgt, uint32_t n,
                                                     uint32_t depth, const PT *SRT, const ET *GE,
-                                                    uint32_t i0, uint32_t i1, uint32_t step, uint64_t *pt = nullptr) {
+                                                    uint32_t i0, uint32_t i1, uint32_t step, Pend &pd,
+                                                    uint64_t *pt = nullptr) {
     const bool prof = pt && threadIdx.x == 0;
     const uint8_t *in = S.in;
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
@@ -505,7 +513,7 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
             const uint4 k = ldw16(in, p);
             key = make_uint4(k.x, k.y, k.z, p);
             ring[64 + lane] = key;
-            const uint32_t low = hash4(in, p) & 0xFFu;
+            const uint32_t low = ((k.x * 0x1E35A7BDu) >> 16) & 0xFFu; /* hash4 of p from its key */
             const uint32_t gs = low ? GE[low - 1] : 0u;
             cap = n - p < PRZM_MAX_MATCH ? n - p : PRZM_MAX_MATCH;
             kmax = i - gs < depth ? i - gs : depth;
@@ -566,9 +574,11 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         }
         lds_fence();
         const uint64_t t_c = prof ? __builtin_amdgcn_s_memrealtime() : 0;
-        if (valid) { /* by position: dist | min(len, 255) << 16, 0 for no match */
+        if (pd.a != NONE32) sc.D[pd.a] = pd.v; /* the previous chunk's result */
+        { /* by position: dist | min(len, 255) << 16, 0 for no match */
             const uint32_t best = thr >> 3;
-            sc.D[p] = best >= 4u ? (p - bq) | (umin(best, 255u) << 16) : 0u;
+            pd.v = best >= 4u ? (p - bq) | (umin(best, 255u) << 16) : 0u;
+            pd.a = valid ? p : NONE32;
         }
         if (prof) {
             const uint64_t t_d = __builtin_amdgcn_s_memrealtime();
@@ -579,9 +589,27 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
 
 /* group the bucket by the hash's low byte (stable), then search it unless it
  * is large; returns true when the bucket was queued for the all-wave search */
+/* the first 256 positions of bucket dgt, element u * 64 + lane in pre[u] (undefined past the
+ * bucket's end).  The loads are unconditional (an index past the end reads T0[0]) so that no
+ * branch makes their number unknown to the wait-count bookkeeping. */
+__device__ __forceinline__ void bucket_fetch(const LzShared &S, const Scr &sc, uint32_t dgt, uint32_t pre[4]) {
+    const uint32_t lane = __lane_id();
+    uint32_t s0 = 0, m0 = 0, s1 = 0, m1 = 0;
+    if (dgt < 256) {
+        s0 = S.bstart[0][dgt]; m0 = S.bstart[0][dgt + 1] - s0;
+        s1 = S.bstart[1][dgt]; m1 = S.bstart[1][dgt + 1] - s1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const uint32_t j = u * 64 + lane;
+        const uint16_t *a = j < m0 ? sc.T0 + s0 + j : j < m0 + m1 ? sc.T1 + s1 + (j - m0) : sc.T0;
+        pre[u] = *a;
+    }
+}
+
 template <bool LDSWS>
 __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt, uint32_t n, uint32_t depth,
-                            bool may_queue, uint64_t *pt) {
+                            bool may_queue, const uint32_t pre[4], Pend &pd, uint64_t *pt) {
     /* pt (diagnostics, thread 0 only): accumulated sort, chunk staging, candidate scan and result
      * write times, and chunks */
     const bool prof = pt && threadIdx.x == 0;
@@ -604,7 +632,7 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const uint32_t j = c0 + u * 64 + lane;
-            pv[u] = j < m ? (j < m0 ? sc.T0[s0 + j] : sc.T1[s1 + j - m0]) : 0u;
+            pv[u] = c0 == 0 ? pre[u] : j < m ? (j < m0 ? sc.T0[s0 + j] : sc.T1[s1 + j - m0]) : 0u;
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -618,6 +646,10 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     if (!LDSWS) __threadfence_block();
     lds_fence();
     if (prof) pt[-2] += __builtin_amdgcn_s_memrealtime() - t_a;
+    if (pd.a != NONE32) { /* this bucket's positions are in: the held-back result can go now */
+        sc.D[pd.a] = pd.v;
+        pd.a = NONE32;
+    }
     /* exclusive scan of the 256 digit counts (4 per lane) */
     {
         uint32_t c4[4], t = 0;
@@ -651,9 +683,9 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
         return true;
     }
     if (prof) pt[0] += t_c - t_a;
-    if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pt);
-    else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pt);
-    else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pt);
+    if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pd, pt);
+    else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pd, pt);
+    else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pd, pt);
     return false;
 }
 
@@ -1375,19 +1407,34 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
         if (tid == 0) S.st[ST_BUCKET] = 0;
         __threadfence_block();
         __syncthreads();
-        for (;;) {
+        Pend pd = {0u, NONE32};
+        {
+            /* the next bucket is taken, and its first positions requested, before the current
+             * one is worked (the chunk results held back in pd keep that wait short) */
             uint32_t dg = 0;
             if (lane == 0) dg = atomicAdd(&S.st[ST_BUCKET], 1u);
             dg = __shfl(dg, 0);
-            if (dg >= 256) break;
-            const uint32_t m = (S.bstart[0][dg + 1] - S.bstart[0][dg]) + (S.bstart[1][dg + 1] - S.bstart[1][dg]);
-            if (m == 0) continue;
-            if (m <= WS_CAP) bucket_sort<true>(S, sc, w, dg, n, depth, false, ts ? ts + 27 : nullptr);
-            else {
-                const bool q = bucket_sort<false>(S, sc, w, dg, n, depth, S.bigq[w] == 0, ts ? ts + 27 : nullptr);
-                if (q && lane == 0) S.bigq[w] = dg + 1;
+            uint32_t pre[4];
+            bucket_fetch(S, sc, dg, pre);
+            while (dg < 256) {
+                uint32_t dn = 0;
+                if (lane == 0) dn = atomicAdd(&S.st[ST_BUCKET], 1u);
+                dn = __shfl(dn, 0);
+                uint32_t nxt[4];
+                bucket_fetch(S, sc, dn, nxt);
+                const uint32_t m = (S.bstart[0][dg + 1] - S.bstart[0][dg]) + (S.bstart[1][dg + 1] - S.bstart[1][dg]);
+                if (m != 0 && m <= WS_CAP) bucket_sort<true>(S, sc, w, dg, n, depth, false, pre, pd, ts ? ts + 27 : nullptr);
+                else if (m != 0) {
+                    const bool q = bucket_sort<false>(S, sc, w, dg, n, depth, S.bigq[w] == 0, pre, pd,
+                                                      ts ? ts + 27 : nullptr);
+                    if (q && lane == 0) S.bigq[w] = dg + 1;
+                }
+                dg = dn;
+#pragma unroll
+                for (int u = 0; u < 4; u++) pre[u] = nxt[u];
             }
         }
+        if (pd.a != NONE32) sc.D[pd.a] = pd.v;
         if (ts && lane == 0 && (w == 0 || w == LZ_NW - 1)) ts[w == 0 ? 14 : 15] = __builtin_amdgcn_s_memrealtime();
         __threadfence_block();
         __syncthreads();
@@ -1400,9 +1447,11 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             const uint8_t *gw = sc.gws + (uint64_t)ow * GWS_STRIDE;
             const uint16_t *SRT = (const uint16_t *)gw + PRZM_BLOCK;
             const uint32_t *GE = (const uint32_t *)(gw + 2u * PRZM_BLOCK * 2u);
-            if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64);
-            else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64);
-            else bucket_search_range<false, SEARCH_BATCH>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64);
+            Pend pq = {0u, NONE32};
+            if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64, pq);
+            else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64, pq);
+            else bucket_search_range<false, SEARCH_BATCH>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64, pq);
+            if (pq.a != NONE32) sc.D[pq.a] = pq.v;
         }
         __threadfence_block();
         __syncthreads();
