@@ -627,12 +627,23 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     uint32_t *cnt = S.sm.tab[w];
     for (int k = 0; k < 4; k++) cnt[lane * 4 + k] = 0;
     lds_fence();
-    for (uint32_t c0 = 0; c0 < m; c0 += 256) {
+    /* the first 256 elements' positions and low digits stay in registers for the scatter */
+    uint32_t lw0[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const uint32_t j = u * 64 + lane;
+        lw0[u] = hash4(in, pre[u]) & 0xFFu;
+        if (j < m) {
+            P[j] = (uint16_t)pre[u];
+            atomicAdd(&cnt[lw0[u]], 1u);
+        }
+    }
+    for (uint32_t c0 = 256; c0 < m; c0 += 256) {
         uint32_t pv[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const uint32_t j = c0 + u * 64 + lane;
-            pv[u] = c0 == 0 ? pre[u] : j < m ? (j < m0 ? sc.T0[s0 + j] : sc.T1[s1 + j - m0]) : 0u;
+            pv[u] = j < m ? (j < m0 ? sc.T0[s0 + j] : sc.T1[s1 + j - m0]) : 0u;
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -658,11 +669,9 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
         for (int k = 0; k < 4; k++) { cnt[lane * 4 + k] = run; run += c4[k]; }
     }
     lds_fence();
-    for (uint32_t c0 = 0; c0 < m; c0 += 64) {
-        const uint32_t j = c0 + lane;
+    auto scatter = [&](uint32_t j, uint32_t pj, uint32_t low) {
         const bool valid = j < m;
-        const uint32_t pj = valid ? P[j] : 0u;
-        const uint32_t low = valid ? (hash4(in, pj) & 0xFFu) : 0u;
+        low = valid ? low : 0u;
         const uint64_t pm = peers8(low, valid);
         const uint32_t base = valid ? cnt[low] : 0u;
         lds_fence();
@@ -671,6 +680,16 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
             if ((pm & lt) == 0) cnt[low] = base + (uint32_t)__popcll(pm);
         }
         lds_fence();
+    };
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) {
+        if (u * 64 >= m) break;
+        scatter(u * 64 + lane, pre[u], lw0[u]);
+    }
+    for (uint32_t c0 = 256; c0 < m; c0 += 64) {
+        const uint32_t j = c0 + lane;
+        const uint32_t pj = j < m ? P[j] : 0u;
+        scatter(j, pj, hash4(in, pj) & 0xFFu);
     }
     if (!LDSWS) __threadfence_block();
     lds_fence();
