@@ -627,7 +627,7 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     uint32_t *cnt = S.sm.tab[w];
     for (int k = 0; k < 4; k++) cnt[lane * 4 + k] = 0;
     lds_fence();
-    /* the first 256 elements' positions and low digits stay in registers for the scatter */
+    /* the first 256 elements' positions and low digits stay in registers for the scatter, */
     uint32_t lw0[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -638,7 +638,25 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
             atomicAdd(&cnt[lw0[u]], 1u);
         }
     }
-    for (uint32_t c0 = 256; c0 < m; c0 += 256) {
+    /* and so do the next 256 (most buckets hold fewer than 512) */
+    uint32_t pv1[4] = {0u, 0u, 0u, 0u}, lw1[4] = {0u, 0u, 0u, 0u};
+    if (m > 256) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t j = 256 + u * 64 + lane;
+            pv1[u] = j < m ? (j < m0 ? sc.T0[s0 + j] : sc.T1[s1 + j - m0]) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t j = 256 + u * 64 + lane;
+            lw1[u] = hash4(in, pv1[u]) & 0xFFu;
+            if (j < m) {
+                P[j] = (uint16_t)pv1[u];
+                atomicAdd(&cnt[lw1[u]], 1u);
+            }
+        }
+    }
+    for (uint32_t c0 = 512; c0 < m; c0 += 256) {
         uint32_t pv[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -686,7 +704,12 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
         if (u * 64 >= m) break;
         scatter(u * 64 + lane, pre[u], lw0[u]);
     }
-    for (uint32_t c0 = 256; c0 < m; c0 += 64) {
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) {
+        if (256 + u * 64 >= m) break;
+        scatter(256 + u * 64 + lane, pv1[u], lw1[u]);
+    }
+    for (uint32_t c0 = 512; c0 < m; c0 += 64) {
         const uint32_t j = c0 + lane;
         const uint32_t pj = j < m ? P[j] : 0u;
         scatter(j, pj, hash4(in, pj) & 0xFFu);
