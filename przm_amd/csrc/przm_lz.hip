@@ -469,9 +469,9 @@ __device__ __forceinline__ uint32_t lcp_ext(const uint8_t *in, uint32_t p, uint3
  *
  * Each wave stages the first 12 bytes and the position of the current and the
  * previous chunk of 64 elements in an LDS ring, so a candidate costs one
- * conflict-free 16-byte read and a branch-free compare.  Lengths are kept as
- * bit indices of the first difference (8 * length + 0..7), so "longer" is one
- * unsigned compare and the best so far is the threshold 8 * best + 7.
+ * conflict-free 16-byte read and a branch-free compare.  A candidate's score is
+ * length << 16 | its position, so the best of a chain (longest, then nearest) is
+ * a plain maximum of scores.
  *
  * No per-candidate range test: a ring entry before the element's group start
  * is another element of the same bucket with a different 16-bit hash, whose
@@ -520,64 +520,62 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         }
         lds_fence();
         const uint64_t t_b = prof ? __builtin_amdgcn_s_memrealtime() : 0;
-        /* thr = 8 * best + 7 with best = 3 (lengths below 4 do not count); capb caps a
-         * key compare at min(cap, 12) bytes; bit indices >= 96 mean all 12 key bytes agree */
-        uint32_t thr = 31u, bq = 0u;
-        const uint32_t capb = 8u * umin(cap, 12u) + 7u, capt = 8u * cap + 7u;
+        /* best = L << 16 | q of the best candidate so far.  A group lists its positions in
+         * ascending order, so a chain's candidates come nearest first with descending q and a
+         * plain maximum keeps the nearest of equal lengths.  L = 3 to start: lengths below 4 do
+         * not count.  capb caps a key compare at min(cap, 12) bytes; lim12 is the score from
+         * which a candidate agrees on all 12 key bytes with more to compare (never when cap <= 12). */
+        uint32_t best = 3u << 16;
+        const uint32_t capb = 8u * umin(cap, 12u) + 7u, bcap = cap << 16;
+        const uint32_t lim12 = cap > 12u ? (12u << 16) : 0xFFFFFFFFu;
         for (uint32_t k0 = 1; k0 <= depth; k0 += B) { /* depth is a multiple of B */
-            if (!__ballot(k0 <= kmax && thr < capt)) break;
-            uint4 qk[B];
-            uint32_t lb[B];
+            if (!__ballot(k0 <= kmax && best < bcap)) break;
+            uint32_t sco[B];
             /* candidates k0 .. k0 + B - 1 sit below one another in the ring */
             const uint4 *rq = ring + (64 + (int32_t)lane - (int32_t)k0 - (int32_t)(B - 1));
-            uint32_t mx = 0;
+            uint32_t mc = 0;
 #pragma unroll
             for (uint32_t u = 0; u < B; u++) { /* in flight together */
                 const uint32_t k = k0 + u;
+                uint4 qk;
                 if (DEEP && k > lane + 64u) {
                     if (k <= kmax) {
                         const uint32_t q = (uint32_t)SRT[i - k];
                         const uint4 b = ldw16(in, q);
-                        qk[u] = make_uint4(b.x, b.y, b.z, q);
+                        qk = make_uint4(b.x, b.y, b.z, q);
                     } else {
-                        qk[u] = filler;
+                        qk = filler;
                     }
                 } else {
-                    qk[u] = rq[B - 1 - u];
+                    qk = rq[B - 1 - u];
                 }
-                lb[u] = umin(diff12(key.x ^ qk[u].x, key.y ^ qk[u].y, key.z ^ qk[u].z), capb);
-                mx = mx > lb[u] ? mx : lb[u];
+                const uint32_t lb = umin(diff12(key.x ^ qk.x, key.y ^ qk.y, key.z ^ qk.z), capb);
+                sco[u] = ((lb >> 3) << 16) | qk.w;
+                mc = mc > sco[u] ? mc : sco[u];
             }
-            if (__ballot(mx >= 96u && cap > 12u)) {
+            if (__ballot(mc >= lim12)) {
                 /* some candidate agrees on all 12 key bytes: its exact length decides.  Once
                  * the best reaches 28 bytes a candidate can only beat it if it agrees at byte
-                 * `best` (one byte compare before the full one). */
+                 * `bl` (one byte compare before the full one). */
 #pragma unroll
                 for (uint32_t u = 0; u < B; u++) {
-                    if (lb[u] >= 96u && cap > 12u) {
-                        const uint32_t q = qk[u].w, best = thr >> 3;
-                        if (best < cap && (best < 28u || in[p + best] == in[q + best]))
-                            lb[u] = 8u * lcp_ext(in, p, q, cap);
+                    if (sco[u] >= lim12) {
+                        const uint32_t q = sco[u] & 0xFFFFu, bl = best >> 16;
+                        if (bl < cap && (bl < 28u || in[p + bl] == in[q + bl]))
+                            sco[u] = (lcp_ext(in, p, q, cap) << 16) | q;
                     }
-                    const bool better = lb[u] > thr;
-                    thr = better ? (lb[u] | 7u) : thr;
-                    bq = better ? qk[u].w : bq;
+                    best = best > sco[u] ? best : sco[u];
                 }
             } else {
-#pragma unroll
-                for (uint32_t u = 0; u < B; u++) {
-                    const bool better = lb[u] > thr;
-                    thr = better ? (lb[u] | 7u) : thr;
-                    bq = better ? qk[u].w : bq;
-                }
+                best = best > mc ? best : mc;
             }
         }
         lds_fence();
         const uint64_t t_c = prof ? __builtin_amdgcn_s_memrealtime() : 0;
         if (pd.a != NONE32) sc.D[pd.a] = pd.v; /* the previous chunk's result */
         { /* by position: dist | min(len, 255) << 16, 0 for no match */
-            const uint32_t best = thr >> 3;
-            pd.v = best >= 4u ? (p - bq) | (umin(best, 255u) << 16) : 0u;
+            const uint32_t L = best >> 16;
+            pd.v = L >= 4u ? (p - (best & 0xFFFFu)) | (umin(L, 255u) << 16) : 0u;
             pd.a = valid ? p : NONE32;
         }
         if (prof) {
