@@ -492,6 +492,18 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
     /* a 4-byte word whose hash4 high byte is not dgt: hash4(0) = 0, hash4(1) = 0x1E35 */
     const uint32_t fz = dgt == 0 ? 1u : 0u;
     const uint4 filler = make_uint4(fz, fz, fz, 0u);
+    /* this lane's position in the current chunk (pc) and the next (pn), and the four words that
+     * hold the current key's bytes: loaded one chunk ahead (indexes past the range repeat its
+     * last element), so that staging a chunk waits only for its group starts */
+    const uint32_t last = i1 - 1u;
+    uint32_t pc = 0, pn = 0, kw0 = 0, kw1 = 0, kw2 = 0, kw3 = 0;
+    if (i0 < i1) {
+        pc = SRT[umin(i0 + lane, last)];
+        pn = SRT[umin(i0 + step + lane, last)];
+        const uint32_t *wp = (const uint32_t *)(in + (pc & ~3u));
+        kw0 = wp[0]; kw1 = wp[1]; kw2 = wp[2]; kw3 = wp[3];
+    }
+    uint4 key = make_uint4(0, 0, 0, 0);
     for (uint32_t c0 = i0; c0 < i1; c0 += step) {
         const uint64_t t_a = prof ? __builtin_amdgcn_s_memrealtime() : 0;
         const uint32_t i = c0 + lane;
@@ -499,24 +511,32 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         /* ring[0, 64): the previous chunk, ring[64, 128): this chunk, so candidate k of
          * lane l is entry 64 + l - k (k <= 64 + l) */
         if (c0 < 64) ring[lane] = filler;
-        else if (step == 64 && c0 != i0) ring[lane] = ring[64 + lane]; /* this wave staged it last time */
+        else if (step == 64 && c0 != i0) ring[lane] = key; /* this lane's key of the last chunk, which was full */
         else {
             const uint32_t ip = c0 - 64 + lane, pp = SRT[ip];
             const uint4 k = ldw16(in, pp);
             ring[lane] = make_uint4(k.x, k.y, k.z, pp);
         }
-        lds_fence();
+        const uint32_t sh = pc & 3u;
+        key = make_uint4(__builtin_amdgcn_alignbyte(kw1, kw0, sh), __builtin_amdgcn_alignbyte(kw2, kw1, sh),
+                         __builtin_amdgcn_alignbyte(kw3, kw2, sh), pc);
+        /* lanes past the range stage a copy of its last element, which no chain of the range reads */
+        ring[64 + lane] = key;
         uint32_t p = 0, kmax = 0, cap = 0;
-        uint4 key = make_uint4(0, 0, 0, 0);
         if (valid) {
-            p = SRT[i];
-            const uint4 k = ldw16(in, p);
-            key = make_uint4(k.x, k.y, k.z, p);
-            ring[64 + lane] = key;
-            const uint32_t low = ((k.x * 0x1E35A7BDu) >> 16) & 0xFFu; /* hash4 of p from its key */
+            p = pc;
+            const uint32_t low = ((key.x * 0x1E35A7BDu) >> 16) & 0xFFu; /* hash4 of p from its key */
             const uint32_t gs = low ? GE[low - 1] : 0u;
             cap = n - p < PRZM_MAX_MATCH ? n - p : PRZM_MAX_MATCH;
             kmax = i - gs < depth ? i - gs : depth;
+        } else {
+            key = make_uint4(0, 0, 0, 0);
+        }
+        if (c0 + step < i1) { /* the next chunk's key words and the position after it */
+            const uint32_t *wp = (const uint32_t *)(in + (pn & ~3u));
+            kw0 = wp[0]; kw1 = wp[1]; kw2 = wp[2]; kw3 = wp[3];
+            pc = pn;
+            pn = SRT[umin(c0 + 2u * step + lane, last)];
         }
         lds_fence();
         const uint64_t t_b = prof ? __builtin_amdgcn_s_memrealtime() : 0;
