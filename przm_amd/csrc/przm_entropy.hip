@@ -303,23 +303,28 @@ __device__ __forceinline__ uint4 rev16(uint4 v) {
 /* a readable zero chunk: the load address of chunks outside a job's stream */
 __device__ __align__(16) uint4 przm_zero16;
 
-/* One coder step of quad lane j for step g (symbols 4g .. 4g + 3; this lane's is byte j of
- * dw, lower is the word of step g - 1), live for symbols below nl.  Updates the state x and
- * the emitted byte count K. */
+/* The table entry of quad lane j's symbol for step g (symbols 4g .. 4g + 3; this lane's is byte
+ * j of dw, lower is the word of step g - 1): it depends on the symbols alone, so a chunk's
+ * entries are read while the chunk before it is coded. */
 template <bool M8>
-__device__ __forceinline__ void rans_step(uint32_t g, uint32_t dw, uint32_t lower, uint32_t j, uint32_t nl,
-                                          const uint32_t *__restrict__ tab, uint8_t *__restrict__ stg,
-                                          uint8_t *dummy, uint8_t *__restrict__ slot_end, uint32_t &x,
-                                          uint32_t &K) {
-    const uint32_t i = 4 * g + j;
-    const bool v = i < nl;
+__device__ __forceinline__ uint32_t rans_entry(uint32_t g, uint32_t dw, uint32_t lower, uint32_t j,
+                                               const uint32_t *__restrict__ tab) {
     uint32_t idx = (dw >> (8 * j)) & 0xFFu;
     if (M8) {
         const uint32_t prev = j ? (dw >> (8 * j - 8)) & 0xFFu : lower >> 24;
-        idx += (i > 0 ? ctx_meta(prev) : 0u) << 8;
+        idx += (4 * g + j > 0 ? ctx_meta(prev) : 0u) << 8;
     }
-    /* a lane that is not live reads some entry of its table and changes nothing */
-    const uint32_t e = tab[idx];
+    return tab[idx];
+}
+
+/* One coder step of quad lane j for step g with its table entry e (freq | cdf << 16), live for
+ * symbols below nl; a lane that is not live has some entry of its table and changes nothing.
+ * Updates the state x and the emitted byte count K. */
+__device__ __forceinline__ void rans_step(uint32_t g, uint32_t e, uint32_t j, uint32_t nl,
+                                          uint8_t *__restrict__ stg, uint8_t *dummy,
+                                          uint8_t *__restrict__ slot_end, uint32_t &x, uint32_t &K) {
+    const uint32_t i = 4 * g + j;
+    const bool v = i < nl;
     const uint32_t f = e & 0xFFFFu, cdf = e >> 16;
     const uint32_t t = 4096u * f - 1u;
     const bool r1 = v && x > t;
@@ -397,17 +402,35 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
     uint4 buf[RANS_PF];
 #pragma unroll
     for (uint32_t d = 0; d < RANS_PF; d++) buf[d] = *chunk(NC - 1 - (int32_t)d);
+    /* entries of the chunk being coded; slot s + 1 of the ring holds the chunk below slot s's */
+    uint32_t e[4];
+    {
+        const uint32_t g0 = 4u * (uint32_t)(NC - 1);
+        e[0] = rans_entry<M8>(g0 + 3, buf[0].w, buf[0].z, j, tab);
+        e[1] = rans_entry<M8>(g0 + 2, buf[0].z, buf[0].y, j, tab);
+        e[2] = rans_entry<M8>(g0 + 1, buf[0].y, buf[0].x, j, tab);
+        e[3] = rans_entry<M8>(g0, buf[0].x, buf[1].w, j, tab);
+    }
     for (int32_t cb = NC - 1; cb >= 0; cb -= (int32_t)RANS_PF) {
 #pragma unroll
         for (uint32_t d = 0; d < RANS_PF; d++) {
             const int32_t c = cb - (int32_t)d;
-            /* the chunk below c: the next slot, or for the last slot the one just reloaded */
-            const uint32_t nw = d + 1 < RANS_PF ? buf[d + 1].w : buf[0].w;
+            /* the entries of chunk c - 1 (its slot was reloaded earlier in this turn when it wraps) */
+            const uint4 nb = buf[(d + 1) % RANS_PF];
+            const uint32_t nlw = buf[(d + 2) % RANS_PF].w;
+            const uint32_t gn = 4u * (uint32_t)(c - 1);
+            uint32_t en[4];
+            en[0] = rans_entry<M8>(gn + 3, nb.w, nb.z, j, tab);
+            en[1] = rans_entry<M8>(gn + 2, nb.z, nb.y, j, tab);
+            en[2] = rans_entry<M8>(gn + 1, nb.y, nb.x, j, tab);
+            en[3] = rans_entry<M8>(gn, nb.x, nlw, j, tab);
             const uint32_t g0 = 4u * (uint32_t)c;
-            rans_step<M8>(g0 + 3, buf[d].w, buf[d].z, j, nl, tab, stg, dummy, slot_end, x, K);
-            rans_step<M8>(g0 + 2, buf[d].z, buf[d].y, j, nl, tab, stg, dummy, slot_end, x, K);
-            rans_step<M8>(g0 + 1, buf[d].y, buf[d].x, j, nl, tab, stg, dummy, slot_end, x, K);
-            rans_step<M8>(g0, buf[d].x, nw, j, nl, tab, stg, dummy, slot_end, x, K);
+            rans_step(g0 + 3, e[0], j, nl, stg, dummy, slot_end, x, K);
+            rans_step(g0 + 2, e[1], j, nl, stg, dummy, slot_end, x, K);
+            rans_step(g0 + 1, e[2], j, nl, stg, dummy, slot_end, x, K);
+            rans_step(g0, e[3], j, nl, stg, dummy, slot_end, x, K);
+#pragma unroll
+            for (int u = 0; u < 4; u++) e[u] = en[u];
             buf[d] = *chunk(c - (int32_t)RANS_PF);
             if (K > cap) { /* the job is not chosen any more: stop it */
                 over = true;
