@@ -92,7 +92,10 @@ struct LzShared {
         } q;
     } big;
     union {
-        uint32_t tab[LZ_NW][256];
+        struct {
+            uint32_t tab[LZ_NW][256];
+            uint64_t pbits[LZ_NW][256]; /* per wave and digit: the lanes holding it (zero between uses) */
+        };
         struct {
             uint32_t bmp[NSEG];
             uint32_t xpos[NSEG];
@@ -201,22 +204,6 @@ __device__ __forceinline__ uint32_t RUN(const uint8_t *in, uint32_t p, uint32_t 
 }
 
 /* ---------------- wave / workgroup helpers ---------------- */
-
-/* lanes of the wave whose 8-bit digit d equals this lane's (valid lanes only): per bit, keep
- * the lanes that agree with this lane's bit, i.e. m &= ~(ballot ^ t) with t = 0 or ~0 the bit
- * sign-extended, one v_bitop3 per 32-bit half */
-__device__ __forceinline__ uint64_t peers8(uint32_t d, bool valid) {
-    const uint64_t v = __ballot(valid);
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-        const uint32_t t = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
-        const uint64_t bb = __ballot(t != 0u);
-        lo &= ~((uint32_t)bb ^ t);
-        hi &= ~((uint32_t)(bb >> 32) ^ t);
-    }
-    return ((uint64_t)hi << 32) | lo;
-}
 
 /* DPP move of x with control CTRL on rows ROWMASK; lanes whose source is out of range, and
  * rows outside the mask, read 0 */
@@ -388,6 +375,28 @@ __device__ Tok eval_tok(const LzShared &S, uint32_t p, uint32_t a, uint32_t n, u
 __device__ __forceinline__ void lds_fence() {
     __builtin_amdgcn_wave_barrier();
     __asm__ volatile("" ::: "memory");
+}
+
+/* Lanes of the wave whose digit d (< 256) equals this lane's, valid lanes only, and the
+ * counter cnt[d] as it was before this step: each lane sets its bit in the wave's word pb[d]
+ * (one LDS atomic OR), then reads the word and the counter back and clears the word, so the
+ * words are zero between calls.  The LDS executes a wave's operations in order, so all ORs
+ * land before any read and all reads before any clear. */
+__device__ __forceinline__ uint64_t peers_lds(uint64_t *pb, const uint32_t *cnt, uint32_t d, bool valid,
+                                              uint32_t *cur) {
+    const uint64_t me = 1ull << __lane_id();
+    if (valid) atomicOr((unsigned long long *)&pb[d], (unsigned long long)me);
+    lds_fence();
+    uint64_t pm = 0;
+    uint32_t c = 0;
+    if (valid) {
+        pm = pb[d];
+        c = cnt[d];
+    }
+    lds_fence();
+    if (valid) pb[d] = 0ull;
+    *cur = c;
+    return pm;
 }
 
 /* ---------------- phase C: one bucket ---------------- */
@@ -708,9 +717,8 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     auto scatter = [&](uint32_t j, uint32_t pj, uint32_t low) {
         const bool valid = j < m;
         low = valid ? low : 0u;
-        const uint64_t pm = peers8(low, valid);
-        const uint32_t base = valid ? cnt[low] : 0u;
-        lds_fence();
+        uint32_t base;
+        const uint64_t pm = peers_lds(S.sm.pbits[w], cnt, low, valid, &base);
         if (valid) {
             SRT[base + (uint32_t)__popcll(pm & lt)] = (uint16_t)pj; /* the position itself, in group order */
             if ((pm & lt) == 0) cnt[low] = base + (uint32_t)__popcll(pm);
@@ -1404,6 +1412,7 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
         }
         if (tid < ST_N) S.st[tid] = 0;
         if (tid < LZ_NW) S.bigq[tid] = 0;
+        for (uint32_t i = tid; i < LZ_NW * 256; i += LZ_NT) (&S.sm.pbits[0][0])[i] = 0ull;
         __syncthreads();
         if (tid == 0) { S.st[ST_R0] = 1; S.st[ST_R1] = 4; S.st[ST_R2] = 8; }
 
@@ -1417,9 +1426,15 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             for (uint32_t i = tid; i < LZ_NW * 256; i += LZ_NT) (&S.sm.tab[0][0])[i] = 0;
             __syncthreads();
             const uint32_t wb = base + w * (PRZM_HALF / LZ_NW);
-            for (uint32_t c = 0; c < PRZM_HALF / LZ_NW; c += 64) {
-                const uint32_t p = wb + c + lane;
-                if (p < end) atomicAdd(&S.sm.tab[w][hash4(S.in, p) >> 8], 1u);
+            /* eight digits at a time, their loads unconditional (positions below PRZM_BLOCK, whose
+             * words lie inside in[]) so that they are in flight together */
+            for (uint32_t c = 0; c < PRZM_HALF / LZ_NW; c += 8 * 64) {
+                uint32_t dgs[8];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; u++) dgs[u] = hash4(S.in, wb + c + u * 64 + lane) >> 8;
+#pragma unroll
+                for (uint32_t u = 0; u < 8; u++)
+                    if (wb + c + u * 64 + lane < end) atomicAdd(&S.sm.tab[w][dgs[u]], 1u);
             }
             __syncthreads();
             {
@@ -1442,13 +1457,15 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             }
             __syncthreads();
             const uint64_t lt = (1ull << lane) - 1ull;
+            /* the digit of the next step is loaded one step ahead */
+            uint32_t dnext = hash4(S.in, wb + lane) >> 8;
             for (uint32_t c = 0; c < PRZM_HALF / LZ_NW; c += 64) {
                 const uint32_t p = wb + c + lane;
                 const bool valid = p < end;
-                const uint32_t dg = valid ? hash4(S.in, p) >> 8 : 0u;
-                const uint64_t pm = peers8(dg, valid);
-                const uint32_t cur = valid ? S.sm.tab[w][dg] : 0u;
-                lds_fence();
+                const uint32_t dg = valid ? dnext : 0u;
+                dnext = hash4(S.in, p + 64 < PRZM_BLOCK ? p + 64 : PRZM_BLOCK) >> 8;
+                uint32_t cur;
+                const uint64_t pm = peers_lds(S.sm.pbits[w], S.sm.tab[w], dg, valid, &cur);
                 if (valid) {
                     S.big.stage[cur + (uint32_t)__popcll(pm & lt)] = (uint16_t)p;
                     if ((pm & lt) == 0) S.sm.tab[w][dg] = cur + (uint32_t)__popcll(pm);
