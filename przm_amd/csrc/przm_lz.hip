@@ -584,13 +584,13 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
             }
             if (__ballot(mc >= lim12)) {
                 /* some candidate agrees on all 12 key bytes: its exact length decides.  Once
-                 * the best reaches 28 bytes a candidate can only beat it if it agrees at byte
+                 * the best reaches 12 bytes a candidate can only beat it if it agrees at byte
                  * `bl` (one byte compare before the full one). */
 #pragma unroll
                 for (uint32_t u = 0; u < B; u++) {
                     if (sco[u] >= lim12) {
                         const uint32_t q = sco[u] & 0xFFFFu, bl = best >> 16;
-                        if (bl < cap && (bl < 28u || in[p + bl] == in[q + bl]))
+                        if (bl < cap && (bl < 12u || in[p + bl] == in[q + bl]))
                             sco[u] = (lcp_ext(in, p, q, cap) << 16) | q;
                     }
                     best = best > sco[u] ? best : sco[u];
