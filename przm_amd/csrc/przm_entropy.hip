@@ -673,8 +673,35 @@ przm_scan_kernel(const uint32_t *__restrict__ len, uint32_t count, uint64_t *__r
     if (tid == 1023) *run = base + part[1023];
 }
 
+/* dst[0, n) = src[0, n) by the workgroup: bytes up to dst's next 16-byte boundary and after
+ * the last full 16 bytes one by one, the rest as aligned 16-byte stores whose bytes come from
+ * aligned source words shifted into place (a source word read is one that holds a byte of
+ * the range, so it lies inside the source's allocation) */
 __device__ __forceinline__ void wr_bytes(uint8_t *dst, const uint8_t *src, uint32_t n) {
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+    const uint32_t t = threadIdx.x, nt = blockDim.x;
+    const uint32_t lead = (16u - (uint32_t)((uintptr_t)dst & 15u)) & 15u;
+    const uint32_t head = lead < n ? lead : n;
+    const uint32_t body = (n - head) & ~15u;
+    for (uint32_t i = t; i < head; i += nt) dst[i] = src[i];
+    for (uint32_t i = head + body + t; i < n; i += nt) dst[i] = src[i];
+    if (!body) return;
+    uint4 *d = (uint4 *)(dst + head);
+    const uint8_t *s = src + head;
+    const uint32_t sh = (uint32_t)((uintptr_t)s & 3u);
+    const uint32_t *sw = (const uint32_t *)(s - sh);
+    if (sh == 0) {
+        for (uint32_t i = t; i < (body >> 4); i += nt) {
+            const uint32_t *w = sw + 4 * i;
+            d[i] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    } else {
+        for (uint32_t i = t; i < (body >> 4); i += nt) {
+            const uint32_t *w = sw + 4 * i;
+            const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4];
+            d[i] = make_uint4(__builtin_amdgcn_alignbyte(w1, w0, sh), __builtin_amdgcn_alignbyte(w2, w1, sh),
+                              __builtin_amdgcn_alignbyte(w3, w2, sh), __builtin_amdgcn_alignbyte(w4, w3, sh));
+        }
+    }
 }
 __device__ __forceinline__ void wr_le32(uint8_t *p, uint32_t v) {
     p[0] = (uint8_t)v; p[1] = (uint8_t)(v >> 8); p[2] = (uint8_t)(v >> 16); p[3] = (uint8_t)(v >> 24);
