@@ -333,13 +333,15 @@ __device__ __forceinline__ void rans_step(uint32_t g, uint32_t e, uint32_t j, ui
     const uint32_t xn = r2 ? x1 >> 8 : x1;
     const uint32_t ne = (uint32_t)r1 + (uint32_t)r2;
     /* bytes of higher lanes come first: lane 3 emits before lane 2, ...  The quad's counts
-     * (0..2 each) as nibbles, summed over the quad by two quad permutes; this lane's offset
-     * is the sum of the nibbles above its own, the step's total the sum of all four. */
-    const uint32_t pn = ne << (4u * j);
+     * (0..2 each) as bytes, gathered over the quad by two quad permutes; this lane's offset
+     * is the sum of the bytes above its own (v_sad_u8 against zero adds a word's four bytes),
+     * the step's total the sum of all four. */
+    const uint32_t pn = ne << (8u * j);
     const uint32_t qn = pn + QPERM(pn, 0xB1);  /* lanes [1, 0, 3, 2] */
     const uint32_t rn = qn + QPERM(qn, 0x4E);  /* lanes [2, 3, 0, 1] */
-    const uint32_t after = (__umul24(rn >> (4u * j + 4u), 0x111u) >> 8) & 0xFu;
-    const uint32_t tot = (__umul24(rn, 0x1111u) >> 12) & 0xFu;
+    const uint32_t above = (uint32_t)(0xFFFFFF00ull << (8u * j)); /* bytes of lanes j + 1 .. 3 */
+    const uint32_t after = __builtin_amdgcn_sad_u8(rn & above, 0u, 0u);
+    const uint32_t tot = __builtin_amdgcn_sad_u8(rn, 0u, 0u);
     const uint32_t k0 = K + after;
     *(r1 ? stg + (k0 & (RANS_STAGE - 1)) : dummy) = (uint8_t)x;
     *(r2 ? stg + ((k0 + 1) & (RANS_STAGE - 1)) : dummy) = (uint8_t)x1;

@@ -49,19 +49,25 @@ def test_biased_reciprocal_divmod():
             assert np.array_equal(rr, (x % f).astype(np.int64)), f
 
 
-def test_quad_nibble_offsets():
-    """rans_step: with each lane's byte count (0..2) as a nibble at 4j, the sum over the quad R
-    gives lane j's offset ((R >> (4j + 4)) * 0x111 >> 8) & 15 = sum of the counts of lanes
-    above j and the step total (R * 0x1111 >> 12) & 15, for all 81 combinations."""
+def _sad0(x):
+    """v_sad_u8(x, 0, 0): the sum of the four bytes of x"""
+    return sum((x >> (8 * i)) & 0xFF for i in range(4))
+
+
+def test_quad_byte_offsets():
+    """rans_step: with each lane's byte count (0..2) as byte j of R (the sum over the quad), lane
+    j's offset sad_u8(R & above_j, 0, 0) with above_j = (0xFFFFFF00 << 8j) mod 2^32 is the sum of
+    the counts of lanes above j, and sad_u8(R, 0, 0) the step total, for all 81 combinations."""
     for c0 in range(3):
         for c1 in range(3):
             for c2 in range(3):
                 for c3 in range(3):
                     ne = [c0, c1, c2, c3]
-                    R = sum(n << (4 * j) for j, n in enumerate(ne))
-                    assert ((R * 0x1111) >> 12) & 15 == sum(ne)
+                    R = sum(n << (8 * j) for j, n in enumerate(ne))
+                    assert _sad0(R) == sum(ne)
                     for j in range(4):
-                        assert (((R >> (4 * j + 4)) * 0x111) >> 8) & 15 == sum(ne[j + 1:])
+                        above = (0xFFFFFF00 << (8 * j)) & 0xFFFFFFFF
+                        assert _sad0(R & above) == sum(ne[j + 1:])
 
 
 def _ffbl(x):
