@@ -93,13 +93,35 @@ def test_frames_match_oracle_and_decode(g):
             assert R.decompress_frames(frames, sizes) == a.tobytes(), name
 
 
-@pytest.mark.parametrize("level", [1, 3, 5, 9])
+@pytest.mark.parametrize("level", [1, 3, 5, 9, 10, 11])
 def test_levels_match_oracle(g, level):
+    """Every chain depth class: 2, 4, 8 (the short-chain search), 32, 64 (chains inside the
+    staged ring) and 128 (chains past the ring, read from the block)."""
     O = oracle()
     a = D.mixture(1 << 20, level)
     frames, sizes, _ = g.CompressFrames(a, level)
     of, osz = O.encode_buffer(a, level)
     assert frames == of and np.array_equal(sizes, osz)
+
+
+def test_deep_chains_on_edge_inputs(g):
+    """Level 11 (chains of 128, most of them longer than the staged ring) on inputs whose groups
+    are longest: all zeros, a period of 2, periodic text, a three-letter alphabet, byte runs.
+    The degenerate ones are kept short: the oracle's chain walk costs depth x match length."""
+    O = oracle()
+    rng = np.random.default_rng(17)
+    cases = {
+        "zeros": np.zeros(8000, np.uint8),
+        "ab": np.frombuffer(b"ab" * 4000, np.uint8).copy(),
+        "text": np.frombuffer(b"The quick brown fox jumps over the lazy dog. " * 180, np.uint8).copy(),
+        "lowalpha": rng.integers(0, 3, 70000, dtype=np.uint8),
+        "runs": np.repeat(rng.integers(0, 256, 3000, dtype=np.uint8), rng.integers(1, 40, 3000))[:90000].copy(),
+        "markov": D.markov(65537, 65537),
+    }
+    for name, a in cases.items():
+        frames, sizes, _ = g.CompressFrames(a, 11)
+        of, osz = O.encode_buffer(a, 11)
+        assert np.array_equal(sizes, osz) and frames == of, name
 
 
 def test_histogram_and_vans_match(g):
