@@ -291,7 +291,7 @@ __constant__ M8Table przm_m8 = M8Table();
 #define QPERM(v, ctrl) ((uint32_t)__builtin_amdgcn_mov_dpp((int)(v), (ctrl), 0xF, 0xF, false))
 
 
-#define RANS_STAGE 128u /* per-job LDS staging of emitted bytes: two 64-byte halves */
+#define RANS_STAGE 256u /* per-job LDS staging of emitted bytes: two 128-byte halves */
 
 /* reverse the byte order of a 16-byte word */
 __device__ __forceinline__ uint4 rev16(uint4 v) {
@@ -347,13 +347,17 @@ __device__ __forceinline__ void rans_step(uint32_t g, uint32_t e, uint32_t j, ui
     *(r2 ? stg + ((k0 + 1) & (RANS_STAGE - 1)) : dummy) = (uint8_t)x1;
     const uint32_t K2 = K + tot;
     __builtin_amdgcn_wave_barrier();
-    if ((K2 >> 6) != (K >> 6)) { /* the run [64m, 64m + 64) is complete: m = K >> 6 */
-        const uint32_t m = K >> 6;
-        /* lane j stores bytes k = 64m + 48 - 16j .. 64m + 63 - 16j, which sit at
-         * slot_end - 64(m + 1) + 16j + (63 - (k - 64m) - 16j) ascending */
-        const uint32_t k_lo = 64u * m + 48u - 16u * j;
-        const uint4 w = *(const uint4 *)(stg + (k_lo & (RANS_STAGE - 1)));
-        *(uint4 *)(slot_end - 64u * (m + 1u) + 16u * j) = rev16(w);
+    if ((K2 >> 7) != (K >> 7)) { /* the run [128m, 128m + 128) is complete: m = K >> 7 */
+        const uint32_t m = K >> 7;
+        /* lane j stores bytes k = 128m + 96 - 32j .. 128m + 127 - 32j, which sit at
+         * slot_end - 128(m + 1) + 32j + (127 - (k - 128m) - 32j) ascending: the upper 16 of
+         * them, reversed, first */
+        const uint32_t k_lo = 128u * m + 96u - 32u * j;
+        const uint4 w1 = *(const uint4 *)(stg + ((k_lo + 16u) & (RANS_STAGE - 1)));
+        const uint4 w0 = *(const uint4 *)(stg + (k_lo & (RANS_STAGE - 1)));
+        uint8_t *d = slot_end - 128u * (m + 1u) + 32u * j;
+        *(uint4 *)d = rev16(w1);
+        *(uint4 *)(d + 16) = rev16(w0);
     }
     __builtin_amdgcn_wave_barrier();
     K = K2;
@@ -383,9 +387,10 @@ __device__ __forceinline__ void rans_step(uint32_t g, uint32_t e, uint32_t j, ui
  * every slot has room for.
  *
  * Output byte k (k = 0, 1, ... in emission order) belongs at slot_end[-1-k].
- * Bytes are first written to the job's LDS staging ring stg[k % 128]; each
- * time a 64-byte run [64m, 64m + 64) is complete the quad stores it as four
- * aligned 16-byte words (slot_end is 16-byte aligned).  At most cap bytes;
+ * Bytes are first written to the job's LDS staging ring stg[k % 256]; each
+ * time a 128-byte run [128m, 128m + 128) is complete the quad stores it as
+ * eight aligned 16-byte words, two per lane (slot_end is 16-byte aligned), so
+ * a wave takes the store branch half as often as with 64-byte runs.  At most cap bytes;
  * returns the byte count (or overflow) and the lane's final state.
  */
 template <bool M8>
@@ -440,8 +445,8 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
             }
         }
     }
-    if (!over) { /* the last, partial run [64 (K >> 6), K) */
-        const uint32_t base = K & ~63u;
+    if (!over) { /* the last, partial run [128 (K >> 7), K) */
+        const uint32_t base = K & ~127u;
         for (uint32_t k = base + j; k < K; k += 4) slot_end[-1 - (int64_t)k] = stg[k & (RANS_STAGE - 1)];
     }
     *state = x;

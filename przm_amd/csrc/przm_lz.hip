@@ -1025,6 +1025,7 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
         }
     }
 
+    QTS(8);
     /* 1. speculative walk of every segment from its start */
     if (segv) {
         uint32_t bm = 0, p = ss, xp = 0;
