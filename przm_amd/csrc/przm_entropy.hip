@@ -288,7 +288,7 @@ struct M8Table {
 __constant__ M8Table przm_m8 = M8Table();
 
 /* value of lane sel[j] of the quad for quad lane j; ctrl = sel0 | sel1 << 2 | sel2 << 4 | sel3 << 6 */
-#define QPERM(v, ctrl) ((uint32_t)__builtin_amdgcn_mov_dpp((int)(v), (ctrl), 0xF, 0xF, false))
+#define QPERM(v, ctrl) ((uint32_t)__builtin_amdgcn_update_dpp(0, (int)(v), (ctrl), 0xF, 0xF, false))
 
 
 #define RANS_STAGE 256u /* per-job LDS staging of emitted bytes: two 128-byte halves */
