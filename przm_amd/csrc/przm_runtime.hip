@@ -934,16 +934,18 @@ int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs,
                           int32_t tuned[3]) {
     if (!g || !bufs || !nbufs) return PRZM_GPU_ERR_ARG;
     CHK(mock_check());
-    /* sample: buffers from the front up to ~1 GiB (several chunks of the larger shapes) */
+    /* sample: buffers from the front up to ~4 GiB, so that the sample spans several launches of
+     * the smaller chunk shapes: each launch ends on its longest coder job (a stream's rANS steps
+     * are sequential), so the number of launches, not only their size, sets the time */
     uint32_t ns = 0;
     uint64_t bytes = 0;
-    while (ns < nbufs && bytes < (1ull << 30)) bytes += bufs[ns++].len;
+    while (ns < nbufs && bytes < (4ull << 30)) bytes += bufs[ns++].len;
     uint64_t bound = 0;
     for (uint32_t i = 0; i < ns; i++) bound += przm_gpu_frames_bound(bufs[i].len);
     void *d_out = nullptr;
     CHK(przm_gpu_malloc(g, bound + 64, &d_out));
     const int grids[] = {g->cus, 2 * g->cus};
-    const int chunks[] = {4096, 8192, 16384};
+    const int chunks[] = {16384, 32768, 65536}; /* 65536 blocks: 32 GB of block workspace */
     double best = 1e30;
     int bg = g->lz_grid, bc = (int)g->chunk;
     int32_t rc = 0;
