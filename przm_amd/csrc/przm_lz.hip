@@ -1542,32 +1542,41 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             const uint32_t b = n < a + QP ? n : a + QP;
             /* the tile's match results [a, b] from the position-ordered results (positions
              * past n - 4 have none) */
-            /* twelve loads at a time (unconditional, clamped into the array, so that they are in
-             * flight together), then their LDS writes; the same trip count in every lane */
-            constexpr uint32_t FR = (QP + 1 + LZ_NT - 1) / LZ_NT, FB = 12;
-            const uint32_t dlast = nh ? nh - 1u : 0u;
+            /* four positions per 16-byte load: thread t of round r holds positions 4t + 2048r ..
+             * 4t + 2048r + 3 (a wave reads 1 KiB contiguous); the nine rounds go in two groups
+             * whose loads are all issued before any is used, unconditionally (clamped into the
+             * array; what lies past nh is masked), so the fill waits for two memory latencies */
+            constexpr uint32_t FR4 = (QP + 1 + 4 * LZ_NT - 1) / (4 * LZ_NT), FB4 = 5;
 #pragma unroll 1
-            for (uint32_t r0 = 0; r0 < FR; r0 += FB) {
-            uint32_t dv[FB];
+            for (uint32_t r0 = 0; r0 < FR4; r0 += FB4) {
+            uint4 dv[FB4];
 #pragma unroll
-            for (uint32_t u = 0; u < FB; u++) {
-                const uint32_t e = a + tid + (r0 + u) * LZ_NT;
-                dv[u] = sc.D[e < dlast ? e : dlast];
+            for (uint32_t u = 0; u < FB4; u++) {
+                const uint32_t e = a + 4u * tid + (r0 + u) * 4u * LZ_NT;
+                dv[u] = *(const uint4 *)(sc.D + (e < PRZM_BLOCK - 4u ? e : PRZM_BLOCK - 4u));
             }
 #pragma unroll
-            for (uint32_t u = 0; u < FB; u++) {
-                const uint32_t r = r0 + u;
-                if (r >= FR) break;
-                const uint32_t i = tid + r * LZ_NT;
-                const bool act = i <= b - a;
-                const uint32_t v = act && a + i < nh ? dv[u] : 0u;
-                if (act) {
-                    S.big.q.dist[DIX(i)] = (uint16_t)v;
-                    S.big.q.lc[LIX(i)] = (uint8_t)(v >> 16);
+            for (uint32_t u = 0; u < FB4; u++) {
+                if (r0 + u >= FR4) break;
+                const uint32_t i = 4u * tid + (r0 + u) * 4u * LZ_NT;
+                const uint32_t vv[4] = {dv[u].x, dv[u].y, dv[u].z, dv[u].w};
+                uint32_t nib = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) {
+                    const bool act = i + j <= b - a;
+                    const uint32_t v = act && a + i + j < nh ? vv[j] : 0u;
+                    if (act) {
+                        S.big.q.dist[DIX(i + j)] = (uint16_t)v;
+                        S.big.q.lc[LIX(i + j)] = (uint8_t)(v >> 16);
+                    }
+                    nib |= ((v & 0xFFFFu) != 0u ? 1u : 0u) << j;
                 }
-                /* lanes 0..31 and 32..63 of a wave cover one 32-position segment each */
-                const uint64_t hb = __ballot((v & 0xFFFFu) != 0u);
-                if (act && (lane & 31u) == 0) S.sm.ps.hm[i >> 5] = (uint32_t)(hb >> (lane & 32u));
+                /* eight lanes cover one 32-position segment: their nibbles make its match mask */
+                uint32_t hv = nib << (4u * (lane & 7u));
+                hv |= __shfl_xor(hv, 1);
+                hv |= __shfl_xor(hv, 2);
+                hv |= __shfl_xor(hv, 4);
+                if ((lane & 7u) == 0 && i <= b - a) S.sm.ps.hm[i >> 5] = hv;
             }
             }
             __syncthreads();
