@@ -1476,7 +1476,10 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             __syncthreads();
             const uint32_t cnt = S.bstart[H][256];
             uint16_t *T = H ? T1w : T0w;
-            for (uint32_t i = tid; i < cnt; i += LZ_NT) T[i] = S.big.stage[i];
+            /* eight positions per 16-byte store (both arrays are 16-byte aligned), the tail one by one */
+            const uint32_t c8 = cnt & ~7u;
+            for (uint32_t i = tid * 8u; i < c8; i += LZ_NT * 8u) *(uint4 *)(T + i) = *(const uint4 *)(S.big.stage + i);
+            for (uint32_t i = c8 + tid; i < cnt; i += LZ_NT) T[i] = S.big.stage[i];
             __syncthreads();
         }
         PTS(2);
