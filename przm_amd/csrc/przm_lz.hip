@@ -2,34 +2,42 @@
  * przm_lz.hip -- per-block match finder + tokenizer for gfx950 (CDNA4).
  *
  * One workgroup (8 wave64s) owns one 64 KiB block at a time and loops over
- * blocks (persistent grid).  The block's bytes live in LDS for the whole
- * kernel.  Result: the block's command / offset / literal streams and the
- * Section-2 substreams (op, meta, var), byte-identical to the oracle.
+ * blocks (persistent grid, blocks handed out by an atomic counter).  The
+ * block's bytes live in LDS for the whole kernel.  Result: the block's
+ * command / offset / literal streams and the Section-2 substreams (op, meta,
+ * var), byte-identical to the oracle.
  *
  * Match finding (compress.go:681-707, 857-862, 374-387).  The chain of p is
- * "every earlier position with the same 16-bit hash4, newest first".  With
- * 160 KiB of LDS there is no room for a 65536-entry head table next to the
- * block, so chains are built by grouping instead of by sequential insertion:
+ * "every earlier position with the same 16-bit hash4, newest first", at most
+ * `depth` of them.  With 160 KiB of LDS there is no room for a 65536-entry
+ * head table next to the block, so chains are built by grouping instead of
+ * by sequential insertion:
  *   A. positions are bucketed by the hash's high byte with a stable LDS
- *      scatter (per-wave digit cursors + wave-ballot peer ranks), one half
- *      block at a time, and spilled to global scratch (coalesced);
- *   C. each bucket (position ordered) is walked by one wave: a 256-entry
- *      head table on the hash's low byte gives each element its in-bucket
- *      predecessor (== same full hash), and the search walks that chain
- *      right away against the LDS copy of the block.  Results (pos, dist)
- *      go to global scratch in bucket order;
- *   E. per half block, results are scattered back into an LDS dist array.
+ *      scatter (per-wave digit cursors; the lanes sharing a digit found
+ *      through a per-wave bitmap word per digit), one half block at a time,
+ *      and spilled to global scratch as 16-byte stores;
+ *   C. each bucket is taken by one wave (large ones are spilled and searched
+ *      by all eight): its positions are grouped stably by the hash's low
+ *      byte, so each group is one full hash in position order (a chain read
+ *      backward), and searched 64 elements at a time from a per-wave LDS ring
+ *      of 12-byte keys; a candidate's score is length << 16 | position, so
+ *      the best of a chain is a plain maximum.  Results go to a per-position
+ *      array in global scratch;
+ *   E. per quarter block, the results are read back into LDS (dist, length,
+ *      and a "has a match" mask per 32 positions).
  *
- * Tokenizing (przm_tokenizer.cuh:115-466, one tile == the block).  The
- * reference walks tokens with one lane.  Here every 64-byte segment of a
- * half block is walked speculatively by its own thread from the segment
- * start; one wave then follows the true path across segments (a true entry
- * that lies on the speculative path inherits its exit; otherwise that
- * segment is re-walked), resolving the tokens that cross segment ends with
- * wave-wide compares.  Emission is two scans over segments (literal and
- * token counts, then byte counts) and a write pass; the ring-register index
- * of each match comes from a backward scan for the last three distinct
- * distances.
+ * Tokenizing (przm_tokenizer.cuh:115-466, one tile == a quarter block).  The
+ * reference walks tokens with one lane.  Here every 32-byte segment of the
+ * quarter is classified and walked speculatively by its own thread from the
+ * segment start; the workgroup then follows the true path across segments by
+ * pointer doubling over segment -> segment-of-exit, re-walking segments whose
+ * entry is off their speculative path and resolving tokens that cross
+ * segment ends with wave-wide compares.  Emission is two scans over segments
+ * (literal and token counts, then the five per-stream byte counts) and a
+ * write pass that keeps each segment's tokens in registers; the literal, cmd
+ * and op bytes are staged in LDS and stored as aligned 16-byte words.  The
+ * ring-register index of each match comes from a backward scan for the last
+ * three distinct distances.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
