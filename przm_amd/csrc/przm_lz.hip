@@ -1012,25 +1012,34 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
         for (int j = 0; j < 10; j++) lw[j] = lwp[j];
 #pragma unroll
         for (int j = 0; j < 9; j++) iw[j] = iwp[j];
+        /* per position k the byte predicates as bits of 32-bit masks; the position
+         * conditions (p + 1 < n, p + 12 <= n, p < se) and the logic then act on whole masks */
+        uint32_t E = 0, G = 0, E1 = 0, GT = 0, RW = 0;
 #pragma unroll
         for (uint32_t k = 0; k < SEG; k++) {
-            const uint32_t p = ss + k;
             const uint32_t lc = (lw[k >> 2] >> (8u * (k & 3u))) & 0xFFu;
             const uint32_t l1c = k + 1 < SEG ? (lw[(k + 1) >> 2] >> (8u * ((k + 1) & 3u))) & 0xFFu : lw[9] & 0xFFu;
-            const bool dk = (h0 >> k) & 1u;
-            const bool d1 = k + 1 < SEG ? (bool)((h0 >> (k + 1)) & 1u) : (bool)(h1 & 1u);
             const uint32_t w4 = (k & 3u) ? __builtin_amdgcn_alignbyte(iw[(k >> 2) + 1], iw[k >> 2], k & 3u) : iw[k >> 2];
-            const bool la = p + 1 < n && d1;
-            const bool la_cx = la && l1c == 255u && lc >= 254u;
-            const bool la_lit = la && !la_cx && l1c > lc + 1u;
-            const bool run = p + 12 <= n && w4 == (w4 & 0xFFu) * 0x01010101u;
-            const bool lit = !dk || (lc != 255u && la_lit);
-            const bool cx = dk && (lc == 255u || (!la_lit && (la_cx || run)));
-            if (p < se) {
-                litm |= (uint32_t)lit << k;
-                cxm |= (uint32_t)cx << k;
-            }
+            E |= (uint32_t)(lc == 255u) << k;
+            G |= (uint32_t)(lc >= 254u) << k;
+            E1 |= (uint32_t)(l1c == 255u) << k;
+            GT |= (uint32_t)(l1c > lc + 1u) << k;
+            RW |= (uint32_t)(w4 == (w4 & 0xFFu) * 0x01010101u) << k;
         }
+        /* positions k < lim of the segment, lim clamped to [0, 32] */
+        auto below = [](int32_t lim) -> uint32_t {
+            return lim >= 32 ? 0xFFFFFFFFu : lim <= 0 ? 0u : (1u << lim) - 1u;
+        };
+        const uint32_t PM1 = below((int32_t)n - 1 - (int32_t)ss);   /* p + 1 < n */
+        const uint32_t PM12 = below((int32_t)n - 11 - (int32_t)ss); /* p + 12 <= n */
+        const uint32_t SE = below((int32_t)(se - ss));               /* p < se */
+        const uint32_t D = h0, D1 = (h0 >> 1) | (h1 << 31);
+        const uint32_t LA = D1 & PM1;
+        const uint32_t LACX = LA & E1 & G;
+        const uint32_t LALIT = LA & ~LACX & GT;
+        const uint32_t RUN = RW & PM12;
+        litm = (~D | (~E & LALIT)) & SE;
+        cxm = (D & (E | (~LALIT & (LACX | RUN)))) & SE;
     }
 
     QTS(8);
