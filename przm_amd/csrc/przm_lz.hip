@@ -1024,7 +1024,7 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
             G |= (uint32_t)(lc >= 254u) << k;
             E1 |= (uint32_t)(l1c == 255u) << k;
             GT |= (uint32_t)(l1c > lc + 1u) << k;
-            RW |= (uint32_t)(w4 == (w4 & 0xFFu) * 0x01010101u) << k;
+            RW |= (uint32_t)(w4 == __builtin_amdgcn_alignbit(w4, w4, 8u)) << k; /* four equal bytes: w4 is its own byte rotation */
         }
         /* positions k < lim of the segment, lim clamped to [0, 32] */
         auto below = [](int32_t lim) -> uint32_t {
