@@ -54,6 +54,9 @@ typedef __attribute__((address_space(1))) uint8_t gu8;
 #define NSEG (QP / SEG)           /* 512: one per thread */
 #define WS_CAP 1536               /* bucket elements kept in LDS per wave */
 #define RING 128                  /* staged search keys per wave: previous + current chunk */
+#ifndef GORD
+#define GORD 2u                   /* classes of the per-bucket group layout (see bucket_sort) */
+#endif
 #ifndef SEARCH_BATCH
 #define SEARCH_BATCH 4            /* chain candidates read per step of the search */
 #endif
@@ -71,7 +74,7 @@ typedef __attribute__((address_space(1))) uint8_t gu8;
 #define LIX(i) ((i) + (((i) >> 5) << 2))
 #define DSZ ((DIX(QP + 1) + 3) & ~1) /* even, so that lc below starts 4-byte aligned */
 #define LSZ (LIX(QP + 1) + 8)
-#define GWS_SLOT (2u * PRZM_BLOCK * 2u + 256u * 4u) /* one large bucket: positions, order, group ends */
+#define GWS_SLOT (2u * PRZM_BLOCK * 2u + 256u * 4u) /* one large bucket: positions, order, group starts */
 #define GWS_STRIDE (2u * GWS_SLOT)                   /* per wave: the queued bucket + one worked in place */
 
 enum { T_LIT = 0, T_RLE = 1, T_MATCH = 2 };
@@ -103,6 +106,7 @@ struct LzShared {
         struct {
             uint32_t tab[LZ_NW][256];
             uint64_t pbits[LZ_NW][256]; /* per wave and digit: the lanes holding it (zero between uses) */
+            uint16_t gstart[LZ_NW][256]; /* per wave and low digit: where its group starts in the bucket */
         };
         struct {
             uint32_t bmp[NSEG];
@@ -479,7 +483,7 @@ __device__ __forceinline__ uint32_t lcp_ext(const uint8_t *in, uint32_t p, uint3
 
 /*
  * Search elements [i0, i1) (sorted order) of a bucket whose positions in group
- * order are SRT and per-digit group ends GE.  Element i's chain is
+ * order are SRT and per-digit group starts GE.  Element i's chain is
  * the up to `depth` elements before it in its group (same 16-bit hash,
  * newest first, compress.go:681-707); the longest common prefix wins, ties
  * keep the nearest, and only lengths >= 4 count.
@@ -543,7 +547,7 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         if (valid) {
             p = pc;
             const uint32_t low = ((key.x * 0x1E35A7BDu) >> 16) & 0xFFu; /* hash4 of p from its key */
-            const uint32_t gs = low ? GE[low - 1] : 0u;
+            const uint32_t gs = GE[low]; /* where the element's group starts */
             cap = n - p < PRZM_MAX_MATCH ? n - p : PRZM_MAX_MATCH;
             kmax = i - gs < depth ? i - gs : depth;
         } else {
@@ -714,12 +718,44 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
         sc.D[pd.a] = pd.v;
         pd.a = NONE32;
     }
-    /* exclusive scan of the 256 digit counts (4 per lane) */
+    /* Group starts.  A chunk of 64 elements takes as many search steps as its longest chain, so
+     * the groups are laid out in GORD classes by the steps their last element needs,
+     * ceil(min(size - 1, depth) / SEARCH_BATCH), most first (GORD = 2: groups with a chain, then
+     * single positions), each class in digit order (one wave scan per class): chunks then hold
+     * chains of like length.  A chain only needs its group to be contiguous and in position
+     * order, so the order of the groups changes no result. */
     {
-        uint32_t c4[4], t = 0;
-        for (int k = 0; k < 4; k++) { c4[k] = cnt[lane * 4 + k]; t += c4[k]; }
-        uint32_t run = wave_incl_sum(t) - t;
-        for (int k = 0; k < 4; k++) { cnt[lane * 4 + k] = run; run += c4[k]; }
+        uint16_t *GS = S.sm.gstart[w];
+        uint32_t c4[4], k4[4], tot[GORD];
+#pragma unroll
+        for (int c = 0; c < (int)GORD; c++) tot[c] = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            c4[k] = cnt[lane * 4 + k];
+            const uint32_t steps = (umin(c4[k] ? c4[k] - 1u : 0u, depth) + SEARCH_BATCH - 1u) / SEARCH_BATCH;
+            k4[k] = (GORD - 1u) - umin(steps, GORD - 1u);
+#pragma unroll
+            for (int c = 0; c < (int)GORD; c++) tot[c] += k4[k] == (uint32_t)c ? c4[k] : 0u;
+        }
+        uint32_t run[GORD], base = 0;
+#pragma unroll
+        for (int c = 0; c < (int)GORD; c++) {
+            const uint32_t incl = wave_incl_sum(tot[c]);
+            run[c] = base + incl - tot[c];
+            base += __shfl(incl, 63);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint32_t st = 0;
+#pragma unroll
+            for (int c = 0; c < (int)GORD; c++) {
+                const bool me = k4[k] == (uint32_t)c;
+                st = me ? run[c] : st;
+                run[c] += me ? c4[k] : 0u;
+            }
+            cnt[lane * 4 + k] = st;
+            GS[lane * 4 + k] = (uint16_t)st;
+        }
     }
     lds_fence();
     auto scatter = [&](uint32_t j, uint32_t pj, uint32_t low) {
@@ -750,18 +786,18 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     }
     if (!LDSWS) __threadfence_block();
     lds_fence();
-    /* cnt[d] now holds the end of digit d's group */
     if (prof) t_c = __builtin_amdgcn_s_memrealtime();
     if (!LDSWS && may_queue) {
         uint32_t *ge = (uint32_t *)(gw + 2u * PRZM_BLOCK * 2u);
-        for (int k = 0; k < 4; k++) ge[lane * 4 + k] = cnt[lane * 4 + k];
+        for (int k = 0; k < 4; k++) ge[lane * 4 + k] = S.sm.gstart[w][lane * 4 + k];
         __threadfence_block();
         return true;
     }
     if (prof) pt[0] += t_c - t_a;
-    if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pd, pt);
-    else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pd, pt);
-    else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, cnt, 0, m, 64, pd, pt);
+    const uint16_t *GS = S.sm.gstart[w];
+    if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, GS, 0, m, 64, pd, pt);
+    else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, GS, 0, m, 64, pd, pt);
+    else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, GS, 0, m, 64, pd, pt);
     return false;
 }
 
