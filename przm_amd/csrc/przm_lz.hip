@@ -742,7 +742,7 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
         for (int c = 0; c < (int)GORD; c++) {
             const uint32_t incl = wave_incl_sum(tot[c]);
             run[c] = base + incl - tot[c];
-            base += __shfl(incl, 63);
+            base += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -1547,13 +1547,13 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
              * one is worked (the chunk results held back in pd keep that wait short) */
             uint32_t dg = 0;
             if (lane == 0) dg = atomicAdd(&S.st[ST_BUCKET], 1u);
-            dg = __shfl(dg, 0);
+            dg = (uint32_t)__builtin_amdgcn_readlane((int)dg, 0);
             uint32_t pre[4];
             bucket_fetch(S, sc, dg, pre);
             while (dg < 256) {
                 uint32_t dn = 0;
                 if (lane == 0) dn = atomicAdd(&S.st[ST_BUCKET], 1u);
-                dn = __shfl(dn, 0);
+                dn = (uint32_t)__builtin_amdgcn_readlane((int)dn, 0);
                 uint32_t nxt[4];
                 bucket_fetch(S, sc, dn, nxt);
                 const uint32_t m = (S.bstart[0][dg + 1] - S.bstart[0][dg]) + (S.bstart[1][dg + 1] - S.bstart[1][dg]);
