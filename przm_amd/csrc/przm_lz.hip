@@ -54,9 +54,6 @@ typedef __attribute__((address_space(1))) uint8_t gu8;
 #define NSEG (QP / SEG)           /* 512: one per thread */
 #define WS_CAP 1536               /* bucket elements kept in LDS per wave */
 #define RING 128                  /* staged search keys per wave: previous + current chunk */
-#ifndef GORD
-#define GORD 2u                   /* classes of the per-bucket group layout (see bucket_sort) */
-#endif
 #ifndef SEARCH_BATCH
 #define SEARCH_BATCH 4            /* chain candidates read per step of the search */
 #endif
@@ -719,40 +716,29 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
         pd.a = NONE32;
     }
     /* Group starts.  A chunk of 64 elements takes as many search steps as its longest chain, so
-     * the groups are laid out in GORD classes by the steps their last element needs,
-     * ceil(min(size - 1, depth) / SEARCH_BATCH), most first (GORD = 2: groups with a chain, then
-     * single positions), each class in digit order (one wave scan per class): chunks then hold
-     * chains of like length.  A chain only needs its group to be contiguous and in position
-     * order, so the order of the groups changes no result. */
+     * the groups with a chain (two or more positions) are laid out first and the single
+     * positions after them, each in digit order: chunks then hold chains of like length.  A
+     * chain only needs its group to be contiguous and in position order, so the layout changes
+     * no result.  One wave scan serves both classes: a lane's chain elements count in the low
+     * 20 bits of the scanned value (at most 65536), its single positions from bit 20 (at most
+     * 256). */
     {
         uint16_t *GS = S.sm.gstart[w];
-        uint32_t c4[4], k4[4], tot[GORD];
-#pragma unroll
-        for (int c = 0; c < (int)GORD; c++) tot[c] = 0;
+        uint32_t c4[4], tp = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             c4[k] = cnt[lane * 4 + k];
-            const uint32_t steps = (umin(c4[k] ? c4[k] - 1u : 0u, depth) + SEARCH_BATCH - 1u) / SEARCH_BATCH;
-            k4[k] = (GORD - 1u) - umin(steps, GORD - 1u);
-#pragma unroll
-            for (int c = 0; c < (int)GORD; c++) tot[c] += k4[k] == (uint32_t)c ? c4[k] : 0u;
+            tp += c4[k] == 1u ? (1u << 20) : c4[k];
         }
-        uint32_t run[GORD], base = 0;
-#pragma unroll
-        for (int c = 0; c < (int)GORD; c++) {
-            const uint32_t incl = wave_incl_sum(tot[c]);
-            run[c] = base + incl - tot[c];
-            base += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-        }
+        const uint32_t incl = wave_incl_sum(tp), excl = incl - tp;
+        const uint32_t all = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        uint32_t rc = excl & 0xFFFFFu, rs = (all & 0xFFFFFu) + (excl >> 20);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            uint32_t st = 0;
-#pragma unroll
-            for (int c = 0; c < (int)GORD; c++) {
-                const bool me = k4[k] == (uint32_t)c;
-                st = me ? run[c] : st;
-                run[c] += me ? c4[k] : 0u;
-            }
+            const bool single = c4[k] == 1u;
+            const uint32_t st = single ? rs : rc;
+            rs += single ? 1u : 0u;
+            rc += single ? 0u : c4[k];
             cnt[lane * 4 + k] = st;
             GS[lane * 4 + k] = (uint16_t)st;
         }
