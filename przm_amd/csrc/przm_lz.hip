@@ -716,31 +716,54 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
         pd.a = NONE32;
     }
     /* Group starts.  A chunk of 64 elements takes as many search steps as its longest chain, so
-     * the groups with a chain (two or more positions) are laid out first and the single
-     * positions after them, each in digit order: chunks then hold chains of like length.  A
-     * chain only needs its group to be contiguous and in position order, so the layout changes
-     * no result.  One wave scan serves both classes: a lane's chain elements count in the low
-     * 20 bits of the scanned value (at most 65536), its single positions from bit 20 (at most
-     * 256). */
+     * the groups are laid out by class, each class in digit order: chunks then hold chains of
+     * like length.  A chain only needs its group to be contiguous and in position order, so
+     * the layout changes no result.  One wave scan serves all classes, each counted in its own
+     * bit field of the scanned value: for a bucket in LDS three 11-bit fields (chains longer
+     * than depth, shorter chains, single positions), otherwise two (chains in the low 20 bits,
+     * at most 65536 elements; single positions from bit 20, at most 256). */
     {
         uint16_t *GS = S.sm.gstart[w];
-        uint32_t c4[4], tp = 0;
+        uint32_t c4[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            c4[k] = cnt[lane * 4 + k];
-            tp += c4[k] == 1u ? (1u << 20) : c4[k];
-        }
-        const uint32_t incl = wave_incl_sum(tp), excl = incl - tp;
-        const uint32_t all = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-        uint32_t rc = excl & 0xFFFFFu, rs = (all & 0xFFFFFu) + (excl >> 20);
+        for (int k = 0; k < 4; k++) c4[k] = cnt[lane * 4 + k];
+        if (m <= WS_CAP) {
+            /* a bucket kept in LDS (at most 1536 elements) has room for three 11-bit fields:
+             * chains longer than depth first (their chunks all take every step), then the
+             * shorter chains, then the single positions */
+            uint32_t tp = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const bool single = c4[k] == 1u;
-            const uint32_t st = single ? rs : rc;
-            rs += single ? 1u : 0u;
-            rc += single ? 0u : c4[k];
-            cnt[lane * 4 + k] = st;
-            GS[lane * 4 + k] = (uint16_t)st;
+            for (int k = 0; k < 4; k++) tp += c4[k] == 1u ? (1u << 22) : c4[k] > depth ? c4[k] : (c4[k] << 11);
+            const uint32_t incl = wave_incl_sum(tp), excl = incl - tp;
+            const uint32_t all = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            const uint32_t nf = all & 0x7FFu, nsh = (all >> 11) & 0x7FFu;
+            uint32_t rf = excl & 0x7FFu, rh = nf + ((excl >> 11) & 0x7FFu), rs = nf + nsh + (excl >> 22);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool single = c4[k] == 1u, full = c4[k] > depth;
+                const uint32_t st = single ? rs : full ? rf : rh;
+                rs += single ? 1u : 0u;
+                rf += full ? c4[k] : 0u;
+                rh += (!single && !full) ? c4[k] : 0u;
+                cnt[lane * 4 + k] = st;
+                GS[lane * 4 + k] = (uint16_t)st;
+            }
+        } else {
+            uint32_t tp = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) tp += c4[k] == 1u ? (1u << 20) : c4[k];
+            const uint32_t incl = wave_incl_sum(tp), excl = incl - tp;
+            const uint32_t all = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            uint32_t rc = excl & 0xFFFFFu, rs = (all & 0xFFFFFu) + (excl >> 20);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool single = c4[k] == 1u;
+                const uint32_t st = single ? rs : rc;
+                rs += single ? 1u : 0u;
+                rc += single ? 0u : c4[k];
+                cnt[lane * 4 + k] = st;
+                GS[lane * 4 + k] = (uint16_t)st;
+            }
         }
     }
     lds_fence();
