@@ -722,6 +722,7 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
      * bit field of the scanned value: for a bucket in LDS three 11-bit fields (chains longer
      * than depth, shorter chains, single positions), otherwise two (chains in the low 20 bits,
      * at most 65536 elements; single positions from bit 20, at most 256). */
+    uint32_t msing; /* where the single positions start: they have no chain, hence no match */
     {
         uint16_t *GS = S.sm.gstart[w];
         uint32_t c4[4];
@@ -738,6 +739,7 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
             const uint32_t all = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
             const uint32_t nf = all & 0x7FFu, nsh = (all >> 11) & 0x7FFu;
             uint32_t rf = excl & 0x7FFu, rh = nf + ((excl >> 11) & 0x7FFu), rs = nf + nsh + (excl >> 22);
+            msing = nf + nsh;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const bool single = c4[k] == 1u, full = c4[k] > depth;
@@ -755,6 +757,7 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
             const uint32_t incl = wave_incl_sum(tp), excl = incl - tp;
             const uint32_t all = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
             uint32_t rc = excl & 0xFFFFFu, rs = (all & 0xFFFFFu) + (excl >> 20);
+            msing = all & 0xFFFFFu;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const bool single = c4[k] == 1u;
@@ -804,9 +807,11 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     }
     if (prof) pt[0] += t_c - t_a;
     const uint16_t *GS = S.sm.gstart[w];
-    if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, GS, 0, m, 64, pd, pt);
-    else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, GS, 0, m, 64, pd, pt);
-    else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, GS, 0, m, 64, pd, pt);
+    /* the groups with a chain are searched; the single positions come last and get "no match" */
+    if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
+    else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
+    else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
+    for (uint32_t i = msing + lane; i < m; i += 64) sc.D[SRT[i]] = 0u;
     return false;
 }
 
