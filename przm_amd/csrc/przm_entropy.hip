@@ -61,10 +61,25 @@ przm_check_kernel(const uint8_t *__restrict__ src, przm_block_t *__restrict__ bl
 
 /* ---------------- histogram + normalisation ---------------- */
 
+/* inclusive wave scan in DPP moves (no LDS round trips): row_shr 1, 2, 4, 8 inside each row of
+ * 16 lanes, then row 0's last lane into row 1 and row 2's into row 3 (row_bcast:15), then
+ * lane 31 into rows 2 and 3 (row_bcast:31); lanes whose source is out of range add 0 */
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t ent_dpp(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWMASK, 0xF, false);
+}
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) {
+    x += ent_dpp<0x111, 0xF>(x);
+    x += ent_dpp<0x112, 0xF>(x);
+    x += ent_dpp<0x114, 0xF>(x);
+    x += ent_dpp<0x118, 0xF>(x);
+    x += ent_dpp<0x142, 0xA>(x);
+    x += ent_dpp<0x143, 0xC>(x);
+    return x;
+}
+/* the sum over the wave, in every lane */
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_incl(v), 63);
 }
 
 /* stream of job kind k (0 lit, 1 op, 2 meta, 3 var) */
@@ -124,12 +139,7 @@ __device__ void normalize_wave(const uint32_t *cnt, uint32_t *key, uint32_t *frq
     __builtin_amdgcn_wave_barrier();
     uint32_t loc[4], acc = 0;
     for (int i = 0; i < 4; i++) { loc[i] = frq[lane * 4 + i]; acc += loc[i]; }
-    uint32_t x = acc;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    uint32_t cum = x - acc;
+    uint32_t cum = wave_scan_incl(acc) - acc;
     for (int i = 0; i < 4; i++) {
         ft->fc[lane * 4 + i] = loc[i] | (cum << 16);
         cum += loc[i];
