@@ -419,7 +419,7 @@ struct Pend {
 
 struct Scr {
     const uint16_t *T0, *T1;
-    uint32_t *D;      /* per position: match distance | min(length, 255) << 16 */
+    uint32_t *D;      /* per position: match distance | min(length, 255) << 16, 0 for no match */
     uint8_t *gws; /* per-wave workspaces, GWS_STRIDE bytes each */
 };
 
@@ -611,10 +611,10 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
         lds_fence();
         const uint64_t t_c = prof ? __builtin_amdgcn_s_memrealtime() : 0;
         if (pd.a != NONE32) sc.D[pd.a] = pd.v; /* the previous chunk's result */
-        { /* by position: dist | min(len, 255) << 16, 0 for no match */
+        { /* by position: dist | min(len, 255) << 16; "no match" (0) is already in place */
             const uint32_t L = best >> 16;
-            pd.v = L >= 4u ? (p - (best & 0xFFFFu)) | (umin(L, 255u) << 16) : 0u;
-            pd.a = valid ? p : NONE32;
+            pd.v = (p - (best & 0xFFFFu)) | (umin(L, 255u) << 16);
+            pd.a = valid && L >= 4u ? p : NONE32;
         }
         if (prof) {
             const uint64_t t_d = __builtin_amdgcn_s_memrealtime();
@@ -807,11 +807,10 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     }
     if (prof) pt[0] += t_c - t_a;
     const uint16_t *GS = S.sm.gstart[w];
-    /* the groups with a chain are searched; the single positions come last and get "no match" */
+    /* the groups with a chain are searched; the single positions come last and keep "no match" */
     if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
     else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
     else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
-    for (uint32_t i = msing + lane; i < m; i += 64) sc.D[SRT[i]] = 0u;
     return false;
 }
 
@@ -1493,6 +1492,10 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             const uint32_t end = nh < base + PRZM_HALF ? (nh > base ? nh : base) : base + PRZM_HALF;
             for (uint32_t i = tid; i < LZ_NW * 256; i += LZ_NT) (&S.sm.tab[0][0])[i] = 0;
             __syncthreads();
+            { /* the half's match results start as "no match" (coalesced): the search writes matches only */
+                uint4 *d4 = (uint4 *)(sc.D + base);
+                for (uint32_t i = tid; i < PRZM_HALF / 4u; i += LZ_NT) d4[i] = make_uint4(0u, 0u, 0u, 0u);
+            }
             const uint32_t wb = base + w * (PRZM_HALF / LZ_NW);
             /* eight digits at a time, their loads unconditional (positions below PRZM_BLOCK, whose
              * words lie inside in[]) so that they are in flight together */
