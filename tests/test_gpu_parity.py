@@ -64,6 +64,21 @@ def test_match_descriptors_match_oracle(g):
             assert bad.size == 0, f"{name} block {b}: {bad.size} positions differ, first {bad[:5]}"
 
 
+def test_results_do_not_carry_between_blocks(g):
+    """Blocks with many matches alternate with blocks with none, more blocks than workgroups, so
+    a workgroup's per-block scratch is reused: each block's results are its own."""
+    O = oracle()
+    rng = np.random.default_rng(23)
+    nb = 640
+    a = np.concatenate([D.markov(65536, 100 + b) if b % 2 == 0 else rng.integers(0, 256, 65536, dtype=np.uint8)
+                        for b in range(nb)])
+    desc, _ = g.MatchDescriptors(a)
+    for b in range(nb):
+        want = O.match_block(a[b * 65536:(b + 1) * 65536])
+        bad = np.nonzero(desc[b * 65536:(b + 1) * 65536] != want)[0]
+        assert bad.size == 0, f"block {b}: {bad.size} positions differ, first {bad[:5]}"
+
+
 def test_token_streams_match_oracle(g):
     O = oracle()
     for name, a in edge_inputs():
