@@ -455,6 +455,23 @@ uint32_t orc_package_commands(const orc_streams *s, uint8_t *out, int choices[3]
     return len;
 }
 
+/* Section 2 of a whole command stream (commands.go:537-568): split, then Mode 8
+ * or flat.  out holds 16 + 2n + 3*640 bytes; choices as orc_package_commands.
+ * Returns the section length, 0 for an empty stream, or -1 when the stream
+ * does not split (an opcode mode G never emits). */
+int64_t orc_package_cmd_stream(const uint8_t *cmd, uint32_t n, uint8_t *out, int choices[3]) {
+    orc_streams s;
+    memset(&s, 0, sizeof s);
+    uint8_t *mem = (uint8_t *)malloc(3 * (size_t)n + 16);
+    s.cmd = (uint8_t *)cmd;
+    s.ncmd = n;
+    s.op = mem; s.meta = mem + n + 4; s.var = mem + 2 * (size_t)n + 8;
+    if (orc_split_commands(cmd, n, s.op, &s.nop, s.meta, &s.nmeta, s.var, &s.nvar)) { free(mem); return -1; }
+    int64_t len = orc_package_commands(&s, out, choices);
+    free(mem);
+    return len;
+}
+
 /* compress.go:113-132 */
 int orc_check_high_entropy(const uint8_t *src, uint32_t n) {
     uint32_t sample = n < 512 ? n : 512;

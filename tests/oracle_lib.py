@@ -31,8 +31,8 @@ def u8(x):
 
 
 class Oracle:
-    def __init__(self):
-        l = ctypes.CDLL(ORACLE)
+    def __init__(self, path=ORACLE):
+        l = ctypes.CDLL(path)
         l.orc_hash4.restype = U32
         l.orc_hash4.argtypes = [P]
         l.orc_search_depth.restype = ctypes.c_int
@@ -65,6 +65,8 @@ class Oracle:
         l.orc_encode_block.restype = U32
         l.orc_encode_buffer.argtypes = [P, U64, ctypes.c_int, P, U64, P, U64, ctypes.POINTER(U64)]
         l.orc_encode_buffer.restype = U64
+        l.orc_package_cmd_stream.argtypes = [P, U32, P, P]
+        l.orc_package_cmd_stream.restype = ctypes.c_int64
         self.l = l
 
     def hash4(self, b4):
@@ -139,6 +141,16 @@ class Oracle:
         ch = ctypes.c_int()
         n = self.l.orc_pack_substream(ptr(a), a.size, ptr(out), ctypes.byref(ch))
         return out[:n].tobytes(), ch.value
+
+    def package_cmds(self, cmd):
+        """Section 2 of a command stream (split, then Mode 8 or flat) and the substream choices."""
+        c = u8(cmd)
+        out = np.zeros(16 + 2 * c.size + 3 * 640, np.uint8)
+        ch = (ctypes.c_int * 3)(-2, -2, -2)
+        n = self.l.orc_package_cmd_stream(ptr(c), c.size, ptr(out), ch)
+        if n < 0:
+            raise ValueError("command stream does not split")
+        return out[:n].tobytes(), list(ch)
 
     def encode_block(self, src, level=6):
         a = u8(src)

@@ -352,16 +352,3 @@ def test_pack_substream_choice_order():
     a = np.zeros(4000, np.uint8)
     p, ch = O.pack_substream(a)
     assert ch in (1, 2) and len(p) < 4001
-
-
-def test_hazard_raw_section_becomes_uncompressed():
-    """A raw literal section that starts 'V','A',0x01 would be misread by the
-    decoders (literals.go:703-708); mode G emits an uncompressed frame."""
-    O = oracle()
-    rng = np.random.default_rng(12)
-    lit = np.concatenate([np.frombuffer(b"VA\x01", np.uint8), rng.integers(0, 256, 600, dtype=np.uint8)])
-    tail = np.frombuffer(b"q" * 5000, np.uint8)
-    a = np.concatenate([lit, tail])
-    f, info = O.encode_block(a)
-    if info.s4v == 0 and info.nlit >= 8:
-        assert info.kind in (2, 3) and f[4] == 0x04
