@@ -192,19 +192,26 @@ typedef struct {
     double submit_ms, start_ms, done_ms; /* clock of the server, ms since creation */
 } przm_result;
 
-/* max_batch_bytes bounds one launch; hbm_budget bounds the device bytes of
- * all admitted requests (inputs + frames + workspace). */
+/* max_batch_bytes bounds one launch and one request.  Each launch slot holds
+ * the device memory of a full launch (inputs, frames, block workspace, match
+ * scratch), reserved at creation; hbm_budget bounds the slots' total, so as
+ * many slots as fit (at most `streams`) are created and a request waits for a
+ * free slot.  PRZM_GPU_ERR_CAPACITY when not even one slot fits. */
 int32_t przm_serve_create(przm_gpu *g, int32_t level, uint64_t hbm_budget, uint64_t max_batch_bytes,
                           uint32_t streams, przm_server **out);
-/* Copies src; returns a request id.  PRZM_GPU_ERR_BUSY when the host queue is full. */
+/* Copies src; returns a request id.  PRZM_GPU_ERR_BUSY when the host queue is
+ * full, PRZM_GPU_ERR_CAPACITY when n > max_batch_bytes, and the error code of a
+ * failed launch once a device error has stopped the server (that launch's and
+ * every queued request's wait returns the same code). */
 int32_t przm_serve_submit(przm_server *s, const uint8_t *src, uint64_t n, uint64_t *id);
 /* Waits up to timeout_ms (-1 = forever) for request id; copies frames into
  * out (cap bytes) and sizes into frame_sizes when given. */
 int32_t przm_serve_wait(przm_server *s, uint64_t id, int64_t timeout_ms, uint8_t *out, uint64_t cap,
                         uint32_t *frame_sizes, uint64_t frames_cap, przm_result *res);
 /* counters: [0] launches, [1] requests done, [2] bytes in, [3] bytes out,
- * [4] peak admitted device bytes, [5] max requests in one launch,
- * [6] launches that overlapped another in flight */
+ * [4] peak device bytes used by the launches in flight, [5] max requests in
+ * one launch, [6] launches that overlapped another in flight, [7] device
+ * bytes reserved by the launch slots (<= hbm_budget) */
 int32_t przm_serve_stats(przm_server *s, uint64_t stats[8]);
 int32_t przm_serve_destroy(przm_server *s);
 

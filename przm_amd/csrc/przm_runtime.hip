@@ -1000,6 +1000,7 @@ struct Slot {
     uint8_t *h_in = nullptr, *h_out = nullptr;
     uint64_t h_in_cap = 0, h_out_cap = 0;
     uint64_t dev_bytes = 0;
+    bool dead = false; /* its stream reported an error: never launched on again */
     hipEvent_t done = nullptr;
     std::vector<uint32_t> flen;
 };
@@ -1017,7 +1018,9 @@ struct przm_server {
     std::deque<std::shared_ptr<Req>> pending;
     std::map<uint64_t, std::shared_ptr<Req>> all;
     uint64_t next_id = 1;
-    uint64_t admitted = 0;
+    uint64_t admitted = 0;  /* bytes the launches in flight use (inside their slots) */
+    uint64_t reserved = 0;  /* device bytes the slots hold: at most the budget */
+    int32_t failed = 0;     /* a device error stopped the server: every later call fails with it */
     bool stop = false;
     std::thread worker;
     double t0 = 0;
@@ -1040,8 +1043,18 @@ int32_t host_grow(uint8_t **p, uint64_t *cap, uint64_t need) {
     return 0;
 }
 
-/* device and pinned buffers of a slot for `nb` blocks over `arena` input bytes; growing only past
- * the size fixed at creation (a single request larger than max_batch_bytes) */
+/* device bytes a context holds */
+uint64_t ctx_bytes(const Ctx &c) {
+    uint64_t t = 0;
+    for (const DevBuf *b : {&c.in, &c.out, &c.blocks, &c.ws, &c.meta, &c.ftab, &c.flen, &c.foff, &c.run, &c.scratch,
+                            &c.dbg, &c.perm})
+        t += b->cap;
+    return t;
+}
+
+/* device and pinned buffers of a slot for `nb` blocks over `arena` input bytes (sized for a full
+ * launch when the server is created; a launch never needs more, since a request is at most
+ * max_batch_bytes) */
 int32_t slot_reserve(przm_server *s, Slot &sl, uint64_t arena, uint32_t nb) {
     przm_gpu *g = s->g;
     const uint64_t bound = arena + 8ull * nb + 64;
@@ -1171,11 +1184,22 @@ void serve_loop(przm_server *s) {
             std::unique_lock<std::mutex> lk(s->mu);
             for (auto &up : s->slots) {
                 Slot &sl = *up;
-                if (!sl.busy) continue;
+                if (!sl.busy || sl.dead) continue;
                 const hipError_t q = hipEventQuery(sl.done);
                 if (q == hipSuccess) done_slots.push_back(&sl);
-                else if (q != hipErrorNotReady) retire_slot(s, sl, PRZM_GPU_ERR_HIP);
-                else any_busy = true;
+                else if (q != hipErrorNotReady) {
+                    /* the slot's stream may still hold copies into its pinned buffers: the slot is
+                     * never used again, and the server fails this launch, every queued request
+                     * and every later submit with the error */
+                    snprintf(g_err, sizeof g_err, "serving launch failed: %s", hipGetErrorString(q));
+                    retire_slot(s, sl, PRZM_GPU_ERR_HIP);
+                    sl.dead = true;
+                    sl.busy = true;
+                    s->failed = PRZM_GPU_ERR_HIP;
+                    for (auto &r : s->pending) r->status = PRZM_GPU_ERR_HIP;
+                    s->pending.clear();
+                    s->cv_done.notify_all();
+                } else any_busy = true;
             }
             if (s->stop && s->pending.empty() && !any_busy && done_slots.empty()) break;
             Slot *free_slot = nullptr;
@@ -1256,27 +1280,48 @@ int32_t przm_serve_create(przm_gpu *g, int32_t level, uint64_t hbm_budget, uint6
     s->max_batch = max_batch_bytes ? max_batch_bytes : (256ull << 20);
     s->t0 = now_ms();
     /* a launch carries at most max_batch_bytes in up to max_blocks blocks (small requests take a
-     * block each); every slot is sized for that now, so the loop never allocates */
+     * block each); every slot is sized for that now, so the loop never allocates.  The slots'
+     * device memory counts against the HBM budget: as many slots as fit (at most `streams`) are
+     * created, and a request waits for a free slot, i.e. for its memory */
     s->max_blocks = (uint32_t)std::min<uint64_t>(s->max_batch / PRZM_BLOCK + 1024, 65536);
     const uint64_t arena = s->max_batch + 256ull * s->max_blocks;
+    auto cleanup = [&]() {
+        for (auto &x : s->slots) {
+            if (x->h_in) (void)hipHostFree(x->h_in);
+            if (x->h_out) (void)hipHostFree(x->h_out);
+            x->c.release();
+        }
+        delete s;
+    };
     for (uint32_t i = 0; i < streams; i++) {
         auto sl = std::make_unique<Slot>();
         if (hipSetDevice(g->dev) != hipSuccess || hipStreamCreateWithFlags(&sl->c.st, hipStreamNonBlocking) != hipSuccess) {
-            delete s;
+            cleanup();
             return PRZM_GPU_ERR_HIP;
         }
         const int32_t rc = slot_reserve(s, *sl, arena, s->max_blocks);
+        const uint64_t bytes = ctx_bytes(sl->c);
+        if (rc == 0 && s->reserved + bytes > s->budget) {
+            /* this slot would exceed the budget: keep the ones that fit */
+            if (sl->h_in) (void)hipHostFree(sl->h_in);
+            if (sl->h_out) (void)hipHostFree(sl->h_out);
+            sl->c.release();
+            if (i == 0) {
+                snprintf(g_err, sizeof g_err, "HBM budget %llu below one launch slot (%llu bytes for max_batch_bytes %llu)",
+                         (unsigned long long)s->budget, (unsigned long long)bytes, (unsigned long long)s->max_batch);
+                cleanup();
+                return PRZM_GPU_ERR_CAPACITY;
+            }
+            break;
+        }
         s->slots.push_back(std::move(sl));
         if (rc) {
-            for (auto &x : s->slots) {
-                if (x->h_in) (void)hipHostFree(x->h_in);
-                if (x->h_out) (void)hipHostFree(x->h_out);
-                x->c.release();
-            }
-            delete s;
+            cleanup();
             return rc;
         }
+        s->reserved += bytes;
     }
+    s->stats[7] = s->reserved;
     s->worker = std::thread(serve_loop, s);
     *out = s;
     return 0;
@@ -1284,9 +1329,15 @@ int32_t przm_serve_create(przm_gpu *g, int32_t level, uint64_t hbm_budget, uint6
 
 int32_t przm_serve_submit(przm_server *s, const uint8_t *src, uint64_t n, uint64_t *id) {
     if (!s || !id || (!src && n)) return PRZM_GPU_ERR_ARG;
+    if (n > s->max_batch) {
+        snprintf(g_err, sizeof g_err, "request of %llu bytes above max_batch_bytes %llu", (unsigned long long)n,
+                 (unsigned long long)s->max_batch);
+        return PRZM_GPU_ERR_CAPACITY;
+    }
     auto r = std::make_shared<Req>();
     r->src.assign(src, src + n);
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->failed) return s->failed;
     if (s->pending.size() > 1000000) return PRZM_GPU_ERR_BUSY;
     r->id = s->next_id++;
     r->submit_ms = now_ms() - s->t0;

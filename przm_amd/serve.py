@@ -59,8 +59,9 @@ class Server:
     def stats(self):
         s = (ctypes.c_uint64 * 8)()
         self._lib.przm_serve_stats(self._h, s)
-        keys = ["launches", "done", "bytes_in", "bytes_out", "peak_admitted", "max_batch_reqs", "overlapped"]
-        return dict(zip(keys, list(s)[:7]))
+        keys = ["launches", "done", "bytes_in", "bytes_out", "peak_admitted", "max_batch_reqs", "overlapped",
+                "reserved"]
+        return dict(zip(keys, list(s)))
 
     def close(self):
         if self._h:

@@ -253,3 +253,13 @@ def refdec():
 
 def have_refdec():
     return os.path.exists(REFDEC)
+
+
+def encode_many(bufs, level=6, threads=None):
+    """The oracle's frames for many buffers, on a thread pool (the C calls release the GIL):
+    list of (frames, sizes)."""
+    from concurrent.futures import ThreadPoolExecutor
+    O = oracle()
+    threads = threads or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(lambda b: O.encode_buffer(b, level), bufs))
