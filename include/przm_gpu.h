@@ -128,6 +128,18 @@ int32_t przm_gpu_encode(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t lev
 int32_t przm_gpu_encode_device(przm_gpu *g, const void *d_src, uint64_t n, int32_t level, void *d_out,
                                uint64_t out_cap, uint64_t *out_len, uint32_t *frame_sizes, uint64_t frames_cap,
                                uint64_t *nframes, przm_gpu_profile *prof);
+/* Same as przm_gpu_encode_device, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc); it has no counterpart in the reference.  Allocates
+ * nothing: every intermediate buffer lives in one static device array of the
+ * library (about 48 MB per GPU, held until the process ends), so n is at most
+ * 1 MiB (16 frames; PRZM_GPU_ERR_CAPACITY above), out_cap must hold
+ * przm_gpu_frames_bound(n) and frame_sizes is required.  All calls on one GPU
+ * share that array: calls that overlap on one GPU (from two handles or two
+ * threads) race. */
+int32_t przm_gpu_encode_device_noalloc(przm_gpu *g, const void *d_src, uint64_t n, int32_t level, void *d_out,
+                                       uint64_t out_cap, uint64_t *out_len, uint32_t *frame_sizes,
+                                       uint64_t frames_cap, uint64_t *nframes);
 /* Batched form: many buffers of a device arena through one hipGraph launch of
  * the persistent pipeline.  out_off[i] = byte offset of buffer i's first
  * frame in d_out (nbufs + 1 entries, the last = total); frame_sizes gets

@@ -68,6 +68,7 @@ SIGNATURES = {
     "przm_gpu_pack_flat": (I32, [P, P, U64, P, U64, U64P, ctypes.POINTER(GpuProfile)]),
     "przm_gpu_encode": (I32, [P, P, U64, I32, P, U64, U64P, P, U64, U64P, ctypes.POINTER(GpuProfile)]),
     "przm_gpu_encode_device": (I32, [P, P, U64, I32, P, U64, U64P, P, U64, U64P, ctypes.POINTER(GpuProfile)]),
+    "przm_gpu_encode_device_noalloc": (I32, [P, P, U64, I32, P, U64, U64P, P, U64, U64P]),
     "przm_gpu_encode_batch_device": (I32, [P, P, ctypes.POINTER(Span), U32, I32, P, U64, P, P, U64, U64P,
                                            ctypes.POINTER(GpuProfile)]),
     "przm_gpu_encode_batch": (I32, [P, P, ctypes.POINTER(Span), U32, I32, P, U64, P, P, U64, U64P,

@@ -135,6 +135,47 @@ __global__ void przm_desc_kernel(const uint8_t *__restrict__ src, const przm_blo
     }
 }
 
+/*
+ * Scratch of the entry point that allocates nothing (przm_gpu_encode_device_noalloc): every
+ * device buffer the stage kernels need for at most NA_BLOCKS blocks on at most NA_GRID match
+ * workgroups, carved out of one static device array.  Host code never passes the array's name to
+ * a kernel or a copy: it uses the device address hipGetSymbolAddress returns for the current
+ * device (na_scratch_ptr), looked up once per device when a handle is created.
+ */
+constexpr uint32_t NA_BLOCKS = 16; /* 1 MiB of input */
+constexpr uint32_t NA_GRID = 8;
+constexpr uint64_t na_al(uint64_t x) { return (x + 255) & ~255ull; }
+constexpr uint64_t NA_OFF_BLOCKS = 0;
+constexpr uint64_t NA_OFF_META = NA_OFF_BLOCKS + na_al(NA_BLOCKS * sizeof(przm_block_t));
+constexpr uint64_t NA_OFF_FTAB = NA_OFF_META + na_al(NA_BLOCKS * sizeof(przm_meta_t));
+constexpr uint64_t NA_OFF_PERM = NA_OFF_FTAB + na_al(NA_BLOCKS * 4ull * sizeof(przm_ftab_t));
+constexpr uint64_t NA_OFF_FLEN = NA_OFF_PERM + na_al(NA_BLOCKS * PRZM_J_N * 4ull);
+constexpr uint64_t NA_OFF_FOFF = NA_OFF_FLEN + na_al(NA_BLOCKS * 4ull);
+constexpr uint64_t NA_OFF_RUN = NA_OFF_FOFF + na_al(NA_BLOCKS * 8ull);
+constexpr uint64_t NA_OFF_WS = NA_OFF_RUN + 256;
+constexpr uint64_t NA_OFF_LZ = NA_OFF_WS + na_al((uint64_t)NA_BLOCKS * PRZM_WS_BYTES);
+constexpr uint64_t NA_BYTES = NA_OFF_LZ + (uint64_t)NA_GRID * LZ_SCRATCH_STRIDE;
+
+__device__ __align__(256) uint8_t scratch[NA_BYTES];
+
+std::mutex g_na_mu;
+void *g_na_ptr[64]; /* device address of `scratch` per device (null until looked up) */
+
+/* looks the array up on device `dev` (which is current) once; 0 or an error code */
+int32_t na_scratch_lookup(int dev) {
+    if (dev < 0 || dev >= 64) return PRZM_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (g_na_ptr[dev]) return 0;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        snprintf(g_err, sizeof g_err, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        return PRZM_GPU_ERR_HIP;
+    }
+    g_na_ptr[dev] = ptr;
+    return 0;
+}
+
 int depth_of_level(int level) { /* compress.go:34-51, default level 6 (compress.go:1038-1040) */
     if (level <= 0) level = 6;
     if (level <= 1) return 2;
@@ -418,6 +459,7 @@ int32_t init_handle(przm_gpu *g, int device) {
     snprintf(g->info.device_name, sizeof g->info.device_name, "%s", p.name);
     snprintf(g->info.arch, sizeof g->info.arch, "%s", p.gcnArchName);
     HIPCHK(hipStreamCreateWithFlags(&g->c.st, hipStreamNonBlocking));
+    CHK(na_scratch_lookup(device));
     return 0;
 }
 
@@ -609,6 +651,53 @@ int32_t przm_gpu_encode_device(przm_gpu *g, const void *d_src, uint64_t n, int32
     fill_outputs(&sp, 1, first, bo, nullptr, frame_sizes, nframes);
     if (out_len) *out_len = bo.total;
     if (prof) prof->total_round_trip_ms = now_ms() - t0;
+    return 0;
+}
+
+int32_t przm_gpu_encode_device_noalloc(przm_gpu *g, const void *d_src, uint64_t n, int32_t level, void *d_out,
+                                       uint64_t out_cap, uint64_t *out_len, uint32_t *frame_sizes, uint64_t frames_cap,
+                                       uint64_t *nframes) {
+    if (!g || (!d_src && n) || !d_out || !frame_sizes) return PRZM_GPU_ERR_ARG;
+    CHK(mock_check());
+    if (n > (uint64_t)NA_BLOCKS * PRZM_BLOCK) {
+        snprintf(g_err, sizeof g_err, "przm_gpu_encode_device_noalloc: %llu bytes above its maximum of %llu",
+                 (unsigned long long)n, (unsigned long long)NA_BLOCKS * PRZM_BLOCK);
+        return PRZM_GPU_ERR_CAPACITY;
+    }
+    const uint32_t nb = n ? (uint32_t)((n + PRZM_BLOCK - 1) / PRZM_BLOCK) : 1u;
+    if (nb > frames_cap || przm_gpu_frames_bound(n) > out_cap) return PRZM_GPU_ERR_CAPACITY;
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPCHK(hipSetDevice(g->dev));
+    uint8_t *base = (uint8_t *)g_na_ptr[g->dev];
+    if (!base) return PRZM_GPU_ERR_UNAVAILABLE;
+    /* the block table on the stack, uploaded into the array (no device allocation) */
+    przm_block_t blocks[NA_BLOCKS];
+    if (n == 0) blocks[0] = {0, 0, PRZM_BF_RAW};
+    for (uint32_t b = 0; b < nb && n; b++)
+        blocks[b] = {(uint64_t)b * PRZM_BLOCK, (uint32_t)std::min<uint64_t>(PRZM_BLOCK, n - (uint64_t)b * PRZM_BLOCK), 0};
+    /* a context whose buffers are views into the array: never ensure()d or release()d */
+    Ctx v;
+    v.st = g->c.st;
+    v.blocks.p = base + NA_OFF_BLOCKS;
+    v.meta.p = base + NA_OFF_META;
+    v.ftab.p = base + NA_OFF_FTAB;
+    v.perm.p = base + NA_OFF_PERM;
+    v.flen.p = base + NA_OFF_FLEN;
+    v.foff.p = base + NA_OFF_FOFF;
+    v.run.p = base + NA_OFF_RUN;
+    v.ws.p = base + NA_OFF_WS;
+    v.scratch.p = base + NA_OFF_LZ;
+    HIPCHK(hipMemcpyAsync(v.blocks.p, blocks, nb * sizeof(przm_block_t), hipMemcpyHostToDevice, v.st));
+    const int grid = std::min<int>(std::min<int>(g->lz_grid, (int)NA_GRID), (int)nb);
+    /* every counter the kernels start from zero is cleared by a fill kernel inside enqueue */
+    CHK(enqueue(g, v, (const uint8_t *)d_src, nb, (uint32_t)depth_of_level(level), (uint8_t *)d_out, true, false, false,
+                grid, nb, nullptr));
+    HIPCHK(hipMemcpyAsync(frame_sizes, v.flen.p, nb * 4ull, hipMemcpyDeviceToHost, v.st));
+    HIPCHK(hipStreamSynchronize(v.st));
+    uint64_t total = 0;
+    for (uint32_t b = 0; b < nb; b++) total += frame_sizes[b];
+    if (out_len) *out_len = total;
+    if (nframes) *nframes = nb;
     return 0;
 }
 
