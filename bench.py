@@ -49,28 +49,51 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(arena, offs, sizes, seconds, threads=None):
-    """The C oracle (oracle/liboracle.so) on a time-bounded sample, all host cores (or `threads`)."""
+def host_cpu():
+    """Core counts and model of this host: nproc, the cores this process may run on, the model
+    name from /proc/cpuinfo."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count() or 1, "usable_cores": usable, "cpu_model": model}
+
+
+def cpu_threads():
+    """Every core this process may use, capped by the host's share for one GPU when the
+    environment states it (OMP_NUM_THREADS)."""
+    usable = host_cpu()["usable_cores"]
+    share = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(usable, int(share))) if share and share.isdigit() else usable
+
+
+def cpu_baseline(bufs, seconds, threads=None, level=6):
+    """The C oracle of the same mode (oracle/liboracle.so: the CPU fallback restated) on the
+    buffers `bufs` in order, until `seconds` have passed or all are done; independent buffers
+    spread over `threads` threads.  Returns (input GB/s, buffers done, bytes done, wall s)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from concurrent.futures import ThreadPoolExecutor
     from oracle_lib import oracle
     O = oracle()
-    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    threads = threads or cpu_threads()
     done_bytes = 0
     t0 = time.perf_counter()
     i = 0
     with ThreadPoolExecutor(threads) as ex:
-        while time.perf_counter() - t0 < seconds and i < len(sizes):
-            batch = list(range(i, min(i + threads * 2, len(sizes))))
+        while time.perf_counter() - t0 < seconds and i < len(bufs):
+            batch = list(range(i, min(i + threads * 2, len(bufs))))
             i += len(batch)
-            futs = [ex.submit(O.encode_buffer, arena[int(offs[j]): int(offs[j]) + int(sizes[j])]) for j in batch]
+            futs = [ex.submit(O.encode_buffer, bufs[j], level) for j in batch]
             for f, j in zip(futs, batch):
                 f.result()
-                done_bytes += int(sizes[j])
+                done_bytes += int(bufs[j].size)
     wall = time.perf_counter() - t0
-    return {"value": round(done_bytes / wall / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"first {i} buffers of the batched workload ({done_bytes} bytes), oracle/liboracle.so, "
-                      f"{threads} threads, {wall:.1f} s"}
+    return done_bytes / wall / 1e9, i, done_bytes, wall
 
 
 def pmc_traffic():
@@ -117,7 +140,7 @@ def main():
     ap.add_argument("--buffers", type=int, default=10000)
     ap.add_argument("--level", type=int, default=6)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-batch-mib", type=int, default=64, help="serving: largest launch (MiB)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
@@ -318,9 +341,27 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(arena, offs, sizes, args.cpu_seconds)
-        cpu1 = cpu_baseline(arena, offs, sizes, max(args.cpu_seconds / 3, 2.0), threads=1)
-        cpu["one_thread"] = {"value": cpu1["value"], "sample": cpu1["sample"]}
+        hc = host_cpu()
+        nt = cpu_threads()
+        bufs = [arena[int(o): int(o) + int(s)] for o, s in zip(offs, sizes)]
+        v, nb_done, by_done, w = cpu_baseline(bufs, args.cpu_seconds, nt, args.level)
+        v1, nb1, by1, w1 = cpu_baseline(bufs, max(args.cpu_seconds / 3, 2.0), 1, args.level)
+        # config 0: one 16 MiB buffer of Markov text as 256 independent 64 KiB frames
+        m0 = D.markov(16 << 20, args.seed)
+        frames0 = [m0[k: k + 65536] for k in range(0, m0.size, 65536)]
+        c0, _, _, c0w = cpu_baseline(frames0, 1e9, nt, args.level)
+        c01, _, _, c01w = cpu_baseline(frames0, 1e9, 1, args.level)
+        cpu = {"value": round(v, 4), "unit": "GB/s", "cores": nt, "kind": "port",
+               "label": "CPU fallback, same mode",
+               "sample": f"the first {nb_done} buffers of the batched workload ({by_done} bytes) through "
+                         f"oracle/liboracle.so (the C restatement of the reference CPU path in this mode), "
+                         f"{nt} threads, {w:.1f} s",
+               "nproc": hc["nproc"], "usable_cores": hc["usable_cores"], "cpu_model": hc["cpu_model"],
+               "one_thread": {"value": round(v1, 4), "sample": f"first {nb1} buffers ({by1} bytes), {w1:.1f} s"},
+               "markov_16MiB": {"value": round(c0, 4), "one_thread": round(c01, 4), "unit": "GB/s",
+                                "sample": f"16 MiB order-1 Markov text (seed {args.seed}) as 256 frames of "
+                                          f"64 KiB, level {args.level}: {nt} threads {c0w:.2f} s, 1 thread "
+                                          f"{c01w:.2f} s"}}
         log(f"cpu baseline: {cpu}")
 
     # measured copy bandwidth (the STREAM-copy figure beside the 8 TB/s peak)

@@ -35,6 +35,7 @@ extern "C" {
 
 typedef struct przm_gpu przm_gpu;
 typedef struct przm_server przm_server;
+typedef struct przm_pinned_pool przm_pinned_pool;
 
 /* GPUCompressor device fields (gpu/bridge.go:283-287).  On AMD:
  * compute_major/minor = gfx major / minor (gfx950 -> 9, 5), sm_count = CUs. */
@@ -78,6 +79,28 @@ typedef struct {
     uint64_t len;
 } przm_span;
 
+/* DeviceTokenStreams (gpu/types.go:90-100): token streams resident in device
+ * memory.  The command, offset and literal streams of every 64 KiB block,
+ * back to back in block order (d_block_lens: u32[num_chunks][3], each
+ * block's cmd / off / lit bytes; d_totals: u32[3], the three totals).  The
+ * pointers are borrowed from the handle and stay valid until its next call
+ * that launches work (every entry point except przm_gpu_download and
+ * przm_gpu_encode_streams); `generation` identifies that call. */
+typedef struct {
+    uint64_t d_cmds;
+    uint64_t d_offs;
+    uint64_t d_lits;
+    uint64_t d_totals;
+    uint32_t cmd_len;
+    uint32_t off_len;
+    uint32_t lit_len;
+    uint32_t num_chunks;   /* blocks (frames) of the input */
+    uint64_t stream;       /* hipStream_t the kernels ran on */
+    uint64_t d_block_lens;
+    uint64_t uncompressed_size;
+    uint64_t generation;
+} przm_device_token_streams;
+
 /* ---- availability and lifecycle (gpu/bridge.go:45-123, 291-449, 513) ---- */
 
 /* IsAvailable (bridge.go:61): 0 and *info filled when a gfx9 device is usable. */
@@ -101,6 +124,12 @@ void przm_gpu_set_mock_error(int32_t code);
  * (len << 16) | dist (block-relative), 0 where there is none.  n entries. */
 int32_t przm_gpu_match(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t level, uint32_t *desc,
                        przm_gpu_profile *prof);
+/* MatchLZ77Streaming (bridge.go:1461, 1216-1458): the same descriptors,
+ * computed chunk by chunk on two streams so that each chunk's upload and the
+ * host's staging of the next overlap the device work on the previous one.
+ * chunk_bytes 0 picks max(n / 2, 16 MiB), at most 64 MiB; rounded to 64 KiB. */
+int32_t przm_gpu_match_streaming(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t level, uint64_t chunk_bytes,
+                                 uint32_t *desc, przm_gpu_profile *prof);
 /* Token streams of every block, concatenated in block order; block_lens
  * gets (cmd, off, lit, op, meta, var, tokens) per block. */
 int32_t przm_gpu_tokenize(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t level, uint8_t *cmd, uint64_t cmd_cap,
@@ -114,9 +143,64 @@ int32_t przm_gpu_histogram(przm_gpu *g, const uint8_t *src, uint64_t n, uint32_t
  * 4 states, stream), core/literals/literals.go:368-384.  n <= 16 MiB. */
 int32_t przm_gpu_vans_encode(przm_gpu *g, const uint8_t *src, uint64_t n, uint8_t *out, uint64_t cap,
                              uint64_t *out_len, przm_gpu_profile *prof);
-/* ModeFlat section [0x00][u32 n][payload] (przm_ans.cu:202-234). */
+/* ModeFlat section [0x00][u32 n][payload] (przm_ans.cu:202-234).  An empty
+ * input gives five zero bytes (PackStreamFlatHost, bridge.go:3088-3090). */
 int32_t przm_gpu_pack_flat(przm_gpu *g, const uint8_t *src, uint64_t n, uint8_t *out, uint64_t cap,
                            uint64_t *out_len, przm_gpu_profile *prof);
+
+/* ---- the same stages on device buffers (ComputeHistogramGPU bridge.go:2890,
+ *      PackStreamFlatGPU 3033, VANSEncode4xGPU 3161); kernel_ms may be NULL ---- */
+
+/* 256-bin histogram of d_in[0, n). */
+int32_t przm_gpu_histogram_device(przm_gpu *g, const void *d_in, uint64_t n, uint32_t counts[256], double *kernel_ms);
+/* d_out = [0x00][u32 n][d_in[0, n)]; *out_len = n + 5 (0 for an empty input). */
+int32_t przm_gpu_pack_flat_device(przm_gpu *g, const void *d_in, void *d_out, uint64_t n, uint64_t *out_len,
+                                  double *kernel_ms);
+/* The V-ANS container of d_in[0, n) with the caller's table: freq sums to 4096,
+ * cdf[s] = sum of freq[< s] with cdf[256] = 4096, mat the serialized matrix
+ * (przm_serialize_matrix).  A byte of the input with frequency 0 is
+ * PRZM_GPU_ERR_ARG (EncodeVANS4xDirect's ErrCorruptedFrame, literals.go:312).
+ * The container's flag byte is 0x00, as the CPU path writes it (literals.go:297). */
+int32_t przm_gpu_vans_encode_device(przm_gpu *g, const void *d_in, void *d_out, uint64_t n, const uint16_t freq[256],
+                                    const uint16_t cdf[257], const uint8_t *mat, uint32_t mat_len, uint64_t out_cap,
+                                    uint64_t *out_len, double *kernel_ms);
+
+/* ---- device-resident token streams (TokenizeDevice bridge.go:2139,
+ *      TokenizeDeviceZeroCopy 2247, TokenizePointersDeviceOnly 2163,
+ *      DownloadDeviceBytes 2253, EncodeD2D 2466) ---- */
+
+/* The handle's device input buffer of at least n bytes (the reference's
+ * slot.dIn), for callers that stage input before
+ * przm_gpu_tokenize_device_resident. */
+int32_t przm_gpu_input_buffer(przm_gpu *g, uint64_t n, void **d);
+/* Host input: match search and tokenizer, streams left in device memory. */
+int32_t przm_gpu_tokenize_device(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t level,
+                                 przm_device_token_streams *out, przm_gpu_profile *prof);
+/* The same from input already in device memory (kept valid until the frames
+ * are encoded). */
+int32_t przm_gpu_tokenize_device_resident(przm_gpu *g, const void *d_src, uint64_t n, int32_t level,
+                                          przm_device_token_streams *out, przm_gpu_profile *prof);
+/* n bytes at device address d_ptr to h. */
+int32_t przm_gpu_download(przm_gpu *g, void *h, uint64_t d_ptr, uint64_t n);
+/* EncodeD2D: the frames of the handle's latest token streams (one per 64 KiB
+ * block, the same bytes przm_gpu_encode gives).  usize is the tokenized
+ * length; window_log must be 0 (each frame's window log follows its block
+ * length, compress.go:183-202).  Streams of an earlier call are PRZM_GPU_ERR_ARG. */
+int32_t przm_gpu_encode_streams(przm_gpu *g, const przm_device_token_streams *s, uint64_t usize, int32_t window_log,
+                                uint8_t *out, uint64_t out_cap, uint64_t *out_len, uint32_t *frame_sizes,
+                                uint64_t frames_cap, uint64_t *nframes, przm_gpu_profile *prof);
+
+/* ---- probes (BenchmarkVRAMBandwidth bridge.go:2674, BenchmarkD2DEntropy
+ *      2547, BenchmarkEntropyVRAM 3325); results in GB/s ---- */
+
+/* 20 device copies of size_mb MiB: (read + write bytes) / time. */
+int32_t przm_gpu_bench_vram(przm_gpu *g, uint32_t size_mb, double *gbps);
+/* Per iteration, command and offset streams of size/4 bytes packed flat and a
+ * literal stream of size/2 copied into one frame buffer: size bytes / time. */
+int32_t przm_gpu_bench_d2d_entropy(przm_gpu *g, uint32_t size_mb, uint32_t iters, double *gbps);
+/* The coder itself: size_mb of skewed bytes as 64 KiB literal streams, each
+ * histogram, normalisation and 4-lane V-ANS per iteration: bytes / time. */
+int32_t przm_gpu_bench_entropy_vram(przm_gpu *g, uint32_t size_mb, uint32_t iters, double *gbps);
 
 /* ---- encode (CompressD2D bridge.go:2486, EncodeD2D 2466) ---- */
 
@@ -177,6 +261,17 @@ int32_t przm_gpu_sync(przm_gpu *g);
 /* AllocPinnedHost / FreePinnedHost (gpu/memory.go:51-90). */
 int32_t przm_pinned_alloc(uint64_t bytes, void **p);
 int32_t przm_pinned_free(void *p);
+/* PinnedBufferPool (gpu/memory.go:99-165): NewPinnedBufferPool allocates
+ * `initial` buffers of buf_size; Get reuses one or allocates; Put keeps up to
+ * max(initial, 8) and frees the rest; Close (destroy) frees the pool. */
+int32_t przm_pinned_pool_create(uint64_t buf_size, uint32_t initial, przm_pinned_pool **out);
+int32_t przm_pinned_pool_get(przm_pinned_pool *p, void **buf);
+int32_t przm_pinned_pool_put(przm_pinned_pool *p, void *buf);
+int32_t przm_pinned_pool_destroy(przm_pinned_pool *p);
+/* GetPinned (bridge.go:452): a buffer of at least size bytes from the handle's
+ * pool of the next power of two (return it with przm_pinned_pool_put; the
+ * handle's pools are freed by przm_gpu_destroy). */
+int32_t przm_gpu_get_pinned(przm_gpu *g, uint64_t size, void **buf, przm_pinned_pool **pool);
 
 /* ---- host helpers (gpu/bridge.go:2777-2886) ---- */
 /* NormalizeFreq: the deterministic normaliser of core/literals/literals.go:78-140

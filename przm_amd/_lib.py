@@ -47,6 +47,12 @@ class Span(ctypes.Structure):
     _fields_ = [("offset", U64), ("len", U64)]
 
 
+class DeviceTokenStreams(ctypes.Structure):  # przm_device_token_streams (gpu/types.go:90-100)
+    _fields_ = [("d_cmds", U64), ("d_offs", U64), ("d_lits", U64), ("d_totals", U64), ("cmd_len", U32),
+                ("off_len", U32), ("lit_len", U32), ("num_chunks", U32), ("stream", U64), ("d_block_lens", U64),
+                ("uncompressed_size", U64), ("generation", U64)]
+
+
 class Result(ctypes.Structure):
     _fields_ = [("id", U64), ("status", I32), ("nframes", U32), ("out_len", U64), ("submit_ms", ctypes.c_double),
                 ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double)]
@@ -61,11 +67,31 @@ SIGNATURES = {
     "przm_gpu_set_mock_unavailable": (None, [I32]),
     "przm_gpu_set_mock_error": (None, [I32]),
     "przm_gpu_match": (I32, [P, P, U64, I32, P, ctypes.POINTER(GpuProfile)]),
+    "przm_gpu_match_streaming": (I32, [P, P, U64, I32, U64, P, ctypes.POINTER(GpuProfile)]),
     "przm_gpu_tokenize": (I32, [P, P, U64, I32, P, U64, P, U64, P, U64, P, U64, P, U64, P, U64, P, U64,
                                 ctypes.POINTER(GpuProfile)]),
     "przm_gpu_histogram": (I32, [P, P, U64, P, ctypes.POINTER(GpuProfile)]),
     "przm_gpu_vans_encode": (I32, [P, P, U64, P, U64, U64P, ctypes.POINTER(GpuProfile)]),
     "przm_gpu_pack_flat": (I32, [P, P, U64, P, U64, U64P, ctypes.POINTER(GpuProfile)]),
+    "przm_gpu_histogram_device": (I32, [P, P, U64, P, ctypes.POINTER(ctypes.c_double)]),
+    "przm_gpu_pack_flat_device": (I32, [P, P, P, U64, U64P, ctypes.POINTER(ctypes.c_double)]),
+    "przm_gpu_vans_encode_device": (I32, [P, P, P, U64, P, P, P, U32, U64, U64P, ctypes.POINTER(ctypes.c_double)]),
+    "przm_gpu_input_buffer": (I32, [P, U64, ctypes.POINTER(P)]),
+    "przm_gpu_tokenize_device": (I32, [P, P, U64, I32, ctypes.POINTER(DeviceTokenStreams),
+                                       ctypes.POINTER(GpuProfile)]),
+    "przm_gpu_tokenize_device_resident": (I32, [P, P, U64, I32, ctypes.POINTER(DeviceTokenStreams),
+                                                ctypes.POINTER(GpuProfile)]),
+    "przm_gpu_download": (I32, [P, P, U64, U64]),
+    "przm_gpu_encode_streams": (I32, [P, ctypes.POINTER(DeviceTokenStreams), U64, I32, P, U64, U64P, P, U64, U64P,
+                                      ctypes.POINTER(GpuProfile)]),
+    "przm_gpu_bench_vram": (I32, [P, U32, ctypes.POINTER(ctypes.c_double)]),
+    "przm_gpu_bench_d2d_entropy": (I32, [P, U32, U32, ctypes.POINTER(ctypes.c_double)]),
+    "przm_gpu_bench_entropy_vram": (I32, [P, U32, U32, ctypes.POINTER(ctypes.c_double)]),
+    "przm_pinned_pool_create": (I32, [U64, U32, ctypes.POINTER(P)]),
+    "przm_pinned_pool_get": (I32, [P, ctypes.POINTER(P)]),
+    "przm_pinned_pool_put": (I32, [P, P]),
+    "przm_pinned_pool_destroy": (I32, [P]),
+    "przm_gpu_get_pinned": (I32, [P, U64, ctypes.POINTER(P), ctypes.POINTER(P)]),
     "przm_gpu_encode": (I32, [P, P, U64, I32, P, U64, U64P, P, U64, U64P, ctypes.POINTER(GpuProfile)]),
     "przm_gpu_encode_device": (I32, [P, P, U64, I32, P, U64, U64P, P, U64, U64P, ctypes.POINTER(GpuProfile)]),
     "przm_gpu_encode_device_noalloc": (I32, [P, P, U64, I32, P, U64, U64P, P, U64, U64P]),

@@ -841,3 +841,131 @@ przm_frame_write_kernel(const uint8_t *__restrict__ src, const przm_block_t *__r
         else wr_bytes(o + p, wsb + PRZM_WS_LIT, l4);
     }
 }
+
+/* ---------------- stage kernels over arbitrary device buffers ---------------- */
+
+/* dst[0, n) = src[0, n) by the whole grid (thread t of nt): as wr_bytes, with aligned 16-byte
+ * stores built from aligned source words */
+__device__ __forceinline__ void wr_bytes_grid(uint8_t *dst, const uint8_t *src, uint64_t n, uint64_t t, uint64_t nt) {
+    const uint64_t lead = (16u - (uint32_t)((uintptr_t)dst & 15u)) & 15u;
+    const uint64_t head = lead < n ? lead : n;
+    const uint64_t body = (n - head) & ~15ull;
+    for (uint64_t i = t; i < head; i += nt) dst[i] = src[i];
+    for (uint64_t i = head + body + t; i < n; i += nt) dst[i] = src[i];
+    if (!body) return;
+    uint4 *d = (uint4 *)(dst + head);
+    const uint8_t *s = src + head;
+    const uint32_t sh = (uint32_t)((uintptr_t)s & 3u);
+    const uint32_t *sw = (const uint32_t *)(s - sh);
+    const uint64_t n16 = body >> 4;
+    if (sh == 0) {
+        for (uint64_t i = t; i < n16; i += nt) {
+            const uint32_t *w = sw + 4 * i;
+            d[i] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    } else {
+        for (uint64_t i = t; i < n16; i += nt) {
+            const uint32_t *w = sw + 4 * i;
+            const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4];
+            d[i] = make_uint4(__builtin_amdgcn_alignbyte(w1, w0, sh), __builtin_amdgcn_alignbyte(w2, w1, sh),
+                              __builtin_amdgcn_alignbyte(w3, w2, sh), __builtin_amdgcn_alignbyte(w4, w3, sh));
+        }
+    }
+}
+
+/* 256-bin byte histogram of src[0, n) over the whole grid: one LDS copy per wave, 16-byte
+ * loads from the first 16-byte boundary on, the copies summed and added to counts (which the
+ * caller cleared) with one atomic per nonzero bin and workgroup */
+extern "C" __global__ void __launch_bounds__(256)
+przm_hist_grid_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__restrict__ counts) {
+    __shared__ uint32_t cnt[4][256];
+    const uint32_t tid = threadIdx.x, w = tid >> 6;
+    for (uint32_t i = tid; i < 4 * 256; i += 256) (&cnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t gt = (uint64_t)blockIdx.x * 256 + tid, gn = (uint64_t)gridDim.x * 256;
+    const uint64_t lead = (16u - (uint32_t)((uintptr_t)src & 15u)) & 15u;
+    const uint64_t head = lead < n ? lead : n;
+    const uint64_t n16 = (n - head) >> 4;
+    if (gt < head) atomicAdd(&cnt[w][src[gt]], 1u);
+    for (uint64_t i = head + (n16 << 4) + gt; i < n; i += gn) atomicAdd(&cnt[w][src[i]], 1u);
+    const uint4 *p4 = (const uint4 *)(src + head);
+    for (uint64_t i = gt; i < n16; i += gn) {
+        const uint4 a = p4[i];
+        const uint32_t wd[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            atomicAdd(&cnt[w][wd[k] & 0xFFu], 1u);
+            atomicAdd(&cnt[w][(wd[k] >> 8) & 0xFFu], 1u);
+            atomicAdd(&cnt[w][(wd[k] >> 16) & 0xFFu], 1u);
+            atomicAdd(&cnt[w][wd[k] >> 24], 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t c = cnt[0][tid] + cnt[1][tid] + cnt[2][tid] + cnt[3][tid];
+    if (c) atomicAdd(&counts[tid], c);
+}
+
+/* the normalised table (freq | cdf << 16) and joint-nibble matrix of a whole-buffer histogram */
+extern "C" __global__ void __launch_bounds__(64)
+przm_normalize_kernel(const uint32_t *__restrict__ counts, przm_ftab_t *__restrict__ ft, uint32_t *__restrict__ mat_len) {
+    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t key[256];
+    __shared__ uint32_t frq[256];
+    for (uint32_t s = threadIdx.x; s < 256; s += 64) cnt[s] = counts[s];
+    __syncthreads();
+    normalize_wave(cnt, key, frq, ft, mat_len);
+}
+
+/* ModeFlat section [0x00][u32 n][payload] (PackStreamFlat, gpu/kernel/przm_ans.cu:202-234) */
+extern "C" __global__ void __launch_bounds__(256)
+przm_pack_flat_kernel(const uint8_t *__restrict__ src, uint64_t n, uint8_t *__restrict__ out) {
+    const uint64_t gt = (uint64_t)blockIdx.x * 256 + threadIdx.x, gn = (uint64_t)gridDim.x * 256;
+    if (gt == 0) {
+        out[0] = 0x00;
+        wr_le32(out + 1, (uint32_t)n);
+    }
+    wr_bytes_grid(out + 5, src, n, gt, gn);
+}
+
+/* the V-ANS container ('V','A',1,0, u32 n, matrix, four states, stream) of a coder run whose
+ * stream ends at slot_end: res = {stream bytes, x0, x1, x2, x3} */
+extern "C" __global__ void __launch_bounds__(256)
+przm_vans_assemble_kernel(const uint8_t *__restrict__ slot_end, const uint32_t *__restrict__ res,
+                          const przm_ftab_t *__restrict__ ft, uint32_t mat_len, uint32_t n, uint8_t *__restrict__ out) {
+    const uint32_t sl = res[0];
+    if (threadIdx.x == 0) {
+        out[0] = 'V'; out[1] = 'A'; out[2] = 0x01; out[3] = 0x00;
+        wr_le32(out + 4, n);
+        for (int j = 0; j < 4; j++) wr_le32(out + 8 + mat_len + 4 * j, res[1 + j]);
+    }
+    wr_bytes(out + 8, ft->mat, mat_len);
+    wr_bytes(out + 8 + mat_len + 16, slot_end - sl, sl);
+}
+
+/* every block's cmd, off and lit streams to three device arrays at the given offsets
+ * (dst_off[3 * b + k]): one workgroup per block */
+extern "C" __global__ void __launch_bounds__(256)
+przm_gather_streams_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__restrict__ meta,
+                           const uint64_t *__restrict__ dst_off, uint8_t *__restrict__ d_cmd,
+                           uint8_t *__restrict__ d_off, uint8_t *__restrict__ d_lit, uint32_t nblocks) {
+    const uint32_t b = blockIdx.x;
+    if (b >= nblocks) return;
+    const uint8_t *wsb = ws + (uint64_t)b * PRZM_WS_BYTES;
+    const przm_meta_t *mt = meta + b;
+    wr_bytes(d_cmd + dst_off[3 * b], wsb + PRZM_WS_CMD, mt->len[PRZM_S_CMD]);
+    wr_bytes(d_off + dst_off[3 * b + 1], wsb + PRZM_WS_OFF, mt->len[PRZM_S_OFF]);
+    wr_bytes(d_lit + dst_off[3 * b + 2], wsb + PRZM_WS_LIT, mt->len[PRZM_S_LIT]);
+}
+
+/* the coder probe's input: block b's literal stream = src[65536 b, ...), its other streams
+ * empty, so the histogram and coder kernels run one V-ANS job per block */
+extern "C" __global__ void __launch_bounds__(256)
+przm_probe_fill_kernel(const uint8_t *__restrict__ src, uint64_t n, uint8_t *__restrict__ ws,
+                       przm_meta_t *__restrict__ meta, uint32_t nblocks) {
+    const uint32_t b = blockIdx.x;
+    if (b >= nblocks) return;
+    const uint64_t o = (uint64_t)b * PRZM_BLOCK;
+    const uint32_t len = (uint32_t)(n - o < PRZM_BLOCK ? n - o : PRZM_BLOCK);
+    if (threadIdx.x < PRZM_S_N) meta[b].len[threadIdx.x] = threadIdx.x == PRZM_S_LIT ? len : 0u;
+    wr_bytes(ws + (uint64_t)b * PRZM_WS_BYTES + PRZM_WS_LIT, src + o, len);
+}

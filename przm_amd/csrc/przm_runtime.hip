@@ -79,7 +79,7 @@ struct DevBuf {
 /* One set of device buffers + stream: the handle owns one, the server one per slot. */
 struct Ctx {
     hipStream_t st = nullptr;
-    DevBuf in, out, blocks, ws, meta, ftab, flen, foff, run, scratch, dbg, perm;
+    DevBuf in, out, blocks, ws, meta, ftab, flen, foff, run, scratch, dbg, perm, tok, tmp;
     uint64_t gen = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -90,7 +90,8 @@ struct Ctx {
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
-        for (DevBuf *b : {&in, &out, &blocks, &ws, &meta, &ftab, &flen, &foff, &run, &scratch, &dbg, &perm}) b->release();
+        for (DevBuf *b : {&in, &out, &blocks, &ws, &meta, &ftab, &flen, &foff, &run, &scratch, &dbg, &perm, &tok, &tmp})
+            b->release();
         for (auto e : ev) (void)hipEventDestroy(e);
         ev.clear();
         if (st) (void)hipStreamDestroy(st);
@@ -194,6 +195,8 @@ double now_ms() {
 
 }  // namespace
 
+struct przm_pinned_pool;
+
 struct przm_gpu {
     int dev = 0;
     przm_gpu_info info{};
@@ -203,6 +206,32 @@ struct przm_gpu {
     bool use_graph = true;
     Ctx c;
     std::mutex mu;
+    /* calls that launch work on c's buffers, counted: token streams left in c by
+     * przm_gpu_tokenize_device are valid only while this has not moved on */
+    uint64_t epoch = 0;
+    struct { uint64_t epoch = 0, n = 0; const uint8_t *src = nullptr; uint32_t nblocks = 0; int level = 6; } tok;
+    std::map<uint64_t, przm_pinned_pool *> pools; /* GetPinned's pools by power-of-two size */
+    /* the two slots of the double-buffered streaming match (bridge.go:157-236 pipelineSlot):
+     * device buffers and stream, pinned staging of one chunk's input and descriptors */
+    struct Pipe {
+        Ctx c;
+        DevBuf desc;
+        uint8_t *h_in = nullptr;
+        uint32_t *h_desc = nullptr;
+        uint64_t h_cap = 0;
+        hipEvent_t done = nullptr;
+        void release() {
+            c.release();
+            desc.release();
+            if (h_in) (void)hipHostFree(h_in);
+            if (h_desc) (void)hipHostFree(h_desc);
+            if (done) (void)hipEventDestroy(done);
+            h_in = nullptr;
+            h_desc = nullptr;
+            done = nullptr;
+            h_cap = 0;
+        }
+    } pipe[2];
 };
 
 namespace {
@@ -256,7 +285,8 @@ int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg)
  * stage_ev (optional): 8 events per chunk recorded between stages.
  */
 int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uint32_t depth, uint8_t *d_out,
-                bool check, bool stop_after_lz, bool dbg, int grid, uint32_t chunk, std::vector<hipEvent_t> *stage_ev) {
+                bool check, bool stop_after_lz, bool dbg, int grid, uint32_t chunk, std::vector<hipEvent_t> *stage_ev,
+                bool skip_lz = false) {
     hipStream_t st = c.st;
     przm_block_t *blocks = (przm_block_t *)c.blocks.p;
     przm_meta_t *meta = (przm_meta_t *)c.meta.p;
@@ -284,6 +314,7 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         const uint32_t nb = std::min(chunk, nblocks - c0);
         const uint32_t gz = (uint32_t)std::min<int>(grid, (int)nb);
         CHK(rec());
+        if (!skip_lz)
         hipLaunchKernelGGL(przm_lz_kernel, dim3(gz), dim3(LZ_NT), 0, st, d_src, blocks + c0, nb, depth,
                            (uint8_t *)c.ws.p, meta + c0, (uint8_t *)c.scratch.p, (uint64_t)LZ_SCRATCH_STRIDE,
                            dbg ? (uint16_t *)c.dbg.p + (uint64_t)c0 * PRZM_BLOCK : (uint16_t *)nullptr,
@@ -330,6 +361,7 @@ struct BatchOut {
 int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bufs, uint32_t nbufs, int level,
                   uint8_t *d_out, uint64_t out_cap, bool use_graph, std::vector<przm_block_t> &blocks,
                   std::vector<uint64_t> &first, BatchOut &bo, przm_gpu_profile *prof) {
+    g->epoch++;
     build_blocks(bufs, nbufs, blocks, first);
     const uint32_t nblocks = (uint32_t)blocks.size();
     uint64_t bound = 0;
@@ -522,6 +554,12 @@ int32_t przm_gpu_destroy(przm_gpu *g) {
         (void)hipSetDevice(g->dev);
         (void)hipStreamSynchronize(g->c.st);
         g->c.release();
+        for (auto &kv : g->pools) (void)przm_pinned_pool_destroy(kv.second);
+        g->pools.clear();
+        for (auto &pp : g->pipe) {
+            if (pp.c.st) (void)hipStreamSynchronize(pp.c.st);
+            pp.release();
+        }
     }
     delete g;
     return 0;
@@ -776,6 +814,7 @@ int32_t przm_gpu_encode(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t lev
 namespace {
 int32_t stage_upload(przm_gpu *g, const uint8_t *src, uint64_t n, std::vector<przm_block_t> &blocks) {
     Ctx &c = g->c;
+    g->epoch++;
     przm_span sp{0, n};
     std::vector<uint64_t> first;
     build_blocks(&sp, 1, blocks, first);
@@ -875,86 +914,7 @@ int32_t przm_gpu_tokenize(przm_gpu *g, const uint8_t *src, uint64_t n, int32_t l
     return 0;
 }
 
-int32_t przm_gpu_histogram(przm_gpu *g, const uint8_t *src, uint64_t n, uint32_t counts[256], przm_gpu_profile *prof) {
-    if (!g || !counts || (!src && n)) return PRZM_GPU_ERR_ARG;
-    CHK(mock_check());
-    std::lock_guard<std::mutex> lk(g->mu);
-    HIPCHK(hipSetDevice(g->dev));
-    Ctx &c = g->c;
-    CHK(c.in.ensure(n + 64, &c.gen));
-    CHK(c.meta.ensure(4096, &c.gen));
-    if (n) HIPCHK(hipMemcpyAsync(c.in.p, src, n, hipMemcpyHostToDevice, c.st));
-    hipLaunchKernelGGL(przm_hist1_kernel, dim3(1), dim3(256), 0, c.st, (const uint8_t *)c.in.p, n,
-                       (uint32_t *)c.meta.p, (przm_ftab_t *)nullptr, (uint32_t *)nullptr);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(counts, c.meta.p, 1024, hipMemcpyDeviceToHost, c.st));
-    HIPCHK(hipStreamSynchronize(c.st));
-    (void)prof;
-    return 0;
-}
-
-int32_t przm_gpu_vans_encode(przm_gpu *g, const uint8_t *src, uint64_t n, uint8_t *out, uint64_t cap,
-                             uint64_t *out_len, przm_gpu_profile *prof) {
-    if (!g || !out || !out_len || (!src && n) || n > (16ull << 20)) return PRZM_GPU_ERR_ARG;
-    CHK(mock_check());
-    std::lock_guard<std::mutex> lk(g->mu);
-    HIPCHK(hipSetDevice(g->dev));
-    if (n == 0) { *out_len = 0; return 0; }
-    Ctx &c = g->c;
-    const uint64_t slot = (2 * n + 64 + 15) & ~15ull; /* the coder's slot end is 16-byte aligned */
-    CHK(c.in.ensure(n + 64, &c.gen));
-    CHK(c.out.ensure(slot, &c.gen));
-    CHK(c.meta.ensure(4096 + sizeof(przm_ftab_t) + 64, &c.gen));
-    uint8_t *mb = (uint8_t *)c.meta.p;
-    przm_ftab_t *ft = (przm_ftab_t *)(mb + 2048);
-    uint32_t *res = (uint32_t *)(mb + 1024);
-    HIPCHK(hipMemcpyAsync(c.in.p, src, n, hipMemcpyHostToDevice, c.st));
-    hipLaunchKernelGGL(przm_hist1_kernel, dim3(1), dim3(256), 0, c.st, (const uint8_t *)c.in.p, n, (uint32_t *)mb, ft,
-                       res + 8);
-    hipLaunchKernelGGL(przm_rans1_kernel, dim3(1), dim3(64), 0, c.st, (const uint8_t *)c.in.p, (uint32_t)n,
-                       (const przm_ftab_t *)ft, (uint8_t *)c.out.p + slot, (uint32_t)(2 * n + 16), res);
-    HIPCHK(hipGetLastError());
-    uint32_t hres[9];
-    przm_ftab_t hft;
-    HIPCHK(hipMemcpyAsync(hres, res, sizeof hres, hipMemcpyDeviceToHost, c.st));
-    HIPCHK(hipMemcpyAsync(&hft, ft, sizeof hft, hipMemcpyDeviceToHost, c.st));
-    HIPCHK(hipStreamSynchronize(c.st));
-    const uint32_t sl = hres[0], ml = hres[8];
-    if (sl == PRZM_ENC_OVERFLOW) return PRZM_GPU_ERR_HIP;
-    const uint64_t total = 8 + ml + 16 + sl;
-    if (total > cap) return PRZM_GPU_ERR_CAPACITY;
-    out[0] = 'V'; out[1] = 'A'; out[2] = 0x01; out[3] = 0x00;
-    for (int k = 0; k < 4; k++) out[4 + k] = (uint8_t)(n >> (8 * k));
-    std::memcpy(out + 8, hft.mat, ml);
-    for (int j = 0; j < 4; j++)
-        for (int k = 0; k < 4; k++) out[8 + ml + 4 * j + k] = (uint8_t)(hres[1 + j] >> (8 * k));
-    HIPCHK(hipMemcpy(out + 8 + ml + 16, (uint8_t *)c.out.p + slot - sl, sl, hipMemcpyDeviceToHost));
-    *out_len = total;
-    (void)prof;
-    return 0;
-}
-
-int32_t przm_gpu_pack_flat(przm_gpu *g, const uint8_t *src, uint64_t n, uint8_t *out, uint64_t cap, uint64_t *out_len,
-                           przm_gpu_profile *prof) {
-    if (!g || !out || !out_len || (!src && n) || n > 0xFFFFFFFFull) return PRZM_GPU_ERR_ARG;
-    CHK(mock_check());
-    if (cap < n + 5) return PRZM_GPU_ERR_CAPACITY;
-    std::lock_guard<std::mutex> lk(g->mu);
-    HIPCHK(hipSetDevice(g->dev));
-    Ctx &c = g->c;
-    CHK(c.in.ensure(n + 64, &c.gen));
-    CHK(c.out.ensure(n + 64, &c.gen));
-    if (n) {
-        HIPCHK(hipMemcpy(c.in.p, src, n, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy((uint8_t *)c.out.p + 5, c.in.p, n, hipMemcpyDeviceToDevice));
-        HIPCHK(hipMemcpy(out + 5, (uint8_t *)c.out.p + 5, n, hipMemcpyDeviceToHost));
-    }
-    out[0] = 0x00;
-    for (int k = 0; k < 4; k++) out[1 + k] = (uint8_t)(n >> (8 * k));
-    *out_len = n + 5;
-    (void)prof;
-    return 0;
-}
+/* przm_gpu_histogram, przm_gpu_vans_encode and przm_gpu_pack_flat: przm_stage_api.hip */
 
 /* ---- host helpers ---- */
 
@@ -1136,7 +1096,7 @@ int32_t host_grow(uint8_t **p, uint64_t *cap, uint64_t need) {
 uint64_t ctx_bytes(const Ctx &c) {
     uint64_t t = 0;
     for (const DevBuf *b : {&c.in, &c.out, &c.blocks, &c.ws, &c.meta, &c.ftab, &c.flen, &c.foff, &c.run, &c.scratch,
-                            &c.dbg, &c.perm})
+                            &c.dbg, &c.perm, &c.tok, &c.tmp})
         t += b->cap;
     return t;
 }
@@ -1497,3 +1457,5 @@ int32_t przm_serve_destroy(przm_server *s) {
 }
 
 }  // extern "C"
+
+#include "przm_stage_api.hip"

@@ -27,6 +27,13 @@ MaxOffset32 = 1048575
 MaxMatchLen64 = 65535
 # frame block of this implementation (one self-contained frame per block)
 BlockSize = 64 * 1024
+# bridge.go:148-153, the reference tokenizer's tile and per-tile stream slots.  Here the tile is
+# the 64 KiB block (window and extension bounded by it), so the reference's GPU tests that
+# expect a 1 MiB window and a 4095 length cap (gpu_tier2_boundaries_test.go) see 64 KiB and 1024.
+TokenizerTileSize = 32768
+MaxTileCmdBytes = 36864
+MaxTileOffBytes = 36864
+MaxTileLitBytes = 36864
 
 
 class GPUError(Exception):
@@ -54,6 +61,93 @@ def UnpackMatch32(desc: int) -> Tuple[int, int]:  # gpu/types.go:36-39
 
 def IsMatch32(desc: int) -> bool:  # gpu/types.go:41-43
     return (desc >> 20) >= MinMatchLen
+
+
+@dataclass
+class MatchDesc64:  # gpu/types.go:46-52
+    Length: int = 0
+    Offset: int = 0
+    Literal: int = 0
+    Flags: int = 0  # 1: is_match
+    Reserved: int = 0
+
+
+@dataclass
+class DeviceTokenStreams:  # gpu/types.go:90-100 (device pointers borrowed from the compressor)
+    DFinalCmds: int = 0
+    DFinalOffs: int = 0
+    DFinalLits: int = 0
+    DTotals: int = 0
+    CmdLen: int = 0
+    OffLen: int = 0
+    LitLen: int = 0
+    NumChunks: int = 0
+    Stream: int = 0
+    DBlockLens: int = 0
+    raw: object = None  # the C struct, handed back to EncodeD2D
+
+
+@dataclass
+class PinnedBuffer:  # gpu/memory.go:93-97
+    Bytes: np.ndarray
+    Ptr: int
+
+
+def AllocPinnedHost(size: int) -> Tuple[np.ndarray, int]:  # memory.go:51
+    if size <= 0:
+        raise GPUError(L.ERR_ARG, f"invalid size {size}")
+    p = ctypes.c_void_p()
+    rc = L.lib().przm_pinned_alloc(size, ctypes.byref(p))
+    if rc:
+        raise GPUError(rc)
+    return _pinned_view(p.value, size), p.value
+
+
+def FreePinnedHost(ptr: int) -> None:  # memory.go:78
+    if ptr:
+        rc = L.lib().przm_pinned_free(ctypes.c_void_p(ptr))
+        if rc:
+            raise GPUError(rc)
+
+
+def _pinned_view(ptr: int, size: int) -> np.ndarray:
+    return np.ctypeslib.as_array((ctypes.c_uint8 * size).from_address(ptr))
+
+
+class PinnedBufferPool:  # memory.go:99-165
+    def __init__(self, bufSize: int, initialCap: int = 0, _handle=None):
+        self._lib = L.lib()
+        self.bufSize = bufSize
+        if _handle is not None:
+            self._h = _handle
+            self._owned = False
+            return
+        h = ctypes.c_void_p()
+        rc = self._lib.przm_pinned_pool_create(bufSize, initialCap, ctypes.byref(h))
+        if rc:
+            raise GPUError(rc)
+        self._h = h
+        self._owned = True
+
+    def Get(self) -> PinnedBuffer:
+        p = ctypes.c_void_p()
+        rc = self._lib.przm_pinned_pool_get(self._h, ctypes.byref(p))
+        if rc:
+            raise GPUError(rc)
+        return PinnedBuffer(_pinned_view(p.value, self.bufSize), p.value)
+
+    def Put(self, buf: PinnedBuffer) -> None:
+        if buf.Ptr:
+            self._lib.przm_pinned_pool_put(self._h, ctypes.c_void_p(buf.Ptr))
+
+    def Close(self) -> None:
+        if self._owned and self._h:
+            self._lib.przm_pinned_pool_destroy(self._h)
+        self._h = None
+
+
+def NewPinnedBufferPool(bufSize: int, initialCap: int = 0) -> PinnedBufferPool:  # memory.go:110
+    return PinnedBufferPool(bufSize, initialCap)
 
 
 @dataclass
@@ -216,13 +310,161 @@ class GPUCompressor:
         a = _u8(src)
         if a.size == 0:
             return [], GPUProfile()
+        if a.size > 16 << 20:  # as the reference: large payloads through the streaming path, 8 MiB chunks
+            return self.MatchLZ77Streaming(a, 8 << 20, level)
         desc, prof = self.MatchDescriptors(a, level)
         pos = np.nonzero(desc)[0]
         out = [LZ77Match(int(p), int(desc[p] & 0xFFFF), int(desc[p] >> 16), int(a[p])) for p in pos]
         return out, prof
 
-    def MatchLZ77Streaming(self, src, chunkSize: int = 8 << 20, level: int = 6):  # bridge.go:1461
-        return self.MatchLZ77(src, level)
+    def MatchDescriptorsStreaming(self, src, chunkSize: int = 0, level: int = 6):
+        """MatchDescriptors through the double-buffered chunked path."""
+        self._mock()
+        a = _u8(src)
+        desc = np.zeros(max(a.size, 1), np.uint32)
+        p = L.GpuProfile()
+        self._check(self._lib.przm_gpu_match_streaming(self._h, _ptr(a), a.size, level, chunkSize, _ptr(desc),
+                                                       ctypes.byref(p)))
+        return desc[:a.size], _profile(p, self)
+
+    def MatchLZ77Streaming(self, src, chunkSize: int = 0, level: int = 6):  # bridge.go:1461
+        a = _u8(src)
+        if a.size == 0:
+            return [], GPUProfile()
+        desc, prof = self.MatchDescriptorsStreaming(a, chunkSize, level)
+        pos = np.nonzero(desc)[0]
+        return [LZ77Match(int(p), int(desc[p] & 0xFFFF), int(desc[p] >> 16), int(a[p])) for p in pos], prof
+
+    # -- device-resident streams and device-buffer stages --
+    def InputBuffer(self, n: int) -> int:
+        """The compressor's device input buffer of at least n bytes (the reference's slot.dIn)."""
+        d = ctypes.c_void_p()
+        self._check(self._lib.przm_gpu_input_buffer(self._h, n, ctypes.byref(d)))
+        return d.value
+
+    def _streams(self, raw: L.DeviceTokenStreams) -> DeviceTokenStreams:
+        return DeviceTokenStreams(raw.d_cmds, raw.d_offs, raw.d_lits, raw.d_totals, raw.cmd_len, raw.off_len,
+                                  raw.lit_len, raw.num_chunks, raw.stream, raw.d_block_lens, raw)
+
+    def TokenizeDevice(self, src, level: int = 6) -> Tuple[DeviceTokenStreams, TokenizerProfile]:  # bridge.go:2139
+        self._mock()
+        a = _u8(src)
+        if a.size == 0:
+            return DeviceTokenStreams(), TokenizerProfile()
+        raw = L.DeviceTokenStreams()
+        p = L.GpuProfile()
+        self._check(self._lib.przm_gpu_tokenize_device(self._h, _ptr(a), a.size, level, ctypes.byref(raw),
+                                                       ctypes.byref(p)))
+        return self._streams(raw), self._tok_profile(raw, p, a.size)
+
+    def TokenizeDeviceZeroCopy(self, src, level: int = 6):  # bridge.go:2247
+        return self.TokenizeDevice(src, level)
+
+    def TokenizePointersDeviceOnly(self, n: int, dIn: Optional[int] = None, level: int = 6):  # bridge.go:2163
+        """Token streams of n bytes already in device memory (the compressor's input buffer by default)."""
+        if n <= 0:
+            return DeviceTokenStreams(), TokenizerProfile()
+        d = dIn if dIn is not None else self.InputBuffer(n)
+        raw = L.DeviceTokenStreams()
+        p = L.GpuProfile()
+        self._check(self._lib.przm_gpu_tokenize_device_resident(self._h, ctypes.c_void_p(d), n, level,
+                                                                ctypes.byref(raw), ctypes.byref(p)))
+        return self._streams(raw), self._tok_profile(raw, p, n)
+
+    def TokenizeDeviceOnly(self, n: int, dIn: Optional[int] = None, level: int = 6):  # bridge.go:1915
+        """Host copies of the cmd / offset / literal streams of n bytes already in device memory."""
+        if n <= 0:
+            return b"", b"", b"", TokenizerProfile()
+        st, prof = self.TokenizePointersDeviceOnly(n, dIn, level)
+        return (self.DownloadDeviceBytes(st.DFinalCmds, st.CmdLen), self.DownloadDeviceBytes(st.DFinalOffs, st.OffLen),
+                self.DownloadDeviceBytes(st.DFinalLits, st.LitLen), prof)
+
+    @staticmethod
+    def _tok_profile(raw, p, n) -> TokenizerProfile:
+        return TokenizerProfile(KernelTimeMs=p.kernel_ms, Pass1Ms=p.stage_ms[1], Pass2Ms=p.kernel_ms - p.stage_ms[1],
+                                TotalRoundTripMs=p.kernel_ms,
+                                ThroughputGBps=n / (p.kernel_ms * 1e6) if p.kernel_ms > 0 else 0.0,
+                                CmdBytes=raw.cmd_len, OffsetBytes=raw.off_len, LiteralBytes=raw.lit_len,
+                                NumChunks=raw.num_chunks)
+
+    def DownloadDeviceBytes(self, dPtr: int, size: int) -> bytes:  # bridge.go:2253
+        if size == 0:
+            return b""
+        out = np.zeros(size, np.uint8)
+        self._check(self._lib.przm_gpu_download(self._h, _ptr(out), dPtr, size))
+        return out.tobytes()
+
+    def EncodeD2D(self, streams: DeviceTokenStreams, uSize: int, windowLog: int = 0):  # bridge.go:2466
+        """Frames of the latest token streams: (frames, frame sizes, profile)."""
+        if uSize <= 0:
+            return b"", np.zeros(0, np.uint32), GPUProfile()
+        bound = int(self._lib.przm_gpu_frames_bound(uSize))
+        out = np.zeros(bound + 64, np.uint8)
+        nb = max((uSize + BlockSize - 1) // BlockSize, 1)
+        sizes = np.zeros(nb, np.uint32)
+        olen, nf = ctypes.c_uint64(), ctypes.c_uint64()
+        p = L.GpuProfile()
+        self._check(self._lib.przm_gpu_encode_streams(self._h, ctypes.byref(streams.raw), uSize, windowLog, _ptr(out),
+                                                      out.size, ctypes.byref(olen), _ptr(sizes), nb, ctypes.byref(nf),
+                                                      ctypes.byref(p)))
+        return out[: olen.value].tobytes(), sizes[: nf.value].copy(), _profile(p, self)
+
+    def ComputeHistogramGPU(self, dIn: int, n: int, stream: int = 0):  # bridge.go:2890
+        """(counts, kernel ms, GB/s) of device bytes dIn[0, n); the compressor's own stream is used."""
+        counts = np.zeros(256, np.uint32)
+        if n <= 0:
+            return counts, 0.0, 0.0
+        ms = ctypes.c_double()
+        self._check(self._lib.przm_gpu_histogram_device(self._h, ctypes.c_void_p(dIn), n, _ptr(counts),
+                                                        ctypes.byref(ms)))
+        return counts, ms.value, n / (ms.value * 1e6) if ms.value > 0 else 0.0
+
+    def PackStreamFlatGPU(self, dIn: int, dOut: int, n: int, stream: int = 0):  # bridge.go:3033
+        if n <= 0:
+            return 0, 0.0, 0.0
+        olen, ms = ctypes.c_uint64(), ctypes.c_double()
+        self._check(self._lib.przm_gpu_pack_flat_device(self._h, ctypes.c_void_p(dIn), ctypes.c_void_p(dOut), n,
+                                                        ctypes.byref(olen), ctypes.byref(ms)))
+        return olen.value, ms.value, 2 * n / (ms.value * 1e6) if ms.value > 0 else 0.0
+
+    def VANSEncode4xGPU(self, dIn: int, dOut: int, n: int, freq, cdf, matBytes, stream: int = 0,
+                        outCap: Optional[int] = None):  # bridge.go:3161
+        if n <= 0:
+            return 0, 0.0, 0.0
+        f = np.ascontiguousarray(freq, np.uint16)
+        c = np.ascontiguousarray(cdf, np.uint16)
+        m = np.frombuffer(bytes(matBytes), np.uint8) if len(matBytes) else np.zeros(1, np.uint8)
+        olen, ms = ctypes.c_uint64(), ctypes.c_double()
+        cap = outCap if outCap is not None else 8 + len(matBytes) + 16 + 2 * n + 16
+        self._check(self._lib.przm_gpu_vans_encode_device(self._h, ctypes.c_void_p(dIn), ctypes.c_void_p(dOut), n,
+                                                          _ptr(f), _ptr(c), _ptr(m), len(matBytes), cap,
+                                                          ctypes.byref(olen), ctypes.byref(ms)))
+        return olen.value, ms.value, n / (ms.value * 1e6) if ms.value > 0 else 0.0
+
+    # -- probes --
+    def BenchmarkVRAMBandwidth(self, sizeMB: int) -> float:  # bridge.go:2674
+        v = ctypes.c_double()
+        self._check(self._lib.przm_gpu_bench_vram(self._h, sizeMB, ctypes.byref(v)))
+        return v.value
+
+    def BenchmarkD2DEntropy(self, sizeMB: int = 16, iterations: int = 10) -> float:  # bridge.go:2547
+        v = ctypes.c_double()
+        self._check(self._lib.przm_gpu_bench_d2d_entropy(self._h, sizeMB, iterations, ctypes.byref(v)))
+        return v.value
+
+    def BenchmarkEntropyVRAM(self, sizeMB: int = 16, iterations: int = 10) -> float:  # bridge.go:3325
+        v = ctypes.c_double()
+        self._check(self._lib.przm_gpu_bench_entropy_vram(self._h, sizeMB, iterations, ctypes.byref(v)))
+        return v.value
+
+    # -- pinned memory --
+    def GetPinned(self, size: int) -> Tuple[PinnedBuffer, PinnedBufferPool]:  # bridge.go:452
+        p, pool = ctypes.c_void_p(), ctypes.c_void_p()
+        self._check(self._lib.przm_gpu_get_pinned(self._h, size, ctypes.byref(p), ctypes.byref(pool)))
+        cap = 1
+        while cap < size:
+            cap <<= 1
+        return PinnedBuffer(_pinned_view(p.value, size), p.value), PinnedBufferPool(cap, _handle=pool)
 
     def TokenizeStreams(self, src, level: int = 6):
         """Every stream of every block: dict of cmd/off/lit/op/meta/var byte arrays + per-block lengths."""
