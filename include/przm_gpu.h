@@ -282,11 +282,6 @@ int32_t przm_build_cdf(const uint16_t freq[256], uint16_t cdf[257]);
 /* SerializeMatrix: joint-nibble matrix of literals.go:143-184 (<= 546 bytes). */
 int32_t przm_serialize_matrix(const uint16_t freq[256], uint8_t *out, uint64_t cap, uint64_t *len);
 
-/* Diagnostics: when d is a device array of 32 x blocks u64, the LZ kernel
- * records up to 32 s_memrealtime (100 MHz) stamps per block at its phase edges;
- * NULL turns it off. */
-void przm_gpu_debug_phase_ts(void *d);
-
 /* Text of the last HIP / capacity error on this thread (empty if none). */
 const char *przm_gpu_last_error(void);
 

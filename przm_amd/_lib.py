@@ -120,7 +120,6 @@ SIGNATURES = {
     "przm_serve_stats": (I32, [P, U64P]),
     "przm_serve_destroy": (I32, [P]),
     "przm_gpu_last_error": (ctypes.c_char_p, []),
-    "przm_gpu_debug_phase_ts": (None, [P]),
 }
 
 _lib = None

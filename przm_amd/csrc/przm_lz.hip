@@ -1031,7 +1031,11 @@ __device__ __forceinline__ Tok seg_tok(const LzShared &S, uint32_t cxm, uint32_t
 
 __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint8_t *__restrict__ wsb,
                            uint64_t *__restrict__ pts) {
+#ifdef PRZM_DIAG
 #define QTS(i) do { if (pts && threadIdx.x == 0) ((__attribute__((address_space(1))) uint64_t *)pts)[i] = __builtin_amdgcn_s_memrealtime(); } while (0)
+#else
+#define QTS(i) ((void)0)
+#endif
     const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
     /* the streams are in global memory: said in their type, since this function is called
      * (not inlined) and would otherwise store through flat addresses, whose completion every
@@ -1430,12 +1434,18 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
 
 /* ---------------- kernel ---------------- */
 
-extern "C" __global__ void __launch_bounds__(LZ_NT)
-przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks, uint32_t nblocks,
-               uint32_t depth, uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta,
-               uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint16_t *__restrict__ dist_dbg,
-               uint64_t *__restrict__ phase_ts, uint32_t *__restrict__ next_block) {
-    __shared__ LzShared S;
+/*
+ * The kernel body.  MATCH_OUT (the match-stage entry point only) also writes every position's
+ * match distance to dist_out; phase_ts (builds with PRZM_DIAG only) receives s_memrealtime
+ * stamps at the phase edges of every block.
+ */
+template <bool MATCH_OUT>
+__device__ __forceinline__ void lz_body(LzShared &S, const uint8_t *__restrict__ src,
+                                        const przm_block_t *__restrict__ blocks, uint32_t nblocks, uint32_t depth,
+                                        uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta,
+                                        uint8_t *__restrict__ scratch, uint64_t scratch_stride,
+                                        uint16_t *__restrict__ dist_out, uint64_t *__restrict__ phase_ts,
+                                        uint32_t *__restrict__ next_block) {
     const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
     uint8_t *my = scratch + (uint64_t)blockIdx.x * scratch_stride;
     Scr sc;
@@ -1458,7 +1468,11 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
         przm_meta_t *mt = meta + blk;
         uint8_t *wsb = ws + (uint64_t)blk * PRZM_WS_BYTES;
         uint64_t *ts = phase_ts ? phase_ts + (uint64_t)blk * 32 : nullptr;
+#ifdef PRZM_DIAG
 #define PTS(i) do { if (ts && tid == 0) ts[i] = __builtin_amdgcn_s_memrealtime(); } while (0)
+#else
+#define PTS(i) ((void)0)
+#endif
         PTS(0);
         if ((bd.flags & PRZM_BF_RAW) || n == 0) {
             if (tid < PRZM_S_N) mt->len[tid] = 0;
@@ -1653,9 +1667,9 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
             }
             }
             __syncthreads();
-            if (dist_dbg)
+            if (MATCH_OUT)
                 for (uint32_t i = tid; i < b - a; i += LZ_NT)
-                    dist_dbg[(uint64_t)blk * PRZM_BLOCK + a + i] = S.big.q.dist[DIX(i)];
+                    dist_out[(uint64_t)blk * PRZM_BLOCK + a + i] = S.big.q.dist[DIX(i)];
             PTS(4 + 2 * Q);
             if (S.st[ST_ENTRY] < b) parse_tile(S, a, b, n, wsb, (ts && Q == 0) ? ts + 16 : nullptr);
             __syncthreads();
@@ -1684,3 +1698,34 @@ przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__
         __syncthreads();
     }
 }
+
+/* the encode pipeline's match finder and tokenizer */
+extern "C" __global__ void __launch_bounds__(LZ_NT)
+przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks, uint32_t nblocks,
+               uint32_t depth, uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta,
+               uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint32_t *__restrict__ next_block) {
+    __shared__ LzShared S;
+    lz_body<false>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, nullptr, nullptr, next_block);
+}
+
+/* the same, also writing every position's match distance (przm_gpu_match's descriptors) */
+extern "C" __global__ void __launch_bounds__(LZ_NT)
+przm_lz_match_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks, uint32_t nblocks,
+                     uint32_t depth, uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta,
+                     uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint16_t *__restrict__ dist_out,
+                     uint32_t *__restrict__ next_block) {
+    __shared__ LzShared S;
+    lz_body<true>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, dist_out, nullptr, next_block);
+}
+
+#ifdef PRZM_DIAG
+/* diagnostics builds: the encode kernel with 32 phase stamps per block */
+extern "C" __global__ void __launch_bounds__(LZ_NT)
+przm_lz_diag_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks, uint32_t nblocks,
+                    uint32_t depth, uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta,
+                    uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint64_t *__restrict__ phase_ts,
+                    uint32_t *__restrict__ next_block) {
+    __shared__ LzShared S;
+    lz_body<false>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, nullptr, phase_ts, next_block);
+}
+#endif

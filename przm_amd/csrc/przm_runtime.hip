@@ -30,7 +30,9 @@
 namespace {
 
 thread_local char g_err[512];
-uint64_t *g_phase_ts = nullptr; /* optional per-block phase timestamps of the LZ kernel (device memory) */
+#ifdef PRZM_DIAG
+uint64_t *g_phase_ts = nullptr; /* per-block phase timestamps of the LZ kernel (device memory) */
+#endif
 std::atomic<int32_t> g_mock_unavailable{0};
 std::atomic<int32_t> g_mock_error{0};
 
@@ -314,12 +316,23 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         const uint32_t nb = std::min(chunk, nblocks - c0);
         const uint32_t gz = (uint32_t)std::min<int>(grid, (int)nb);
         CHK(rec());
-        if (!skip_lz)
-        hipLaunchKernelGGL(przm_lz_kernel, dim3(gz), dim3(LZ_NT), 0, st, d_src, blocks + c0, nb, depth,
-                           (uint8_t *)c.ws.p, meta + c0, (uint8_t *)c.scratch.p, (uint64_t)LZ_SCRATCH_STRIDE,
-                           dbg ? (uint16_t *)c.dbg.p + (uint64_t)c0 * PRZM_BLOCK : (uint16_t *)nullptr,
-                           g_phase_ts ? g_phase_ts + (uint64_t)c0 * 32 : (uint64_t *)nullptr,
-                           (uint32_t *)((uint8_t *)c.run.p + 64) + c0 / chunk);
+        uint32_t *counter = (uint32_t *)((uint8_t *)c.run.p + 64) + c0 / chunk;
+        if (skip_lz) {
+        } else if (dbg) {
+            hipLaunchKernelGGL(przm_lz_match_kernel, dim3(gz), dim3(LZ_NT), 0, st, d_src, blocks + c0, nb, depth,
+                               (uint8_t *)c.ws.p, meta + c0, (uint8_t *)c.scratch.p, (uint64_t)LZ_SCRATCH_STRIDE,
+                               (uint16_t *)c.dbg.p + (uint64_t)c0 * PRZM_BLOCK, counter);
+#ifdef PRZM_DIAG
+        } else if (g_phase_ts) {
+            hipLaunchKernelGGL(przm_lz_diag_kernel, dim3(gz), dim3(LZ_NT), 0, st, d_src, blocks + c0, nb, depth,
+                               (uint8_t *)c.ws.p, meta + c0, (uint8_t *)c.scratch.p, (uint64_t)LZ_SCRATCH_STRIDE,
+                               g_phase_ts + (uint64_t)c0 * 32, counter);
+#endif
+        } else {
+            hipLaunchKernelGGL(przm_lz_kernel, dim3(gz), dim3(LZ_NT), 0, st, d_src, blocks + c0, nb, depth,
+                               (uint8_t *)c.ws.p, meta + c0, (uint8_t *)c.scratch.p, (uint64_t)LZ_SCRATCH_STRIDE,
+                               counter);
+        }
         HIPCHK(hipGetLastError());
         if (stop_after_lz) continue;
         CHK(rec());
@@ -503,9 +516,11 @@ extern "C" {
 
 const char *przm_gpu_last_error(void) { return g_err; }
 
-/* Development hook: when set, the LZ kernel writes 16 s_memrealtime stamps per block
- * (100 MHz) into this device array (16 x blocks u64). */
-void przm_gpu_debug_phase_ts(void *d) { g_phase_ts = (uint64_t *)d; }
+#ifdef PRZM_DIAG
+/* Diagnostics builds only (not part of the library's interface): when set, the LZ kernel writes
+ * up to 32 s_memrealtime stamps (100 MHz) per block into this device array (32 x blocks u64). */
+extern "C" void przm_gpu_debug_phase_ts(void *d) { g_phase_ts = (uint64_t *)d; }
+#endif
 
 void przm_gpu_set_mock_unavailable(int32_t on) { g_mock_unavailable.store(on ? 1 : 0); }
 void przm_gpu_set_mock_error(int32_t code) { g_mock_error.store(code); }
