@@ -199,7 +199,7 @@ def main():
     fsz = np.zeros(nblocks, np.uint32)
     nf = ctypes.c_uint64()
 
-    tuned = (ctypes.c_int32 * 3)()
+    tuned = (ctypes.c_int32 * 4)()
     if args.lz_grid > 0 and args.chunk > 0:
         lib.przm_gpu_set_tuning(g.handle, args.lz_grid, args.chunk)
         tuned[0], tuned[1] = args.lz_grid, args.chunk
@@ -209,7 +209,9 @@ def main():
         rc = lib.przm_gpu_autotune(g.handle, d_src, spans, nbuf, args.level, tuned)
         if rc:
             raise G.GPUError(rc)
-        log(f"autotune lz_grid={tuned[0]} chunk={tuned[1]} in {time.perf_counter() - t:.1f}s")
+        log(f"autotune lz_grid={tuned[0]} chunk={tuned[1]} rans_jobs={tuned[2]} search_batch={tuned[3]} in "
+            f"{time.perf_counter() - t:.1f}s")
+    lib.przm_gpu_get_tuning(g.handle, tuned)
 
     def batched(prof=None):
         rc = lib.przm_gpu_encode_batch_device(g.handle, d_src, spans, nbuf, args.level, d_out, bound + 64,
@@ -385,7 +387,10 @@ def main():
                                "(JSON / int32 tables / motif-random / uniform), 64 KiB frames, level 6 (chain depth 16), "
                                "one hipGraph launch per step",
                    "buffers": nbuf, "input_bytes": total_in, "frame_bytes": total_frames, "blocks": nblocks,
-                   "level": args.level, "lz_grid": int(tuned[0]), "chunk_blocks": int(tuned[1])},
+                   "level": args.level, "lz_grid": int(tuned[0]), "chunk_blocks": int(tuned[1]),
+                   "rans_jobs_per_workgroup": int(tuned[2]), "search_batch": int(tuned[3]),
+                   "tuning": "fixed" if (args.lz_grid > 0 and args.chunk > 0) else
+                             ("defaults" if args.no_tune else "autotuned")},
         "device_ms_per_step": round(float(np.mean(kms)), 3),
         "stages_ms": stages,
         "ungraphed_device_ms": round(prof.kernel_ms, 3),

@@ -240,12 +240,20 @@ int32_t przm_gpu_encode_batch(przm_gpu *g, const uint8_t *src, const przm_span *
 uint64_t przm_gpu_frames_bound(uint64_t n);
 
 /* ---- autotuner ---- */
-/* Times candidate launch shapes on a sample of the given device arena and
- * keeps the fastest for this handle; tuned[0..2] = lz grid, chunk blocks,
- * rans jobs per workgroup. */
+/* Times candidate launch shapes on a sample (up to ~4 GiB from the front) of
+ * the given device arena, one stage at a time, and keeps the fastest for this
+ * handle: the match kernel's workgroups and the blocks per launch, then its
+ * chain candidates per search step, then the coder's jobs per workgroup.
+ * tuned[0..3] = lz grid, chunk blocks, coder jobs per workgroup, search batch. */
 int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs, uint32_t nbufs, int32_t level,
-                          int32_t tuned[3]);
+                          int32_t tuned[4]);
+/* lz_grid, chunk_blocks: 0 keeps the current value. */
 int32_t przm_gpu_set_tuning(przm_gpu *g, int32_t lz_grid, int32_t chunk_blocks);
+/* search_batch in {2, 4, 8}, rans_jobs in {16, 32}; 0 keeps the current value.
+ * No shape changes an output byte. */
+int32_t przm_gpu_set_kernel_shape(przm_gpu *g, int32_t search_batch, int32_t rans_jobs);
+/* tuned[0..3] as przm_gpu_autotune reports them. */
+int32_t przm_gpu_get_tuning(przm_gpu *g, int32_t tuned[4]);
 /* use_graph = 0 runs the batched form as direct launches (per-stage profiling). */
 int32_t przm_gpu_set_options(przm_gpu *g, int32_t use_graph);
 

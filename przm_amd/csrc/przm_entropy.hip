@@ -242,6 +242,8 @@ przm_hist1_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__restr
 
 /* ---------------- rANS ---------------- */
 
+/* coder workgroups of RANS_NT threads (one lane quad per job, RANS_JOBS jobs): the default;
+ * przm_rans_kernel_128 has twice as many per workgroup (the autotuner picks one) */
 #define RANS_NT 64
 #define RANS_JOBS (RANS_NT / 4)
 
@@ -515,19 +517,27 @@ przm_rans_order_kernel(const przm_meta_t *__restrict__ meta, uint32_t nblocks, u
  * LDS.  Inside a kind the blocks come in the order perm gives
  * (przm_rans_order_kernel).  grid = PRZM_J_N * ceil(nblocks / RANS_JOBS).
  */
-extern "C" __global__ void __launch_bounds__(RANS_NT)
-przm_rans_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const przm_ftab_t *__restrict__ ftab,
-                 uint32_t nblocks, const uint32_t *__restrict__ perm) {
-    __shared__ union {
-        uint32_t vt[RANS_JOBS][256];
+template <uint32_t NT>
+struct RansShared {
+    union {
+        uint32_t vt[NT / 4][256];
         uint32_t m8[8][256];
     } T;
-    __shared__ __align__(16) uint8_t stage[RANS_JOBS][RANS_STAGE + 16]; /* + a dummy byte per lane */
+    __align__(16) uint8_t stage[NT / 4][RANS_STAGE + 16]; /* + a dummy byte per lane */
+};
+
+template <uint32_t NT>
+__device__ __forceinline__ void rans_body(RansShared<NT> &SH, uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta,
+                                          const przm_ftab_t *__restrict__ ftab, uint32_t nblocks,
+                                          const uint32_t *__restrict__ perm) {
+    constexpr uint32_t RANS_JOBS_T = NT / 4;
+    auto &T = SH.T;
+    auto &stage = SH.stage;
     const uint32_t tid = threadIdx.x, q = tid >> 2, j = tid & 3;
-    const uint32_t per_kind = (nblocks + RANS_JOBS - 1) / RANS_JOBS;
+    const uint32_t per_kind = (nblocks + RANS_JOBS_T - 1) / RANS_JOBS_T;
     const uint32_t kind = blockIdx.x / per_kind;
     if (kind >= PRZM_J_N) return;
-    const uint32_t slot = (blockIdx.x - kind * per_kind) * RANS_JOBS + q;
+    const uint32_t slot = (blockIdx.x - kind * per_kind) * RANS_JOBS_T + q;
     const uint32_t blk = slot < nblocks ? perm[(uint64_t)kind * nblocks + slot] : nblocks;
     const bool m8 = kind == PRZM_J_OP_M8 || kind == PRZM_J_META_M8 || kind == PRZM_J_VAR_M8;
     bool run = blk < nblocks;
@@ -542,7 +552,7 @@ przm_rans_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const
         run = n >= minlen;
     }
     if (m8) {
-        for (uint32_t e = tid; e < 8u * 256u; e += RANS_NT) (&T.m8[0][0])[e] = przm_m8.fc[e >> 8][e & 255u];
+        for (uint32_t e = tid; e < 8u * 256u; e += NT) (&T.m8[0][0])[e] = przm_m8.fc[e >> 8][e & 255u];
     } else if (run) {
         const przm_ftab_t *ft = ftab + (uint64_t)blk * 4 + tsel;
         for (uint32_t s = j; s < 256; s += 4) T.vt[q][s] = ft->fc[s];
@@ -564,6 +574,20 @@ przm_rans_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const
     przm_meta_t *mt = meta + blk;
     if (j == 0) mt->enc_len[kind] = K;
     mt->enc_state[kind][j] = x;
+}
+
+/* grid = PRZM_J_N * ceil(nblocks / (threads / 4)) */
+extern "C" __global__ void __launch_bounds__(64)
+przm_rans_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const przm_ftab_t *__restrict__ ftab,
+                 uint32_t nblocks, const uint32_t *__restrict__ perm) {
+    __shared__ RansShared<64> SH;
+    rans_body<64>(SH, ws, meta, ftab, nblocks, perm);
+}
+extern "C" __global__ void __launch_bounds__(128)
+przm_rans_kernel_128(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, const przm_ftab_t *__restrict__ ftab,
+                     uint32_t nblocks, const uint32_t *__restrict__ perm) {
+    __shared__ RansShared<128> SH;
+    rans_body<128>(SH, ws, meta, ftab, nblocks, perm);
 }
 
 /* a single V-ANS job over an arbitrary buffer (one quad); slot_end 16-byte aligned */

@@ -54,9 +54,8 @@ typedef __attribute__((address_space(1))) uint8_t gu8;
 #define NSEG (QP / SEG)           /* 512: one per thread */
 #define WS_CAP 1536               /* bucket elements kept in LDS per wave */
 #define RING 128                  /* staged search keys per wave: previous + current chunk */
-#ifndef SEARCH_BATCH
-#define SEARCH_BATCH 4            /* chain candidates read per step of the search */
-#endif
+/* chain candidates read per step of the search: a template parameter B of the kernels (2, 4 or
+ * 8, picked per workload by the autotuner; 4 by default) */
 #define WS_MAXV 5                 /* values per workgroup scan */
 #define NONE16 0xFFFFu
 #define NONE32 0xFFFFFFFFu
@@ -643,7 +642,7 @@ __device__ __forceinline__ void bucket_fetch(const LzShared &S, const Scr &sc, u
     }
 }
 
-template <bool LDSWS>
+template <bool LDSWS, uint32_t B>
 __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt, uint32_t n, uint32_t depth,
                             bool may_queue, const uint32_t pre[4], Pend &pd, uint64_t *pt) {
     /* pt (diagnostics, thread 0 only): accumulated sort, chunk staging, candidate scan and result
@@ -808,9 +807,9 @@ __device__ bool bucket_sort(LzShared &S, const Scr &sc, uint32_t w, uint32_t dgt
     if (prof) pt[0] += t_c - t_a;
     const uint16_t *GS = S.sm.gstart[w];
     /* the groups with a chain are searched; the single positions come last and keep "no match" */
-    if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
-    else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
-    else bucket_search_range<false, SEARCH_BATCH>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
+    if (depth > 64u) bucket_search_range<true, B>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
+    else if (depth < B) bucket_search_range<false, 2>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
+    else bucket_search_range<false, B>(S, sc, dgt, n, depth, SRT, GS, 0, msing, 64, pd, pt);
     return false;
 }
 
@@ -1439,7 +1438,7 @@ __device__ void parse_tile(LzShared &S, uint32_t a, uint32_t b, uint32_t n, uint
  * match distance to dist_out; phase_ts (builds with PRZM_DIAG only) receives s_memrealtime
  * stamps at the phase edges of every block.
  */
-template <bool MATCH_OUT>
+template <bool MATCH_OUT, uint32_t B>
 __device__ __forceinline__ void lz_body(LzShared &S, const uint8_t *__restrict__ src,
                                         const przm_block_t *__restrict__ blocks, uint32_t nblocks, uint32_t depth,
                                         uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta,
@@ -1588,9 +1587,9 @@ __device__ __forceinline__ void lz_body(LzShared &S, const uint8_t *__restrict__
                 uint32_t nxt[4];
                 bucket_fetch(S, sc, dn, nxt);
                 const uint32_t m = (S.bstart[0][dg + 1] - S.bstart[0][dg]) + (S.bstart[1][dg + 1] - S.bstart[1][dg]);
-                if (m != 0 && m <= WS_CAP) bucket_sort<true>(S, sc, w, dg, n, depth, false, pre, pd, ts ? ts + 27 : nullptr);
+                if (m != 0 && m <= WS_CAP) bucket_sort<true, B>(S, sc, w, dg, n, depth, false, pre, pd, ts ? ts + 27 : nullptr);
                 else if (m != 0) {
-                    const bool q = bucket_sort<false>(S, sc, w, dg, n, depth, S.bigq[w] == 0, pre, pd,
+                    const bool q = bucket_sort<false, B>(S, sc, w, dg, n, depth, S.bigq[w] == 0, pre, pd,
                                                       ts ? ts + 27 : nullptr);
                     if (q && lane == 0) S.bigq[w] = dg + 1;
                 }
@@ -1613,9 +1612,9 @@ __device__ __forceinline__ void lz_body(LzShared &S, const uint8_t *__restrict__
             const uint16_t *SRT = (const uint16_t *)gw + PRZM_BLOCK;
             const uint32_t *GE = (const uint32_t *)(gw + 2u * PRZM_BLOCK * 2u);
             Pend pq = {0u, NONE32};
-            if (depth > 64u) bucket_search_range<true, SEARCH_BATCH>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64, pq);
-            else if (depth < SEARCH_BATCH) bucket_search_range<false, 2>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64, pq);
-            else bucket_search_range<false, SEARCH_BATCH>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64, pq);
+            if (depth > 64u) bucket_search_range<true, B>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64, pq);
+            else if (depth < B) bucket_search_range<false, 2>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64, pq);
+            else bucket_search_range<false, B>(S, sc, dg, n, depth, SRT, GE, w * 64, m, LZ_NW * 64, pq);
             if (pq.a != NONE32) sc.D[pq.a] = pq.v;
         }
         __threadfence_block();
@@ -1699,14 +1698,19 @@ __device__ __forceinline__ void lz_body(LzShared &S, const uint8_t *__restrict__
     }
 }
 
-/* the encode pipeline's match finder and tokenizer */
-extern "C" __global__ void __launch_bounds__(LZ_NT)
-przm_lz_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks, uint32_t nblocks,
-               uint32_t depth, uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta,
-               uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint32_t *__restrict__ next_block) {
-    __shared__ LzShared S;
-    lz_body<false>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, nullptr, nullptr, next_block);
-}
+/* the encode pipeline's match finder and tokenizer, B chain candidates per search step */
+#define PRZM_LZ_KERNEL(NAME, B)                                                                                       \
+    extern "C" __global__ void __launch_bounds__(LZ_NT)                                                              \
+    NAME(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks, uint32_t nblocks, uint32_t depth, \
+         uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, uint8_t *__restrict__ scratch,                     \
+         uint64_t scratch_stride, uint32_t *__restrict__ next_block) {                                               \
+        __shared__ LzShared S;                                                                                       \
+        lz_body<false, B>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, nullptr, nullptr,       \
+                          next_block);                                                                               \
+    }
+PRZM_LZ_KERNEL(przm_lz_kernel, 4)
+PRZM_LZ_KERNEL(przm_lz_kernel_b2, 2)
+PRZM_LZ_KERNEL(przm_lz_kernel_b8, 8)
 
 /* the same, also writing every position's match distance (przm_gpu_match's descriptors) */
 extern "C" __global__ void __launch_bounds__(LZ_NT)
@@ -1715,7 +1719,7 @@ przm_lz_match_kernel(const uint8_t *__restrict__ src, const przm_block_t *__rest
                      uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint16_t *__restrict__ dist_out,
                      uint32_t *__restrict__ next_block) {
     __shared__ LzShared S;
-    lz_body<true>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, dist_out, nullptr, next_block);
+    lz_body<true, 4>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, dist_out, nullptr, next_block);
 }
 
 #ifdef PRZM_DIAG
@@ -1726,6 +1730,6 @@ przm_lz_diag_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restr
                     uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint64_t *__restrict__ phase_ts,
                     uint32_t *__restrict__ next_block) {
     __shared__ LzShared S;
-    lz_body<false>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, nullptr, phase_ts, next_block);
+    lz_body<false, 4>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, nullptr, phase_ts, next_block);
 }
 #endif

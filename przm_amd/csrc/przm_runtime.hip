@@ -85,7 +85,7 @@ struct Ctx {
     uint64_t gen = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    struct { const void *src; void *out; uint32_t nblocks, depth, grid, chunk; uint64_t gen; } gkey{};
+    struct { const void *src; void *out; uint32_t nblocks, depth, grid, chunk, sb, rnt; uint64_t gen; } gkey{};
     std::vector<hipEvent_t> ev;
     void release() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -205,6 +205,8 @@ struct przm_gpu {
     int cus = 0;
     int lz_grid = 0;
     uint32_t chunk = 8192;
+    uint32_t search_batch = 4; /* chain candidates per search step: 2, 4 or 8 (which LZ kernel) */
+    uint32_t rans_nt = 64;     /* coder threads per workgroup: 64 or 128 (which coder kernel) */
     bool use_graph = true;
     Ctx c;
     std::mutex mu;
@@ -329,9 +331,9 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
                                g_phase_ts + (uint64_t)c0 * 32, counter);
 #endif
         } else {
-            hipLaunchKernelGGL(przm_lz_kernel, dim3(gz), dim3(LZ_NT), 0, st, d_src, blocks + c0, nb, depth,
-                               (uint8_t *)c.ws.p, meta + c0, (uint8_t *)c.scratch.p, (uint64_t)LZ_SCRATCH_STRIDE,
-                               counter);
+            auto *k = g->search_batch == 2 ? przm_lz_kernel_b2 : g->search_batch == 8 ? przm_lz_kernel_b8 : przm_lz_kernel;
+            hipLaunchKernelGGL(k, dim3(gz), dim3(LZ_NT), 0, st, d_src, blocks + c0, nb, depth, (uint8_t *)c.ws.p,
+                               meta + c0, (uint8_t *)c.scratch.p, (uint64_t)LZ_SCRATCH_STRIDE, counter);
         }
         HIPCHK(hipGetLastError());
         if (stop_after_lz) continue;
@@ -341,7 +343,9 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         CHK(rec());
         hipLaunchKernelGGL(przm_rans_order_kernel, dim3(PRZM_J_N), dim3(1024), 0, st, (const przm_meta_t *)(meta + c0), nb,
                            (uint32_t *)c.perm.p);
-        hipLaunchKernelGGL(przm_rans_kernel, dim3(PRZM_J_N * ((nb + RANS_JOBS - 1) / RANS_JOBS)), dim3(RANS_NT), 0, st,
+        const uint32_t jobs = g->rans_nt / 4;
+        hipLaunchKernelGGL(g->rans_nt == 128 ? przm_rans_kernel_128 : przm_rans_kernel,
+                           dim3(PRZM_J_N * ((nb + jobs - 1) / jobs)), dim3(g->rans_nt), 0, st,
                            (uint8_t *)c.ws.p, meta + c0, (const przm_ftab_t *)c.ftab.p, nb, (const uint32_t *)c.perm.p);
         CHK(rec());
         hipLaunchKernelGGL(przm_frame_size_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, blocks + c0, meta + c0,
@@ -403,7 +407,7 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
     if (use_graph) {
         const bool hit = c.gexec && c.gkey.src == d_src && c.gkey.out == d_out && c.gkey.nblocks == nblocks &&
                          c.gkey.depth == depth && c.gkey.grid == (uint32_t)grid && c.gkey.chunk == chunk &&
-                         c.gkey.gen == c.gen;
+                         c.gkey.sb == g->search_batch && c.gkey.rnt == g->rans_nt && c.gkey.gen == c.gen;
         if (!hit) {
             if (c.gexec) { (void)hipGraphExecDestroy(c.gexec); c.gexec = nullptr; }
             if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
@@ -415,7 +419,7 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
             HIPCHK(e2);
             c.graph = gr;
             HIPCHK(hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
-            c.gkey = {d_src, d_out, nblocks, depth, (uint32_t)grid, chunk, c.gen};
+            c.gkey = {d_src, d_out, nblocks, depth, (uint32_t)grid, chunk, g->search_batch, g->rans_nt, c.gen};
         }
         HIPCHK(hipGraphLaunch(c.gexec, c.st));
     } else {
@@ -596,6 +600,26 @@ int32_t przm_gpu_set_tuning(przm_gpu *g, int32_t lz_grid, int32_t chunk_blocks) 
     std::lock_guard<std::mutex> lk(g->mu);
     if (lz_grid > 0) g->lz_grid = lz_grid;
     if (chunk_blocks > 0) g->chunk = (uint32_t)chunk_blocks;
+    return 0;
+}
+
+int32_t przm_gpu_set_kernel_shape(przm_gpu *g, int32_t search_batch, int32_t rans_jobs) {
+    if (!g) return PRZM_GPU_ERR_ARG;
+    if (search_batch && search_batch != 2 && search_batch != 4 && search_batch != 8) return PRZM_GPU_ERR_ARG;
+    if (rans_jobs && rans_jobs != 16 && rans_jobs != 32) return PRZM_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (search_batch) g->search_batch = (uint32_t)search_batch;
+    if (rans_jobs) g->rans_nt = (uint32_t)rans_jobs * 4;
+    return 0;
+}
+
+int32_t przm_gpu_get_tuning(przm_gpu *g, int32_t tuned[4]) {
+    if (!g || !tuned) return PRZM_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    tuned[0] = g->lz_grid;
+    tuned[1] = (int32_t)g->chunk;
+    tuned[2] = (int32_t)(g->rans_nt / 4);
+    tuned[3] = (int32_t)g->search_batch;
     return 0;
 }
 
@@ -995,7 +1019,7 @@ int32_t przm_serialize_matrix(const uint16_t freq[256], uint8_t *out, uint64_t c
 /* ---- autotuner ---- */
 
 int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs, uint32_t nbufs, int32_t level,
-                          int32_t tuned[3]) {
+                          int32_t tuned[4]) {
     if (!g || !bufs || !nbufs) return PRZM_GPU_ERR_ARG;
     CHK(mock_check());
     /* sample: buffers from the front up to ~4 GiB, so that the sample spans several launches of
@@ -1008,34 +1032,55 @@ int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs,
     for (uint32_t i = 0; i < ns; i++) bound += przm_gpu_frames_bound(bufs[i].len);
     void *d_out = nullptr;
     CHK(przm_gpu_malloc(g, bound + 64, &d_out));
+    int32_t rc = 0;
+    /* best of three passes of the sample (graph replays) in the current shape */
+    auto time_shape = [&](double *t) -> int32_t {
+        *t = 1e30;
+        for (int rep = 0; rep < 3; rep++) {
+            przm_gpu_profile p{};
+            CHK(przm_gpu_encode_batch_device(g, d_src, bufs, ns, level, d_out, bound + 64, nullptr, nullptr, 0,
+                                             nullptr, &p));
+            if (rep > 0) *t = std::min(*t, p.kernel_ms);
+        }
+        return 0;
+    };
+    /* coordinate search, one stage at a time: the LZ grid and the blocks per launch, then the LZ
+     * kernel's chain candidates per step, then the coder's jobs per workgroup */
     const int grids[] = {g->cus, 2 * g->cus};
     const int chunks[] = {16384, 32768, 65536}; /* 65536 blocks: 32 GB of block workspace */
-    double best = 1e30;
-    int bg = g->lz_grid, bc = (int)g->chunk;
-    int32_t rc = 0;
+    int bg = g->lz_grid, bc = (int)g->chunk, bsb = 4, brj = 16;
+    przm_gpu_set_kernel_shape(g, bsb, brj);
+    double best = 1e30, t = 0;
     for (int gi : grids) {
         for (int ci : chunks) {
             przm_gpu_set_tuning(g, gi, ci);
-            double t = 1e30;
-            for (int rep = 0; rep < 3 && rc == 0; rep++) {
-                przm_gpu_profile p{};
-                rc = przm_gpu_encode_batch_device(g, d_src, bufs, ns, level, d_out, bound + 64, nullptr, nullptr, 0,
-                                                  nullptr, &p);
-                if (rep > 0) t = std::min(t, p.kernel_ms);
-            }
-            if (rc) break;
+            if ((rc = time_shape(&t))) break;
             if (t < best) { best = t; bg = gi; bc = ci; }
         }
         if (rc) break;
     }
+    przm_gpu_set_tuning(g, bg, bc);
+    for (int sb : {2, 8}) {
+        if (rc) break;
+        przm_gpu_set_kernel_shape(g, sb, brj);
+        if ((rc = time_shape(&t))) break;
+        if (t < best) { best = t; bsb = sb; }
+    }
+    przm_gpu_set_kernel_shape(g, bsb, brj);
+    if (!rc) {
+        przm_gpu_set_kernel_shape(g, bsb, 32);
+        rc = time_shape(&t);
+        if (!rc && t < best) { best = t; brj = 32; }
+    }
     (void)przm_gpu_free(g, d_out);
     przm_gpu_set_tuning(g, bg, bc);
+    przm_gpu_set_kernel_shape(g, bsb, brj);
     {
         std::lock_guard<std::mutex> lk(g->mu);
         if (g->c.gexec) { (void)hipGraphExecDestroy(g->c.gexec); g->c.gexec = nullptr; }
         if (g->c.graph) { (void)hipGraphDestroy(g->c.graph); g->c.graph = nullptr; }
     }
-    if (tuned) { tuned[0] = bg; tuned[1] = bc; tuned[2] = RANS_JOBS; }
+    if (tuned) { tuned[0] = bg; tuned[1] = bc; tuned[2] = brj; tuned[3] = bsb; }
     return rc;
 }
 

@@ -199,3 +199,57 @@ def test_hazard_frame_is_uncompressed(g):
     _, info = oracle().encode_block(a)
     assert info.kind == 3
     assert frames == of and frames[4] == 0x04 and np.array_equal(sizes, osz)
+
+
+def _shape(g, sb, rj):
+    from przm_amd import _lib as L
+    assert L.lib().przm_gpu_set_kernel_shape(g.handle, sb, rj) == 0
+
+
+def test_kernel_shapes_give_identical_frames(g):
+    """Every shape the autotuner may pick (chain candidates per search step 2 / 4 / 8, coder jobs
+    per workgroup 16 / 32) gives the default shape's frames, on every chain-depth class."""
+    arena, offs, sizes = D.batch(40, 1024, 1 << 20, 71)
+    bufs = [arena[int(o): int(o) + int(s)] for o, s in zip(offs, sizes)]
+    deep = np.frombuffer(b"The quick brown fox jumps over the lazy dog. " * 1500, np.uint8).copy()
+    ref_batch, _ = g.EncodeBatch(bufs)
+    ref_lv = {lv: g.CompressFrames(deep, lv)[0] for lv in (1, 3, 6, 10, 11)}
+    try:
+        for sb, rj in ((2, 16), (8, 16), (4, 32), (2, 32), (8, 32)):
+            _shape(g, sb, rj)
+            res, _ = g.EncodeBatch(bufs)
+            for i, ((f, s), (rf, rs)) in enumerate(zip(res, ref_batch)):
+                assert f == rf and np.array_equal(s, rs), (sb, rj, i)
+            for lv, want in ref_lv.items():
+                assert g.CompressFrames(deep, lv)[0] == want, (sb, rj, lv)
+    finally:
+        _shape(g, 4, 16)
+    assert ref_lv[11] == oracle().encode_buffer(deep, 11)[0]
+
+
+def test_autotune_picks_a_valid_shape(g):
+    from przm_amd import _lib as L
+    lib = L.lib()
+    arena, offs, sizes = D.batch(60, 1024, 2 << 20, 72)
+    d = ctypes.c_void_p()
+    assert lib.przm_gpu_malloc(g.handle, arena.size + 64, ctypes.byref(d)) == 0
+    try:
+        lib.przm_gpu_h2d(g.handle, d, ctypes.c_void_p(arena.ctypes.data), arena.size)
+        spans = (L.Span * len(sizes))()
+        for i, (o, s) in enumerate(zip(offs, sizes)):
+            spans[i].offset, spans[i].len = int(o), int(s)
+        tuned = (ctypes.c_int32 * 4)()
+        assert lib.przm_gpu_autotune(g.handle, d, spans, len(sizes), 6, tuned) == 0
+        assert tuned[0] in (g.SMCount, 2 * g.SMCount) and tuned[1] in (16384, 32768, 65536)
+        assert tuned[2] in (16, 32) and tuned[3] in (2, 4, 8)
+        got = (ctypes.c_int32 * 4)()
+        lib.przm_gpu_get_tuning(g.handle, got)
+        assert list(got) == list(tuned)
+        bufs = [arena[int(o): int(o) + int(s)] for o, s in zip(offs, sizes)]
+        res, _ = g.EncodeBatch(bufs)
+        want = encode_many(bufs)
+        assert all(f == of for (f, _), (of, _) in zip(res, want))
+    finally:
+        lib.przm_gpu_free(g.handle, d)
+        _tuning(g, g.SMCount, 8192)
+        _shape(g, 4, 16)
