@@ -1060,6 +1060,44 @@ int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs,
         if (rc) break;
     }
     przm_gpu_set_tuning(g, bg, bc);
+    /* past the sample: the whole batch in one launch (one coder tail instead of one per chunk),
+     * when its block workspace fits in free device memory with a quarter to spare; compared with
+     * the best chunk on the whole batch */
+    uint64_t nb_all = 0, bound_all = 0;
+    for (uint32_t i = 0; i < nbufs; i++) {
+        nb_all += bufs[i].len ? (bufs[i].len + PRZM_BLOCK - 1) / PRZM_BLOCK : 1;
+        bound_all += przm_gpu_frames_bound(bufs[i].len);
+    }
+    size_t mfree = 0, mtotal = 0;
+    if (!rc && nb_all > (uint64_t)bc && nb_all < (1ull << 31) && hipMemGetInfo(&mfree, &mtotal) == hipSuccess) {
+        const uint64_t held = g->c.ws.cap + g->c.ftab.cap + g->c.perm.cap;
+        const uint64_t need = nb_all * ((uint64_t)PRZM_WS_BYTES + 4 * sizeof(przm_ftab_t) + PRZM_J_N * 4);
+        if (need + bound_all + (mtotal >> 2) < mfree + held) {
+            (void)przm_gpu_free(g, d_out);
+            d_out = nullptr;
+            rc = przm_gpu_malloc(g, bound_all + 64, &d_out);
+            const uint32_t saved_ns = ns;
+            const uint64_t saved_bound = bound;
+            const double best_sample = best;
+            ns = nbufs;
+            bound = bound_all;
+            double t_best = 1e30, t_all = 1e30;
+            if (!rc) rc = time_shape(&t_best);
+            if (!rc) {
+                przm_gpu_set_tuning(g, bg, (int)nb_all);
+                rc = time_shape(&t_all);
+            }
+            if (!rc && t_all < t_best) bc = (int)nb_all;
+            przm_gpu_set_tuning(g, bg, bc);
+            if (!rc && bc == (int)nb_all) {
+                best = t_all; /* the later stages are timed on the whole batch too */
+            } else {
+                ns = saved_ns; /* the later stages time the sample again */
+                bound = saved_bound;
+                best = best_sample;
+            }
+        }
+    }
     for (int sb : {2, 8}) {
         if (rc) break;
         przm_gpu_set_kernel_shape(g, sb, brj);
@@ -1072,7 +1110,7 @@ int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs,
         rc = time_shape(&t);
         if (!rc && t < best) { best = t; brj = 32; }
     }
-    (void)przm_gpu_free(g, d_out);
+    if (d_out) (void)przm_gpu_free(g, d_out);
     przm_gpu_set_tuning(g, bg, bc);
     przm_gpu_set_kernel_shape(g, bsb, brj);
     {
