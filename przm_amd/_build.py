@@ -2,6 +2,7 @@
 
 libprzm_gpu.so    HIP kernels + host runtime behind include/przm_gpu.h
 libprzm_datagen.so seeded synthetic inputs (host C)
+libprzm_decode.so  CPU decoder of the frames (host C, include/przm_decode.h)
 """
 import os
 import subprocess
@@ -43,8 +44,16 @@ def build_datagen():
     return out
 
 
+def build_decoder():
+    out = os.path.join(HERE, "libprzm_decode.so")
+    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", "-Wextra",
+          os.path.join(CSRC, "przm_decode.c"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build():
-    return [build_gpu(), build_datagen()]
+    return [build_gpu(), build_datagen(), build_decoder()]
 
 
 if __name__ == "__main__":

@@ -106,6 +106,8 @@ def test_frames_match_oracle_and_decode(g):
         assert frames == of, f"{name}: frame bytes differ"
         if R is not None and a.size:
             assert R.decompress_frames(frames, sizes) == a.tobytes(), name
+        from przm_amd import decode as X
+        assert X.DecompressFrames(frames, sizes) == a.tobytes(), name
 
 
 @pytest.mark.parametrize("level", [1, 3, 5, 9, 10, 11])
@@ -169,10 +171,12 @@ def test_batched_equals_alone(g):
 
 def test_full_size_markov_256mib_roundtrip(g):
     """256 MiB, 4096 blocks: the plain-GPU case at its real size; decode round trip."""
+    from przm_amd import decode as X
     a = D.markov(256 << 20, 1)
     frames, sizes, prof = g.CompressFrames(a)
     assert sizes.size == 4096 and int(sizes.sum()) == len(frames)
     assert len(frames) < a.size
+    assert X.DecompressFrames(frames, sizes, a.size) == a.tobytes()  # every frame, the library's own decoder
     if have_refdec():
         R = refdec()
         f = np.frombuffer(frames, np.uint8)

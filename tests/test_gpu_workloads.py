@@ -91,11 +91,14 @@ def test_batched_workload_shape_matches_oracle(g):
 
 
 def test_mixture_1gib_plain(g):
-    """The 1 GiB mixture buffer: 16,384 frames whose sizes add up, every 37th through the reference
-    decoder, every 8th against the oracle and the first 64 byte for byte."""
+    """The 1 GiB mixture buffer: 16,384 frames whose sizes add up, all through the library's own
+    decoder, every 37th through the reference decoder, every 8th against the oracle and the
+    first 64 byte for byte."""
+    from przm_amd import decode as X
     a = D.mixture(1 << 30, 2)
     frames, sizes, prof = g.CompressFrames(a)
     assert sizes.size == 16384 and int(sizes.sum()) == len(frames)
+    assert X.DecompressFrames(frames, sizes, a.size) == a.tobytes()  # every frame, the library's own decoder
     f = np.frombuffer(frames, np.uint8)
     offs = np.concatenate([[0], np.cumsum(sizes.astype(np.int64))])
     if have_refdec():
