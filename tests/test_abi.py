@@ -71,3 +71,16 @@ def test_pack_match32_reference_kats():
         assert G.UnpackMatch32(packed) == (length, offset)
     assert G.PackMatch32(3, 10) == 0
     assert G.PackMatch32(10, 0) == 0
+
+
+DEC_HEADER = os.path.join(ROOT, "include", "przm_decode.h")
+DEC_LIB = os.path.join(ROOT, "przm_amd", "libprzm_decode.so")
+
+
+@pytest.mark.skipif(not os.path.exists(DEC_LIB), reason="libprzm_decode.so not built (run __graft_entry__.build())")
+def test_decoder_library_exports_its_header():
+    text = re.sub(r"/\*.*?\*/", "", open(DEC_HEADER).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(przm_decode_\w+)\s*\(", text)))
+    assert names == ["przm_decode_frame", "przm_decode_frames", "przm_decode_size"], names
+    lib = ctypes.CDLL(DEC_LIB)
+    assert all(hasattr(lib, n) for n in names)
