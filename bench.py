@@ -112,6 +112,7 @@ def profile_only(args, g, lib, L, D, G):
     """The program of the rocprofv3 --pmc passes: a copy kernel of known size, then one batched pass."""
     if args.lz_grid > 0 and args.chunk > 0:
         lib.przm_gpu_set_tuning(g.handle, args.lz_grid, args.chunk)
+    lib.przm_gpu_set_kernel_shape(g.handle, args.search_batch, args.rans_jobs)
     n = 256 << 20
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     for x in (a, b):
@@ -150,6 +151,8 @@ def main():
     ap.add_argument("--lz-grid", type=int, default=0, help="fixed LZ workgroups (skips the autotuner with --chunk)")
     ap.add_argument("--chunk", type=int, default=0, help="fixed blocks per launch (skips the autotuner with --lz-grid)")
     ap.add_argument("--profile-only", action="store_true")
+    ap.add_argument("--search-batch", type=int, default=0, help="fixed chain candidates per search step (2, 4, 8)")
+    ap.add_argument("--rans-jobs", type=int, default=0, help="fixed coder jobs per workgroup (16, 32)")
     ap.add_argument("--profile-buffers", type=int, default=1500)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -202,6 +205,7 @@ def main():
     tuned = (ctypes.c_int32 * 4)()
     if args.lz_grid > 0 and args.chunk > 0:
         lib.przm_gpu_set_tuning(g.handle, args.lz_grid, args.chunk)
+        lib.przm_gpu_set_kernel_shape(g.handle, args.search_batch, args.rans_jobs)
         tuned[0], tuned[1] = args.lz_grid, args.chunk
         log(f"fixed tuning lz_grid={args.lz_grid} chunk={args.chunk}")
     elif not args.no_tune:
@@ -296,9 +300,10 @@ def main():
         ach = alg / (avg_ms * 1e6)
         tr = pmc_traffic()
         traffic = int(tr["hbm_bytes_per_block"] * nblocks / max(n_lz, 1)) if tr else None
+        kname = {2: "przm_lz_kernel_b2", 8: "przm_lz_kernel_b8"}.get(int(tuned[3]), "przm_lz_kernel")
         roofline = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBPS, 5), "traffic": traffic,
-                    "kernel": "przm_lz_kernel", "launches": launches, "avg_launch_ms": round(avg_ms, 3),
+                    "kernel": kname, "launches": launches, "avg_launch_ms": round(avg_ms, 3),
                     "algorithmic_bytes_per_launch": int(alg), "blocks_per_launch": round(nblocks / max(n_lz, 1), 1),
                     "traffic_source": "profiles/lz_kernel_pmc.json" if tr else None}
 
