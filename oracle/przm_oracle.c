@@ -25,6 +25,8 @@
  *   Meta8 encoder             core/commands/vans_meta.go:45-150
  *   substream / Mode 8        core/commands/commands.go:371-444, 537-568
  *   Section 3 flat LEB128     core/distance/distance.go:825-849 (flat branch)
+ *   Section 3 prefix option   core/distance/distance.go:165-202, 217-247,
+ *                             323-376, 692-849 (PackageOffsetsSection)
  *   Section 4                 core/literals/literals.go:664-693 without the
  *                             sheaf trials
  *   frame assembly            compress.go:113-132, 183-202, 962-1033
@@ -43,6 +45,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "../przm_amd/csrc/przm_dist_freq.h"
 #include "../przm_amd/csrc/przm_meta8_freq.h"
 
 #define ORC_BLOCK 65536u
@@ -472,6 +475,137 @@ int64_t orc_package_cmd_stream(const uint8_t *cmd, uint32_t n, uint8_t *out, int
     return len;
 }
 
+/* ------------------------------------------------------------------ */
+/* Section 3 as a distance-prefix container (the encoder option that   */
+/* brings PackageOffsetsSection of the reference CPU path to mode G).  */
+
+/* distance.go:165-180: code, extra bits and their count of a distance d >= 1 */
+void orc_dist_code(uint32_t d, uint8_t *code, uint32_t *extra, uint8_t *nb) {
+    if (d <= 4) { *code = (uint8_t)(d - 1); *extra = 0; *nb = 0; return; }
+    if (d <= 16777216u) {
+        const uint32_t u = d - 1;
+        const uint8_t b = (uint8_t)(32 - __builtin_clz(u) - 2);
+        const uint32_t half = (u >> b) & 1u;
+        *code = (uint8_t)(2 * b + 2 + half);
+        *extra = d - (((2u + half) << b) + 1u);
+        *nb = b;
+        return;
+    }
+    *code = 48; *extra = d - 16777217u; *nb = 24;
+}
+
+/* distance.go:190-202 */
+static int dist_ctx(uint8_t prev) { return prev < 6 ? 0 : prev < 14 ? 1 : prev < 26 ? 2 : 3; }
+
+/* distance.go:323-376: the codes by one rANS state whose table is chosen by the class of the
+ * previous code (0 before the first); [0x0C][u32 final state][stream].  out holds 5 + 2n. */
+uint32_t orc_dist_m12_encode(const uint8_t *codes, uint32_t n, uint8_t *out) {
+    uint16_t cdf[4][64];
+    for (int c = 0; c < 4; c++) {
+        uint32_t cum = 0;
+        for (int s = 0; s < 64; s++) { cdf[c][s] = (uint16_t)cum; cum += przm_dist_freq[c][s]; }
+    }
+    uint8_t *stream = (uint8_t *)malloc(2 * (size_t)n + 16);
+    uint32_t ns = 0, x = 65536;
+    for (uint32_t i = n; i-- > 0;) {
+        const uint8_t sym = codes[i];
+        const int ctx = dist_ctx(i ? codes[i - 1] : 0);
+        const uint32_t f = przm_dist_freq[ctx][sym], c = cdf[ctx][sym];
+        while (x > 4096 * f - 1) { stream[ns++] = (uint8_t)x; x >>= 8; }
+        x = (x / f) * 4096 + c + (x % f);
+    }
+    out[0] = 0x0C;
+    wr32(out + 1, x);
+    for (uint32_t k = 0; k < ns; k++) out[5 + k] = stream[ns - 1 - k];
+    free(stream);
+    return 5 + ns;
+}
+
+static uint32_t varint_len(uint32_t d) { /* distance.go:760-775 */
+    return d <= 0x7F ? 1 : d <= 0x3FFF ? 2 : d <= 0x1FFFFF ? 3 : d <= 0x0FFFFFFF ? 4 : 5;
+}
+
+/* PackDistancePrefix with the mode chosen automatically and the coded code stream
+ * (distance.go:692-748), SelectPrefixMode (750-758), assemblePrefixContainer (777-823).
+ * d[i] in [1, 2^24].  out holds 13 + 5n + 700 bytes.  info (optional): [0] 0 Mode-12 /
+ * 1 adaptive V-ANS code stream, [1] code stream bytes, [2] extra-bit bytes, [3] mode byte. */
+uint32_t orc_pack_distance_prefix(const uint32_t *d, uint32_t n, uint8_t *out, uint32_t info[4]) {
+    if (!n) return 0;
+    uint8_t *codes = (uint8_t *)malloc((size_t)n + 16);
+    uint8_t *extra = (uint8_t *)malloc(3 * (size_t)n + 16);
+    uint8_t *cs = (uint8_t *)malloc(2 * (size_t)n + 700);
+    uint8_t *alt = (uint8_t *)malloc(2 * (size_t)n + 700);
+    uint32_t ne = 0, orig = 0, cnt = 0;
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < n; i++) { /* BitWriter, MSB first (distance.go:217-247) */
+        uint8_t c, nb;
+        uint32_t e;
+        orc_dist_code(d[i], &c, &e, &nb);
+        codes[i] = c;
+        if (nb) {
+            acc = (acc << nb) | ((uint64_t)e & ((1ull << nb) - 1));
+            cnt += nb;
+            while (cnt >= 8) { cnt -= 8; extra[ne++] = (uint8_t)(acc >> cnt); }
+        }
+        orig += varint_len(d[i]);
+    }
+    if (cnt) extra[ne++] = (uint8_t)(acc << (8 - cnt));
+    uint32_t cl = orc_dist_m12_encode(codes, n, cs), kind = 0;
+    if (n >= 64) {
+        const uint32_t al = orc_vans4x_adaptive(codes, n, alt);
+        if (al < cl) { cl = al; memcpy(cs, alt, al); kind = 1; }
+    }
+    uint32_t h;
+    if (orig <= 255 && n <= 255 && cl <= 255) {
+        out[0] = 0x0B; out[1] = (uint8_t)orig; out[2] = (uint8_t)n; out[3] = (uint8_t)cl; h = 4;
+    } else if (orig <= 65535 && n <= 65535 && cl <= 65535) {
+        out[0] = 0x0A;
+        out[1] = (uint8_t)orig; out[2] = (uint8_t)(orig >> 8);
+        out[3] = (uint8_t)n; out[4] = (uint8_t)(n >> 8);
+        out[5] = (uint8_t)cl; out[6] = (uint8_t)(cl >> 8);
+        h = 7;
+    } else {
+        out[0] = 0x06; wr32(out + 1, orig); wr32(out + 5, n); wr32(out + 9, cl); h = 13;
+    }
+    memcpy(out + h, cs, cl);
+    memcpy(out + h + cl, extra, ne);
+    if (info) { info[0] = kind; info[1] = cl; info[2] = ne; info[3] = out[0]; }
+    free(codes); free(extra); free(cs); free(alt);
+    return h + cl + ne;
+}
+
+/* PackageOffsetsSection (distance.go:825-849) for a block's LEB128 offset stream off[0, noff):
+ * the prefix container when it is shorter than the varints, else 0x09 + the varints.
+ * Mode-G rule added on top of the reference: a container whose code stream is exactly as
+ * long as the match count is written flat too, since both reference decoders take such a
+ * code stream for raw codes (distance.go:601-604, src/przm_dec.c:1097-1098).
+ * out holds 16 + 3 noff + 700.  Returns the section length (0 for no offsets); *mode gets
+ * the first byte, *code_kind 0 Mode-12 / 1 V-ANS (or -1 for a flat section). */
+uint32_t orc_package_offsets(const uint8_t *off, uint32_t noff, uint8_t *out, int *mode, int *code_kind) {
+    if (mode) *mode = 0;
+    if (code_kind) *code_kind = -1;
+    if (!noff) return 0;
+    uint32_t *d = (uint32_t *)malloc(((size_t)noff + 1) * sizeof(uint32_t));
+    uint32_t n = 0, raw = 0, v = 0, sh = 0;
+    for (uint32_t i = 0; i < noff; i++) {
+        v |= (uint32_t)(off[i] & 0x7F) << sh;
+        sh += 7;
+        if (!(off[i] & 0x80)) { d[n++] = v; raw += varint_len(v); v = 0; sh = 0; }
+    }
+    uint32_t info[4] = {0, 0, 0, 0};
+    const uint32_t pl = orc_pack_distance_prefix(d, n, out, info);
+    free(d);
+    if (pl < raw && info[1] != n) {
+        if (mode) *mode = out[0];
+        if (code_kind) *code_kind = (int)info[0];
+        return pl;
+    }
+    out[0] = 0x09;
+    memcpy(out + 1, off, noff);
+    if (mode) *mode = 0x09;
+    return 1 + noff;
+}
+
 /* compress.go:113-132 */
 int orc_check_high_entropy(const uint8_t *src, uint32_t n) {
     uint32_t sample = n < 512 ? n : 512;
@@ -510,16 +644,20 @@ typedef struct {
     uint32_t s2, s3, s4;
     int32_t sub_choice[3];
     uint32_t s4_vans;
+    uint32_t s3_mode;     /* first byte of Section 3: 0x09 flat, 0x06 / 0x0A / 0x0B prefix; 0 none */
+    int32_t s3_code;      /* prefix code stream: 0 Mode-12, 1 adaptive V-ANS; -1 flat or none */
 } orc_frame_info;
 
 /*
  * One block (n <= 65536) -> one frame.  out holds 8 + n bytes.
  * compress.go:962-1033 with the mode-G sections described above.
  */
-uint32_t orc_encode_block(const uint8_t *src, uint32_t n, int level, uint8_t *out, orc_frame_info *info) {
+uint32_t orc_encode_block_s3(const uint8_t *src, uint32_t n, int level, int s3_prefix, uint8_t *out,
+                             orc_frame_info *info) {
     orc_frame_info tmp_info;
     if (!info) info = &tmp_info;
     memset(info, 0, sizeof(*info));
+    info->s3_code = -1;
     if (n == 0 || orc_check_high_entropy(src, n)) {
         info->kind = n ? 1 : 2;
         return raw_frame(src, n, out);
@@ -538,7 +676,19 @@ uint32_t orc_encode_block(const uint8_t *src, uint32_t n, int level, uint8_t *ou
 
     uint8_t *s2 = (uint8_t *)malloc(16 + 2 * (size_t)s.ncmd + 3 * 640);
     uint32_t l2 = orc_package_commands(&s, s2, info->sub_choice);
-    uint32_t l3 = s.noff ? 1 + s.noff : 0;
+    /* Section 3: flat 0x09, or with s3_prefix PackageOffsetsSection's choice */
+    uint8_t *s3 = (uint8_t *)malloc(16 + 3 * (size_t)s.noff + 700);
+    uint32_t l3;
+    if (s3_prefix) {
+        int mode, ck;
+        l3 = orc_package_offsets(s.off, s.noff, s3, &mode, &ck);
+        info->s3_mode = (uint32_t)mode;
+        info->s3_code = ck;
+    } else {
+        l3 = s.noff ? 1 + s.noff : 0;
+        if (l3) { s3[0] = 0x09; memcpy(s3 + 1, s.off, s.noff); }
+        info->s3_mode = l3 ? 0x09 : 0;
+    }
     uint8_t *s4 = (uint8_t *)malloc(8 + 600 + 16 + 2 * (size_t)s.nlit);
     uint32_t l4 = 0;
     int vans = 0;
@@ -574,20 +724,24 @@ uint32_t orc_encode_block(const uint8_t *src, uint32_t n, int level, uint8_t *ou
         out[p + 1] = (uint8_t)orc_window_log(n);
         p += 8;
         memcpy(out + p, s2, l2); p += l2;
-        if (l3) { out[p] = 0x09; memcpy(out + p + 1, s.off, s.noff); p += l3; }
+        memcpy(out + p, s3, l3); p += l3;
         memcpy(out + p, s4, l4); p += l4;
         len = p;
     }
-    free(s2); free(s4); free(mem);
+    free(s2); free(s3); free(s4); free(mem);
     return len;
+}
+
+uint32_t orc_encode_block(const uint8_t *src, uint32_t n, int level, uint8_t *out, orc_frame_info *info) {
+    return orc_encode_block_s3(src, n, level, 0, out, info);
 }
 
 /* A buffer -> one frame per 64 KiB block (an empty buffer -> one empty
  * uncompressed frame, as compress.go:1036-1114 produces for u == 0).
  * frame_sizes gets one entry per frame; returns total bytes, or 0 when
  * out_cap / frames_cap is too small. */
-uint64_t orc_encode_buffer(const uint8_t *src, uint64_t n, int level, uint8_t *out, uint64_t out_cap,
-                           uint32_t *frame_sizes, uint64_t frames_cap, uint64_t *nframes) {
+uint64_t orc_encode_buffer_s3(const uint8_t *src, uint64_t n, int level, int s3_prefix, uint8_t *out,
+                              uint64_t out_cap, uint32_t *frame_sizes, uint64_t frames_cap, uint64_t *nframes) {
     uint64_t nb = n ? (n + ORC_BLOCK - 1) / ORC_BLOCK : 1;
     if (nb > frames_cap) return 0;
     uint64_t total = 0;
@@ -596,7 +750,7 @@ uint64_t orc_encode_buffer(const uint8_t *src, uint64_t n, int level, uint8_t *o
         uint64_t off = b * ORC_BLOCK;
         uint32_t len = (uint32_t)(n - off < ORC_BLOCK ? n - off : ORC_BLOCK);
         if (!n) len = 0;
-        uint32_t fl = orc_encode_block(src + off, len, level, tmp, NULL);
+        uint32_t fl = orc_encode_block_s3(src + off, len, level, s3_prefix, tmp, NULL);
         if (total + fl > out_cap) { free(tmp); return 0; }
         memcpy(out + total, tmp, fl);
         frame_sizes[b] = fl;
@@ -605,6 +759,11 @@ uint64_t orc_encode_buffer(const uint8_t *src, uint64_t n, int level, uint8_t *o
     free(tmp);
     *nframes = nb;
     return total;
+}
+
+uint64_t orc_encode_buffer(const uint8_t *src, uint64_t n, int level, uint8_t *out, uint64_t out_cap,
+                           uint32_t *frame_sizes, uint64_t frames_cap, uint64_t *nframes) {
+    return orc_encode_buffer_s3(src, n, level, 0, out, out_cap, frame_sizes, frames_cap, nframes);
 }
 
 /* Stream-level entry point for tests: tokenize one block into caller

@@ -17,7 +17,7 @@ U64 = ctypes.c_uint64
 class FrameInfo(ctypes.Structure):
     _fields_ = [("kind", U32), ("ncmd", U32), ("noff", U32), ("nlit", U32), ("nop", U32), ("nmeta", U32),
                 ("nvar", U32), ("ntok", U32), ("s2", U32), ("s3", U32), ("s4", U32), ("sub", ctypes.c_int32 * 3),
-                ("s4v", U32)]
+                ("s4v", U32), ("s3_mode", U32), ("s3_code", ctypes.c_int32)]
 
 
 def ptr(a):
@@ -67,6 +67,18 @@ class Oracle:
         l.orc_encode_buffer.restype = U64
         l.orc_package_cmd_stream.argtypes = [P, U32, P, P]
         l.orc_package_cmd_stream.restype = ctypes.c_int64
+        l.orc_encode_block_s3.argtypes = [P, U32, ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(FrameInfo)]
+        l.orc_encode_block_s3.restype = U32
+        l.orc_encode_buffer_s3.argtypes = [P, U64, ctypes.c_int, ctypes.c_int, P, U64, P, U64, ctypes.POINTER(U64)]
+        l.orc_encode_buffer_s3.restype = U64
+        l.orc_dist_code.argtypes = [U32, P, P, P]
+        l.orc_dist_code.restype = None
+        l.orc_dist_m12_encode.argtypes = [P, U32, P]
+        l.orc_dist_m12_encode.restype = U32
+        l.orc_pack_distance_prefix.argtypes = [P, U32, P, P]
+        l.orc_pack_distance_prefix.restype = U32
+        l.orc_package_offsets.argtypes = [P, U32, P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        l.orc_package_offsets.restype = U32
         self.l = l
 
     def hash4(self, b4):
@@ -95,6 +107,31 @@ class Oracle:
         out = {n: bufs[i][: lens[i]].copy() for i, n in enumerate(names)}
         out["ntok"] = int(lens[6])
         return out
+
+    def distance_to_code(self, d):
+        c, e, nb = ctypes.c_uint8(), ctypes.c_uint32(), ctypes.c_uint8()
+        r = self.l.przm_distance_to_code(d, ctypes.byref(c), ctypes.byref(e), ctypes.byref(nb))
+        if r != 0:
+            raise ValueError(f"distance_to_code error {r}")
+        return c.value, e.value, nb.value
+
+    def dist4_decode(self, stream, n):
+        s = u8(stream)
+        out = np.zeros(max(n, 1), np.uint8)
+        r = self.l.przm_vans_decode_dist4(ptr(s), s.size, ptr(out), out.size, n)
+        if r != 0:
+            raise ValueError(f"dist4 decode error {r}")
+        return out[:n].tobytes()
+
+    def prefix_decode(self, container, cap=1 << 20):
+        """A distance-prefix container (0x06 / 0x0A / 0x0B) back to its LEB128 varints."""
+        c = u8(container)
+        out = np.zeros(cap, np.uint8)
+        ln = ctypes.c_size_t()
+        r = self.l.przm_distance_prefix_decode(ptr(c), c.size, ptr(out), out.size, ctypes.byref(ln))
+        if r != 0:
+            raise ValueError(f"prefix decode error {r}")
+        return out[: ln.value].tobytes()
 
     def split(self, cmd):
         c = u8(cmd)
@@ -152,21 +189,52 @@ class Oracle:
             raise ValueError("command stream does not split")
         return out[:n].tobytes(), list(ch)
 
-    def encode_block(self, src, level=6):
+    def encode_block(self, src, level=6, s3_prefix=False):
+        """One frame; s3_prefix: Section 3 as PackageOffsetsSection chooses it."""
         a = u8(src)
         out = np.zeros(a.size + 64, np.uint8)
         info = FrameInfo()
-        n = self.l.orc_encode_block(ptr(a), a.size, level, ptr(out), ctypes.byref(info))
+        n = self.l.orc_encode_block_s3(ptr(a), a.size, level, int(s3_prefix), ptr(out), ctypes.byref(info))
         return out[:n].tobytes(), info
 
-    def encode_buffer(self, src, level=6):
+    def encode_buffer(self, src, level=6, s3_prefix=False):
         a = u8(src)
         nb = max((a.size + 65535) // 65536, 1)
         out = np.zeros(a.size + 8 * nb + 64, np.uint8)
         sizes = np.zeros(nb, np.uint32)
         nf = U64()
-        tot = self.l.orc_encode_buffer(ptr(a), a.size, level, ptr(out), out.size, ptr(sizes), nb, ctypes.byref(nf))
+        tot = self.l.orc_encode_buffer_s3(ptr(a), a.size, level, int(s3_prefix), ptr(out), out.size, ptr(sizes), nb,
+                                          ctypes.byref(nf))
         return out[:tot].tobytes(), sizes[: nf.value].copy()
+
+    def dist_code(self, d):
+        """(code, extra, extra bit count) of distance d (DistanceToCode)."""
+        c, e, nb = ctypes.c_uint8(), ctypes.c_uint32(), ctypes.c_uint8()
+        self.l.orc_dist_code(d, ctypes.byref(c), ctypes.byref(e), ctypes.byref(nb))
+        return c.value, e.value, nb.value
+
+    def dist_m12(self, codes):
+        """The Mode-12 code stream [0x0C][u32 state][bytes] of distance codes."""
+        c = u8(codes)
+        out = np.zeros(2 * c.size + 16, np.uint8)
+        n = self.l.orc_dist_m12_encode(ptr(c), c.size, ptr(out))
+        return out[:n].tobytes()
+
+    def pack_distance_prefix(self, dists):
+        """PackDistancePrefix(auto mode, coded codes): (container, [code kind, code len, extra len, mode])."""
+        d = np.ascontiguousarray(dists, dtype=np.uint32)
+        out = np.zeros(13 + 5 * d.size + 700, np.uint8)
+        info = np.zeros(4, np.uint32)
+        n = self.l.orc_pack_distance_prefix(ptr(d), d.size, ptr(out), ptr(info))
+        return out[:n].tobytes(), [int(x) for x in info]
+
+    def package_offsets(self, off):
+        """Section 3 of a LEB128 offset stream (PackageOffsetsSection): (section, mode byte, code kind)."""
+        o = u8(off)
+        out = np.zeros(16 + 3 * o.size + 700, np.uint8)
+        mode, ck = ctypes.c_int(), ctypes.c_int()
+        n = self.l.orc_package_offsets(ptr(o), o.size, ptr(out), ctypes.byref(mode), ctypes.byref(ck))
+        return out[:n].tobytes(), mode.value, ck.value
 
 
 class RefDecoder:
@@ -180,6 +248,12 @@ class RefDecoder:
         l.przm_vans_decode_literals.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t]
         l.przm_vans_decode_meta8.restype = ctypes.c_int32
         l.przm_vans_decode_meta8.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, ctypes.c_size_t]
+        l.przm_distance_to_code.restype = ctypes.c_int32
+        l.przm_distance_to_code.argtypes = [U32, P, P, P]
+        l.przm_vans_decode_dist4.restype = ctypes.c_int32
+        l.przm_vans_decode_dist4.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, ctypes.c_size_t]
+        l.przm_distance_prefix_decode.restype = ctypes.c_int32
+        l.przm_distance_prefix_decode.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         l.przm_split_commands.restype = ctypes.c_int32
         l.przm_split_commands.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
                                           P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P, ctypes.c_size_t,
@@ -220,6 +294,31 @@ class RefDecoder:
         if r != 0:
             raise ValueError(f"meta8 decode error {r}")
         return out[:n].tobytes()
+
+    def distance_to_code(self, d):
+        c, e, nb = ctypes.c_uint8(), ctypes.c_uint32(), ctypes.c_uint8()
+        r = self.l.przm_distance_to_code(d, ctypes.byref(c), ctypes.byref(e), ctypes.byref(nb))
+        if r != 0:
+            raise ValueError(f"distance_to_code error {r}")
+        return c.value, e.value, nb.value
+
+    def dist4_decode(self, stream, n):
+        s = u8(stream)
+        out = np.zeros(max(n, 1), np.uint8)
+        r = self.l.przm_vans_decode_dist4(ptr(s), s.size, ptr(out), out.size, n)
+        if r != 0:
+            raise ValueError(f"dist4 decode error {r}")
+        return out[:n].tobytes()
+
+    def prefix_decode(self, container, cap=1 << 20):
+        """A distance-prefix container (0x06 / 0x0A / 0x0B) back to its LEB128 varints."""
+        c = u8(container)
+        out = np.zeros(cap, np.uint8)
+        ln = ctypes.c_size_t()
+        r = self.l.przm_distance_prefix_decode(ptr(c), c.size, ptr(out), out.size, ctypes.byref(ln))
+        if r != 0:
+            raise ValueError(f"prefix decode error {r}")
+        return out[: ln.value].tobytes()
 
     def split(self, cmd):
         c = u8(cmd)

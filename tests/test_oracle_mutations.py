@@ -4,7 +4,7 @@ Every case below plants one plausible slip in a copy of oracle/przm_oracle.c
 (a threshold off by one, a cap moved, a comparison made non-strict, two
 candidates tried in the other order, a register update dropped), compiles it
 with gcc into a temporary directory and runs the known-answer checks of
-tests/test_oracle_kats.py against it.  A mutant that passes every check means
+tests/test_oracle_kats.py and tests/offsets_cases.py against it.  A mutant that passes every check means
 the checks do not pin that rule.  CPU only; the real oracle is not touched.
 
 One written rule cannot change an output and has no mutant here: the run
@@ -19,7 +19,8 @@ import subprocess
 import numpy as np
 import pytest
 
-from oracle_lib import Oracle
+from oracle_lib import Oracle, have_refdec, refdec
+from offsets_cases import offsets_failures
 from tokenizer_kats import KATS, run_kat
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -63,6 +64,24 @@ MUTANTS = [
     ("choice_meta8_non_strict", "if (l < best) { best = l; ch = 1;", "if (l <= best) { best = l; ch = 1;"),
     ("mode8_keep_non_strict", "if (pos < s->ncmd) {", "if (pos <= s->ncmd) {"),
     ("hazard_dropped", "int hazard = !vans && l4 >= 8 &&", "int hazard = 0 && l4 >= 8 &&"),
+    # Section 3 as a distance-prefix container
+    ("dist_code_half_bit", "const uint32_t half = (u >> b) & 1u;", "const uint32_t half = (u >> (b + 1)) & 1u;"),
+    ("m12_context_class_6", "return prev < 6 ? 0 : prev < 14 ? 1 : prev < 26 ? 2 : 3;",
+     "return prev <= 6 ? 0 : prev < 14 ? 1 : prev < 26 ? 2 : 3;"),
+    ("m12_first_context", "const int ctx = dist_ctx(i ? codes[i - 1] : 0);", "const int ctx = dist_ctx(i ? codes[i - 1] : 26);"),
+    ("m12_renorm_early", "while (x > 4096 * f - 1) { stream[ns++] = (uint8_t)x; x >>= 8; }",
+     "while (x >= 4096 * f - 1) { stream[ns++] = (uint8_t)x; x >>= 8; }"),
+    ("m12_initial_state", "uint32_t ns = 0, x = 65536;", "uint32_t ns = 0, x = 65537;"),
+    ("m12_bytes_not_reversed", "out[5 + k] = stream[ns - 1 - k];", "out[5 + k] = stream[k];"),
+    ("bit_flush_dropped", "if (cnt) extra[ne++] = (uint8_t)(acc << (8 - cnt));", ""),
+    ("vans_codes_from_65", "if (n >= 64) {\n        const uint32_t al", "if (n > 64) {\n        const uint32_t al"),
+    ("vans_codes_non_strict", "if (al < cl) {", "if (al <= cl) {"),
+    ("byte_header_varints_254", "if (orig <= 255 && n <= 255 && cl <= 255) {", "if (orig < 255 && n <= 255 && cl <= 255) {"),
+    ("byte_header_count_254", "if (orig <= 255 && n <= 255 && cl <= 255) {", "if (orig <= 255 && n < 255 && cl <= 255) {"),
+    ("compact_header_varints", "} else if (orig <= 65535 && n <= 65535 && cl <= 65535) {",
+     "} else if (orig < 65535 && n <= 65535 && cl <= 65535) {"),
+    ("prefix_non_strict", "if (pl < raw && info[1] != n) {", "if (pl <= raw && info[1] != n) {"),
+    ("code_length_rule_dropped", "if (pl < raw && info[1] != n) {", "if (pl < raw) {"),
 ]
 
 
@@ -92,7 +111,7 @@ def decision_failures(O):
     a = np.concatenate([lit, np.frombuffer(b"q" * 5000, np.uint8)])
     if O.encode_block(a)[1].kind != 3:
         bad.append("hazard")
-    return bad
+    return bad + offsets_failures(O, refdec() if have_refdec() else None)
 
 
 def test_real_oracle_passes_every_check():
