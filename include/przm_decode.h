@@ -8,15 +8,17 @@
  *     (flag 0x40), Section 1 as the frame lays it out (8 + 2 * byte 5 bytes);
  *   - Section 2 as Mode 8 (op / meta / var substreams, each flat 0x09, length
  *     0x00 + u32, Meta8 or adaptive V-ANS), flat 0x09, length 0x00 + u32 or raw;
- *   - Section 3 as 0x09 + LEB128 offsets or 0x00 + u32 length;
+ *   - Section 3 as 0x09 + LEB128 offsets, 0x00 + u32 length, or a distance-prefix
+ *     container 0x06 / 0x0A / 0x0B whose code stream is raw, Mode 12 (0x0C), adaptive
+ *     V-ANS or flat (core/distance/distance.go:563-690);
  *   - Section 4 as 4-lane V-ANS (docs/PRZM_SPEC.md:582-647) or raw literals;
  *   - the opcodes 0x00-0x7F (fused repeat, registers 0..2), 0x80-0xBF (explicit
  *     offset), 0xC0-0xC2 (long repeat), 0xF0 (literal run), 0xF1 (byte run) and
  *     0xF3-0xFA (short literal runs), with the ring registers {1, 4, 8}
  *     (docs/PRZM_SPEC.md:468-526, decompress.go).
- * Anything else (stride, dictionaries, normalisation, delta masks, the
- * distance-prefix offset modes, sheaf and nibble-plane literal sections, the
- * other opcodes) is refused with PRZM_DEC_ERR_UNSUPPORTED rather than guessed.
+ * Anything else (stride, dictionaries, normalisation, delta masks, sheaf and
+ * nibble-plane literal sections, the other opcodes) is refused with
+ * PRZM_DEC_ERR_UNSUPPORTED rather than guessed.
  * Decoded bytes equal the reference decoders' on every frame it accepts.
  */
 #ifndef PRZM_DECODE_H

@@ -4,7 +4,7 @@
  * Written from the format description (docs/PRZM_SPEC.md:94-154 frame,
  * 457-647 commands, offsets and V-ANS; decompress.go:113-260 header and
  * section framing; core/commands/commands.go:205-258, 474-520, 755-789
- * Section 2; core/distance/distance.go:525-545 Section 3;
+ * Section 2; core/distance/distance.go:262-321, 525-690 Section 3;
  * core/literals/literals.go:696-736 Section 4; core/commands/vans_meta.go:45-68
  * Meta8 contexts) for the features listed in the header.  Every read is
  * bounds-checked; a frame either decodes completely or returns an error.
@@ -14,6 +14,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "przm_dist_freq.h"
 #include "przm_meta8_freq.h"
 
 #define MAGIC 0x4D5A5250u    /* "PRZM" little-endian */
@@ -275,8 +276,126 @@ static int section2(const uint8_t *s2, size_t n, uint8_t **buf, const uint8_t **
     return 0;
 }
 
-/* Section 3 -> the LEB128 offset stream (distance.go:525-545 for the modes read here) */
-static int section3(const uint8_t *s3, size_t n, const uint8_t **out, size_t *on) {
+/* distance codes coded by one rANS state whose table is chosen by the class of the previous
+ * code ([0x0C][u32 state][stream], distance.go:287-321); like the V-ANS containers, the
+ * stream must end on the encoder's initial state with every byte used */
+static int dist_codes_m12(const uint8_t *s, size_t sn, uint8_t *dst, uint32_t n) {
+    if (sn < 5 || s[0] != 0x0C) return PRZM_DEC_ERR_CORRUPT;
+    uint16_t cdf[4][64];
+    uint8_t *slot = (uint8_t *)malloc(4 * 4096);
+    if (!slot) return PRZM_DEC_ERR_CAPACITY;
+    for (int c = 0; c < 4; c++) {
+        uint32_t cum = 0;
+        for (int k = 0; k < 64; k++) {
+            cdf[c][k] = (uint16_t)cum;
+            memset(slot + 4096 * c + cum, k, przm_dist_freq[c][k]);
+            cum += przm_dist_freq[c][k];
+        }
+    }
+    uint32_t x = le32(s + 1);
+    size_t sp = 5;
+    uint8_t prev = 0;
+    int rc = x < 65536u ? PRZM_DEC_ERR_CORRUPT : 0;
+    for (uint32_t i = 0; i < n && !rc; i++) {
+        const int c = prev < 6 ? 0 : prev < 14 ? 1 : prev < 26 ? 2 : 3;
+        const uint32_t sl = x & 4095u;
+        const uint8_t sym = slot[4096 * c + sl];
+        dst[i] = sym;
+        prev = sym;
+        x = (uint32_t)przm_dist_freq[c][sym] * (x >> 12) + sl - cdf[c][sym];
+        while (x < 65536u && !rc) {
+            if (sp >= sn) rc = PRZM_DEC_ERR_CORRUPT;
+            else x = (x << 8) | s[sp++];
+        }
+    }
+    free(slot);
+    if (!rc && (sp != sn || x != 65536u)) rc = PRZM_DEC_ERR_CORRUPT;
+    return rc;
+}
+
+/* a distance-prefix container (0x06 / 0x0A / 0x0B: distance.go:563-690, src/przm_dec.c:1057-1137)
+ * -> its LEB128 varints in a new buffer *buf */
+static int prefix_offsets(const uint8_t *s3, size_t n, uint8_t **buf, size_t *on) {
+    uint32_t orig, count, cl;
+    size_t h;
+    if (s3[0] == 0x06) {
+        if (n < 13) return PRZM_DEC_ERR_CORRUPT;
+        orig = le32(s3 + 1); count = le32(s3 + 5); cl = le32(s3 + 9); h = 13;
+    } else if (s3[0] == 0x0A) {
+        if (n < 7) return PRZM_DEC_ERR_CORRUPT;
+        orig = (uint32_t)s3[1] | (uint32_t)s3[2] << 8;
+        count = (uint32_t)s3[3] | (uint32_t)s3[4] << 8;
+        cl = (uint32_t)s3[5] | (uint32_t)s3[6] << 8;
+        h = 7;
+    } else {
+        if (n < 4) return PRZM_DEC_ERR_CORRUPT;
+        orig = s3[1]; count = s3[2]; cl = s3[3]; h = 4;
+    }
+    if (h + (size_t)cl > n || orig > MAX_FRAME || count > orig) return PRZM_DEC_ERR_CORRUPT;
+    if (count == 0) return orig ? PRZM_DEC_ERR_CORRUPT : 0;
+    const uint8_t *cs = s3 + h, *ex = s3 + h + cl;
+    const size_t en = n - h - cl;
+    uint8_t *codes = (uint8_t *)malloc((size_t)count + 16);
+    uint8_t *out = (uint8_t *)malloc((size_t)orig + 16);
+    int rc = codes && out ? 0 : PRZM_DEC_ERR_CAPACITY;
+    /* the code stream (distance.go:600-631): raw when exactly count bytes, else Mode 12, an
+     * adaptive V-ANS container or a flat stream */
+    if (rc) {
+    } else if (cl == count) {
+        memcpy(codes, cs, count);
+    } else if (cl > 0 && cs[0] == 0x0C) {
+        rc = dist_codes_m12(cs, cl, codes, count);
+    } else if (cl >= 8 && cs[0] == 'V' && cs[1] == 'A' && cs[2] == 0x01) {
+        if (cs[3] != 0) rc = PRZM_DEC_ERR_UNSUPPORTED;
+        else if (le32(cs + 4) != count) rc = PRZM_DEC_ERR_CORRUPT;
+        else {
+            const int64_t r = vans_container(cs, cl, codes, count, 0);
+            rc = r < 0 ? (int)r : 0;
+        }
+    } else if (cl > 0 && cs[0] == 0x09) {
+        if (cl - 1 < count) rc = PRZM_DEC_ERR_CORRUPT;
+        else memcpy(codes, cs + 1, count);
+    } else if (cl >= 5 && cs[0] == 0x00) {
+        const uint32_t l = le32(cs + 1);
+        if (l < count || 5 + (size_t)l > cl) rc = PRZM_DEC_ERR_CORRUPT;
+        else memcpy(codes, cs + 5, count);
+    } else {
+        rc = PRZM_DEC_ERR_CORRUPT;
+    }
+    /* extra bits, most significant first (distance.go:262-285), and the varints */
+    size_t ep = 0, op = 0;
+    uint64_t acc = 0;
+    uint32_t bits = 0;
+    for (uint32_t i = 0; i < count && !rc; i++) {
+        const uint32_t c = codes[i];
+        if (c > 48) { rc = PRZM_DEC_ERR_CORRUPT; break; }
+        const uint32_t nb = c < 4 ? 0 : c == 48 ? 24 : (c - 2) >> 1;
+        const uint32_t base = c < 4 ? c + 1 : c == 48 ? 16777217u : ((2u + (c & 1u)) << nb) + 1u;
+        while (bits < nb) {
+            if (ep >= en) { rc = PRZM_DEC_ERR_CORRUPT; break; }
+            acc = acc << 8 | ex[ep++];
+            bits += 8;
+        }
+        if (rc) break;
+        bits -= nb;
+        uint32_t d = base + (nb ? (uint32_t)(acc >> bits) & ((1u << nb) - 1u) : 0u);
+        if (d > MAX_FRAME) { rc = PRZM_DEC_ERR_RANGE; break; }
+        do {
+            if (op >= orig) { rc = PRZM_DEC_ERR_CORRUPT; break; }
+            out[op++] = (uint8_t)(d >= 0x80 ? (d & 0x7Fu) | 0x80u : d);
+            d >>= 7;
+        } while (d);
+    }
+    if (!rc && op != orig) rc = PRZM_DEC_ERR_CORRUPT;
+    free(codes);
+    if (rc) { free(out); return rc; }
+    *buf = out;
+    *on = orig;
+    return 0;
+}
+
+/* Section 3 -> the LEB128 offset stream (distance.go:525-545): *out points into s3 or *buf */
+static int section3(const uint8_t *s3, size_t n, uint8_t **buf, const uint8_t **out, size_t *on) {
     *out = NULL;
     *on = 0;
     if (n == 0) return 0;
@@ -289,8 +408,12 @@ static int section3(const uint8_t *s3, size_t n, const uint8_t **out, size_t *on
         *on = l;
         return 0;
     }
-    /* the distance-prefix modes 0x06 / 0x0A / 0x0B are not read here; any other mode byte is
-     * ErrUnsupportedMode in the reference too */
+    if (s3[0] == 0x06 || s3[0] == 0x0A || s3[0] == 0x0B) {
+        const int rc = prefix_offsets(s3, n, buf, on);
+        *out = *buf;
+        return rc;
+    }
+    /* any other mode byte is ErrUnsupportedMode in the reference too */
     return PRZM_DEC_ERR_UNSUPPORTED;
 }
 
@@ -428,14 +551,15 @@ int64_t przm_decode_frame(const uint8_t *src, size_t n, uint8_t *dst, size_t cap
     const size_t l1 = p + 8 <= n ? 8 + 2 * (size_t)src[p + 5] : 0;
     const size_t o2 = p + l1, o3 = o2 + lc, o4 = o3 + lo;
     if (o4 > n) return PRZM_DEC_ERR_CORRUPT;
-    uint8_t *b2 = NULL, *b4 = NULL;
+    uint8_t *b2 = NULL, *b3 = NULL, *b4 = NULL;
     const uint8_t *cmd, *off, *lit;
     size_t nc, nf, nl;
     int64_t rc = section2(src + o2, lc, &b2, &cmd, &nc);
-    if (!rc) rc = section3(src + o3, lo, &off, &nf);
+    if (!rc) rc = section3(src + o3, lo, &b3, &off, &nf);
     if (!rc) rc = section4(src + o4, n - o4, &b4, &lit, &nl);
     if (!rc) rc = execute(cmd, nc, off, nf, lit, nl, (flags & F_MIN3) ? 3u : 4u, dst, u);
     free(b2);
+    free(b3);
     free(b4);
     return rc;
 }

@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from oracle_lib import oracle, refdec, have_refdec
+from offsets_cases import copy_block
 from tokenizer_kats import KATS
 
 from przm_amd import datagen as D
@@ -44,6 +45,33 @@ def test_roundtrip_oracle_frames():
         assert X.DecompressFrames(frames, sizes) == a.tobytes(), name
 
 
+def _prefix_blocks():
+    """Blocks whose Section 3 is a distance-prefix container or hits one of its edge rules."""
+    T = json.load(open(os.path.join(HERE, "golden", "offsets_ties.json")))
+    for key in ("hazard_block", "renorm_block"):
+        yield key, copy_block(T[key]["distances"], T[key]["seed"])
+
+
+def test_roundtrip_prefix_offset_frames():
+    """Frames whose Section 3 the oracle writes as PackageOffsetsSection chooses it."""
+    O = oracle()
+    used = 0
+    for name, a in list(_inputs()) + list(_prefix_blocks()):
+        frames, sizes = O.encode_buffer(a, 6, True)
+        assert X.DecompressFrames(frames, sizes) == a.tobytes(), name
+        f = np.frombuffer(frames, np.uint8)
+        pos = 0
+        for sz in sizes:
+            fr = f[pos: pos + int(sz)]
+            pos += int(sz)
+            if fr.size > 12 and fr[4] & 0x04 == 0:
+                d = 12 + (2 if fr[4] & 0x08 else 0)
+                lc = int.from_bytes(fr[8:11].tobytes() if fr[4] & 0x08 else fr[8:10].tobytes(), "little")
+                s3 = d + 8 + lc
+                used += int(fr[s3] in (0x06, 0x0A, 0x0B) and lc < fr.size - s3)
+    assert used > 10
+
+
 @pytest.mark.parametrize("level", [1, 3, 5, 9, 11])
 def test_roundtrip_levels(level):
     a = D.mixture(1 << 20, 82 + level)
@@ -60,11 +88,12 @@ def test_roundtrip_golden_fixtures():
 
 
 @needs_ref
-def test_equals_reference_decoder():
+@pytest.mark.parametrize("prefix", [False, True])
+def test_equals_reference_decoder(prefix):
     """Frame by frame, the same bytes as the reference C99 decoder."""
     O, R = oracle(), refdec()
-    for name, a in _inputs():
-        frames, sizes = O.encode_buffer(a)
+    for name, a in list(_inputs()) + list(_prefix_blocks()):
+        frames, sizes = O.encode_buffer(a, 6, prefix)
         f = np.frombuffer(frames, np.uint8)
         pos = 0
         for s in sizes:
@@ -131,8 +160,23 @@ def test_errors():
     assert _code(_frame(8, b"\x43", b"", b"abcd")) == X.ERR_UNSUPPORTED
     assert _code(_frame(4, b"\xF6", b"", b"abcd", flags=0x01)) == X.ERR_UNSUPPORTED
     assert _code(_frame(4, b"\xF6", b"", b"abcd", flags=0x10)) == X.ERR_UNSUPPORTED
-    # a distance-prefix offset section
-    assert _code(_frame(8, b"\xF6\x80", b"\x04", b"abcd", s3_mode=b"\x06")) == X.ERR_UNSUPPORTED
+    # an offset section mode the format does not define, and a short prefix container
+    assert _code(_frame(8, b"\xF6\x80", b"\x04", b"abcd", s3_mode=b"\x0C")) == X.ERR_UNSUPPORTED
+    assert _code(_frame(8, b"\xF6\x80", b"\x04", b"abcd", s3_mode=b"\x06")) == X.ERR_CORRUPT
+    # prefix containers (0x0B: varint bytes, count, code bytes) holding the one distance 4
+    # (code 3, no extra bits): raw codes, then a code above 48, a varint count that does not
+    # match, a Mode-12 stream with a byte left over, and extra bits that run out
+    ok = b"\x01\x01\x01\x03"
+    assert X.DecompressFrame(_frame(8, b"\xF6\x80", ok, b"abcd", s3_mode=b"\x0B")) == b"abcdabcd"
+    assert _code(_frame(8, b"\xF6\x80", b"\x01\x01\x01\x31", b"abcd", s3_mode=b"\x0B")) == X.ERR_CORRUPT
+    assert _code(_frame(8, b"\xF6\x80", b"\x02\x01\x01\x03", b"abcd", s3_mode=b"\x0B")) == X.ERR_CORRUPT
+    m12 = oracle().dist_m12(bytes([3]))
+    assert X.DecompressFrame(_frame(8, b"\xF6\x80", bytes([1, 1, len(m12)]) + m12, b"abcd",
+                                    s3_mode=b"\x0B")) == b"abcdabcd"
+    assert _code(_frame(8, b"\xF6\x80", bytes([1, 1, len(m12) + 1]) + m12 + b"\x00", b"abcd",
+                        s3_mode=b"\x0B")) == X.ERR_CORRUPT
+    # distance 1000 = code 19 + 231 in 8 extra bits, with the extra byte missing
+    assert _code(_frame(8, b"\xF6\x80", b"\x02\x01\x01\x13", b"abcd", s3_mode=b"\x0B")) == X.ERR_CORRUPT
     # a V-ANS literal section whose weights sum to 4095
     vans = bytearray(oracle().vans(np.frombuffer(b"abcd" * 10, np.uint8)))
     assert vans[8:10] == b"\x40\x00" and vans[10:12] == b"\x1E\x00"  # row 6, columns 1-4
@@ -148,8 +192,12 @@ def test_corrupted_frames_fail_cleanly():
     O = oracle()
     rng = np.random.default_rng(83)
     frames = []
-    for a in (D.mixture(65536, 84), D.markov(30000, 85), np.frombuffer(b"abcabcabd" * 2000, np.uint8).copy()):
-        f, s = O.encode_buffer(a)
+    T = json.load(open(os.path.join(HERE, "golden", "offsets_ties.json")))
+    rb = copy_block(T["renorm_block"]["distances"], T["renorm_block"]["seed"])
+    for a, pre in ((D.mixture(65536, 84), False), (D.markov(30000, 85), False),
+                   (np.frombuffer(b"abcabcabd" * 2000, np.uint8).copy(), False), (D.mixture(65536, 86), True),
+                   (rb, True)):
+        f, s = O.encode_buffer(a, 6, pre)
         frames.append((np.frombuffer(f, np.uint8), a))
     for f, a in frames:
         for cut in range(0, f.size, max(1, f.size // 300)):
