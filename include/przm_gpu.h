@@ -257,6 +257,23 @@ int32_t przm_gpu_get_tuning(przm_gpu *g, int32_t tuned[4]);
 /* use_graph = 0 runs the batched form as direct launches (per-stage profiling). */
 int32_t przm_gpu_set_options(przm_gpu *g, int32_t use_graph);
 
+/* ---- Section 3 of the frames ---- */
+#define PRZM_OFFSETS_FLAT 0   /* 0x09 + the LEB128 varints: the reference GPU path's frames */
+#define PRZM_OFFSETS_PREFIX 1 /* the distance-prefix container when shorter */
+/* How every frame this handle writes from now on codes its match distances.
+ * PRZM_OFFSETS_PREFIX writes Section 3 as the reference CPU path does
+ * (packageSections, compress.go:956 -> PackageOffsetsSection,
+ * core/distance/distance.go:825-849): distance codes and MSB-first extra bits,
+ * the codes as a Mode-12 stream (distance.go:323-376) or, from 64 distances on
+ * and when strictly shorter, an adaptive V-ANS container, in the smallest
+ * header mode (0x0B / 0x0A / 0x06); taken only when strictly shorter than the
+ * varints, and not when the code stream is exactly as long as the match count
+ * (both reference decoders would read it as raw codes).  Every decoder of the
+ * format reads both.  The option adds about 104 KB of device workspace per
+ * block of a launch.  Set it while no call or server is running on the handle;
+ * przm_gpu_encode_device_noalloc supports it within its fixed limit. */
+int32_t przm_gpu_set_offsets_mode(przm_gpu *g, int32_t mode);
+
 /* ---- device memory for callers without an allocator ---- */
 int32_t przm_gpu_malloc(przm_gpu *g, uint64_t bytes, void **d);
 int32_t przm_gpu_free(przm_gpu *g, void *d);

@@ -71,7 +71,9 @@ typedef struct {
     uint32_t choice[3];            /* substream coders chosen: 0 flat, 1 Meta8, 2 V-ANS; -1 flat section */
     uint32_t s_len[3];             /* S2, S3, S4 bytes */
     uint32_t s4_vans;
-    uint32_t pad[3];
+    uint32_t s3_mode;              /* Section 3's first byte: 0x09 flat, 0x06 / 0x0A / 0x0B prefix, 0 none */
+    uint32_t s3_code;              /* prefix code stream: 0 Mode 12, 1 adaptive V-ANS */
+    uint32_t s3_code_len;          /* prefix code stream bytes (head included) */
 } przm_meta_t;
 
 /* Normalized frequency tables per V-ANS job (lit, op, meta, var). */
@@ -79,5 +81,37 @@ typedef struct {
     uint32_t fc[256];       /* freq | cdf << 16 */
     uint8_t mat[552];       /* serialized joint-nibble matrix */
 } przm_ftab_t;
+
+/*
+ * Section 3 as a distance-prefix container (an encoder option): per-block
+ * offsets workspace.  A block has at most 16384 explicit distances (each
+ * match covers >= 4 bytes), each below 65536, so at most 14 extra bits and
+ * 12 code bits (frequency >= 1 of 4096) each.  Code-stream slots are filled
+ * backward from their 16-byte aligned ends.
+ */
+typedef struct {
+    uint32_t n;             /* explicit distances (the match count) */
+    uint32_t orig;          /* their LEB128 bytes (the offset stream's length) */
+    uint32_t extra;         /* extra-bit bytes */
+    uint32_t m12_len;       /* Mode-12 stream bytes after its 5-byte head, or PRZM_ENC_OVERFLOW */
+    uint32_t m12_state;
+    uint32_t v_len;         /* V-ANS stream bytes of the codes (n >= 64), or PRZM_ENC_OVERFLOW */
+    uint32_t v_state[4];
+    uint32_t mat_len;       /* joint-nibble matrix bytes of the codes' table */
+    uint32_t pad[5];
+} przm_offs_t;
+
+#define PRZM_OFFS_MAX 16384u
+#define PRZM_OWS_HDR 0u                                       /* przm_offs_t */
+#define PRZM_OWS_FTAB 64u                                     /* przm_ftab_t of the codes */
+#define PRZM_OWS_CODES 1664u                                  /* one code per distance */
+#define PRZM_OWS_CODES_CAP (PRZM_OFFS_MAX + 64u)
+#define PRZM_OWS_EXTRA (PRZM_OWS_CODES + PRZM_OWS_CODES_CAP)  /* extra bits, MSB first */
+#define PRZM_OWS_EXTRA_CAP (PRZM_OFFS_MAX * 14u / 8u + 64u)
+#define PRZM_OWS_M12 (PRZM_OWS_EXTRA + PRZM_OWS_EXTRA_CAP)    /* Mode-12 stream (slot end aligned) */
+#define PRZM_OWS_M12_CAP (PRZM_OFFS_MAX * 12u / 8u + 64u)
+#define PRZM_OWS_VANS (PRZM_OWS_M12 + PRZM_OWS_M12_CAP)       /* V-ANS stream of the codes */
+#define PRZM_OWS_VANS_CAP (2u * PRZM_OFFS_MAX + 64u)
+#define PRZM_OWS_BYTES (PRZM_OWS_VANS + PRZM_OWS_VANS_CAP)
 
 #endif

@@ -609,13 +609,23 @@ przm_rans1_kernel(const uint8_t *__restrict__ src, uint32_t n, const przm_ftab_t
 
 struct Plan {
     uint32_t kind, flags, l2, l3, l4, ext, sub_len[3], sub_ch[3], s4v;
+    uint32_t s3m, s3c, s3cl; /* Section 3: mode byte (0 none), prefix code stream kind and bytes */
 };
+
+/* SelectPrefixMode (distance.go:750-758): the container's mode byte and header bytes */
+__device__ __forceinline__ uint32_t prefix_mode(uint32_t orig, uint32_t n, uint32_t cl, uint32_t *h) {
+    if (orig <= 255u && n <= 255u && cl <= 255u) { *h = 4; return 0x0Bu; }
+    if (orig <= 65535u && n <= 65535u && cl <= 65535u) { *h = 7; return 0x0Au; }
+    *h = 13;
+    return 0x06u;
+}
 
 __device__ __forceinline__ uint32_t rd32g(const uint8_t *p) {
     return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
 }
 
-__device__ Plan plan_frame(const przm_block_t &bd, const przm_meta_t *mt, const uint8_t *wsb) {
+__device__ Plan plan_frame(const przm_block_t &bd, const przm_meta_t *mt, const uint8_t *wsb,
+                           const przm_offs_t *of) {
     Plan P = {};
     const uint32_t n = bd.len;
     if ((bd.flags & PRZM_BF_RAW) || n == 0) { P.kind = 1; return P; }
@@ -638,9 +648,29 @@ __device__ Plan plan_frame(const przm_block_t &bd, const przm_meta_t *mt, const 
     if (ncmd == 0) P.l2 = 0;
     else if (pack < ncmd) P.l2 = pack;
     else { P.l2 = 1 + ncmd; P.sub_ch[0] = P.sub_ch[1] = P.sub_ch[2] = 0xFFFFFFFFu; }
-    /* Section 3: flat LEB128 */
+    /* Section 3: flat LEB128, or with the prefix option PackageOffsetsSection's choice
+     * (distance.go:692-849): the shorter code stream (V-ANS only when strictly shorter than
+     * Mode 12, from 64 distances on), and the container only when strictly shorter than the
+     * varints and its code stream is not exactly the match count long */
     const uint32_t noff = mt->len[PRZM_S_OFF];
     P.l3 = noff ? 1 + noff : 0;
+    P.s3m = noff ? 0x09u : 0u;
+    P.s3c = 0;
+    P.s3cl = 0;
+    if (of && noff) {
+        const uint32_t nd = of->n;
+        uint32_t cl = of->m12_len == PRZM_ENC_OVERFLOW ? 0xFFFFFFFFu : 5u + of->m12_len, ck = 0;
+        if (nd >= 64u && of->v_len != PRZM_ENC_OVERFLOW) {
+            const uint32_t tv = 8u + of->mat_len + 16u + of->v_len;
+            if (tv < cl) { cl = tv; ck = 1; }
+        }
+        if (cl != 0xFFFFFFFFu) {
+            uint32_t h;
+            const uint32_t m = prefix_mode(noff, nd, cl, &h);
+            const uint32_t l3 = h + cl + of->extra;
+            if (l3 < noff && cl != nd) { P.l3 = l3; P.s3m = m; P.s3c = ck; P.s3cl = cl; }
+        }
+    }
     /* Section 4: literals.go:368-384 / 664-693 without the sheaf trials */
     const uint32_t nlit = mt->len[PRZM_S_LIT];
     P.s4v = 0;
@@ -671,20 +701,26 @@ __device__ __forceinline__ uint32_t frame_bytes(const przm_block_t &bd, const Pl
     return 8 + (P.ext ? 6 : 4) + 8 + P.l2 + P.l3 + P.l4;
 }
 
+/* ows: the blocks' offsets workspaces with the prefix option, nullptr without */
 extern "C" __global__ void __launch_bounds__(256)
 przm_frame_size_kernel(const przm_block_t *__restrict__ blocks, przm_meta_t *__restrict__ meta,
-                       const uint8_t *__restrict__ ws, uint32_t nblocks, uint32_t *__restrict__ frame_len) {
+                       const uint8_t *__restrict__ ws, uint32_t nblocks, uint32_t *__restrict__ frame_len,
+                       const uint8_t *__restrict__ ows) {
     const uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x;
     if (blk >= nblocks) return;
     const przm_block_t bd = blocks[blk];
     przm_meta_t *mt = meta + blk;
-    const Plan P = plan_frame(bd, mt, ws + (uint64_t)blk * PRZM_WS_BYTES);
+    const przm_offs_t *of = ows ? (const przm_offs_t *)(ows + (uint64_t)blk * PRZM_OWS_BYTES + PRZM_OWS_HDR) : nullptr;
+    const Plan P = plan_frame(bd, mt, ws + (uint64_t)blk * PRZM_WS_BYTES, of);
     const uint32_t fl = frame_bytes(bd, P);
     mt->frame_len = fl;
     mt->frame_kind = P.kind;
     for (int s = 0; s < 3; s++) mt->choice[s] = P.sub_ch[s];
     mt->s_len[0] = P.l2; mt->s_len[1] = P.l3; mt->s_len[2] = P.l4;
     mt->s4_vans = P.s4v;
+    mt->s3_mode = P.s3m;
+    mt->s3_code = P.s3c;
+    mt->s3_code_len = P.s3cl;
     frame_len[blk] = fl;
 }
 
@@ -777,7 +813,7 @@ extern "C" __global__ void __launch_bounds__(256)
 przm_frame_write_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks,
                         const przm_meta_t *__restrict__ meta, const uint8_t *__restrict__ ws,
                         const przm_ftab_t *__restrict__ ftab, const uint64_t *__restrict__ frame_off,
-                        uint8_t *__restrict__ out, uint32_t nblocks) {
+                        uint8_t *__restrict__ out, uint32_t nblocks, const uint8_t *__restrict__ ows) {
     const uint32_t blk = blockIdx.x;
     if (blk >= nblocks) return;
     const przm_block_t bd = blocks[blk];
@@ -853,9 +889,42 @@ przm_frame_write_kernel(const uint8_t *__restrict__ src, const przm_block_t *__r
         p += l2;
     }
     /* Section 3 */
-    if (l3) {
+    if (l3 && mt->s3_mode == 0x09u) {
         if (threadIdx.x == 0) o[p] = 0x09;
         wr_bytes(o + p + 1, wsb + PRZM_WS_OFF, l3 - 1);
+        p += l3;
+    } else if (l3) { /* assemblePrefixContainer (distance.go:777-823) */
+        const uint8_t *ob = ows + (uint64_t)blk * PRZM_OWS_BYTES;
+        const przm_offs_t *of = (const przm_offs_t *)(ob + PRZM_OWS_HDR);
+        const uint32_t m = mt->s3_mode, cl = mt->s3_code_len, orig = mt->len[PRZM_S_OFF], nd = of->n;
+        const uint32_t h = m == 0x0Bu ? 4u : m == 0x0Au ? 7u : 13u;
+        uint8_t *s3 = o + p;
+        if (threadIdx.x == 0) {
+            s3[0] = (uint8_t)m;
+            if (m == 0x0Bu) {
+                s3[1] = (uint8_t)orig; s3[2] = (uint8_t)nd; s3[3] = (uint8_t)cl;
+            } else if (m == 0x0Au) {
+                s3[1] = (uint8_t)orig; s3[2] = (uint8_t)(orig >> 8); s3[3] = (uint8_t)nd; s3[4] = (uint8_t)(nd >> 8);
+                s3[5] = (uint8_t)cl; s3[6] = (uint8_t)(cl >> 8);
+            } else {
+                wr_le32(s3 + 1, orig); wr_le32(s3 + 5, nd); wr_le32(s3 + 9, cl);
+            }
+        }
+        uint8_t *cs = s3 + h;
+        if (mt->s3_code == 0) { /* [0x0C][u32 state][stream] (distance.go:364-375) */
+            if (threadIdx.x == 0) { cs[0] = 0x0C; wr_le32(cs + 1, of->m12_state); }
+            wr_bytes(cs + 5, ob + PRZM_OWS_M12 + PRZM_OWS_M12_CAP - of->m12_len, of->m12_len);
+        } else { /* 'V','A',1,0, u32 n, matrix, four states, stream */
+            const uint32_t ml = of->mat_len, sl = of->v_len;
+            if (threadIdx.x == 0) {
+                cs[0] = 'V'; cs[1] = 'A'; cs[2] = 0x01; cs[3] = 0x00;
+                wr_le32(cs + 4, nd);
+                for (int j = 0; j < 4; j++) wr_le32(cs + 8 + ml + 4 * j, of->v_state[j]);
+            }
+            wr_bytes(cs + 8, ((const przm_ftab_t *)(ob + PRZM_OWS_FTAB))->mat, ml);
+            wr_bytes(cs + 8 + ml + 16, ob + PRZM_OWS_VANS + PRZM_OWS_VANS_CAP - sl, sl);
+        }
+        wr_bytes(cs + cl, ob + PRZM_OWS_EXTRA, of->extra);
         p += l3;
     }
     /* Section 4 */

@@ -26,6 +26,7 @@
 #include "../../include/przm_gpu.h"
 #include "przm_entropy.hip"
 #include "przm_lz.hip"
+#include "przm_offsets.hip"
 
 namespace {
 
@@ -81,18 +82,18 @@ struct DevBuf {
 /* One set of device buffers + stream: the handle owns one, the server one per slot. */
 struct Ctx {
     hipStream_t st = nullptr;
-    DevBuf in, out, blocks, ws, meta, ftab, flen, foff, run, scratch, dbg, perm, tok, tmp;
+    DevBuf in, out, blocks, ws, meta, ftab, flen, foff, run, scratch, dbg, perm, tok, tmp, ows;
     uint64_t gen = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    struct { const void *src; void *out; uint32_t nblocks, depth, grid, chunk, sb, rnt; uint64_t gen; } gkey{};
+    struct { const void *src; void *out; uint32_t nblocks, depth, grid, chunk, sb, rnt, s3; uint64_t gen; } gkey{};
     std::vector<hipEvent_t> ev;
     void release() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
-        for (DevBuf *b : {&in, &out, &blocks, &ws, &meta, &ftab, &flen, &foff, &run, &scratch, &dbg, &perm, &tok, &tmp})
+        for (DevBuf *b : {&in, &out, &blocks, &ws, &meta, &ftab, &flen, &foff, &run, &scratch, &dbg, &perm, &tok, &tmp, &ows})
             b->release();
         for (auto e : ev) (void)hipEventDestroy(e);
         ev.clear();
@@ -157,7 +158,8 @@ constexpr uint64_t NA_OFF_FOFF = NA_OFF_FLEN + na_al(NA_BLOCKS * 4ull);
 constexpr uint64_t NA_OFF_RUN = NA_OFF_FOFF + na_al(NA_BLOCKS * 8ull);
 constexpr uint64_t NA_OFF_WS = NA_OFF_RUN + 256;
 constexpr uint64_t NA_OFF_LZ = NA_OFF_WS + na_al((uint64_t)NA_BLOCKS * PRZM_WS_BYTES);
-constexpr uint64_t NA_BYTES = NA_OFF_LZ + (uint64_t)NA_GRID * LZ_SCRATCH_STRIDE;
+constexpr uint64_t NA_OFF_OWS = NA_OFF_LZ + (uint64_t)NA_GRID * LZ_SCRATCH_STRIDE;
+constexpr uint64_t NA_BYTES = NA_OFF_OWS + (uint64_t)NA_BLOCKS * PRZM_OWS_BYTES;
 
 __device__ __align__(256) uint8_t scratch[NA_BYTES];
 
@@ -207,6 +209,7 @@ struct przm_gpu {
     uint32_t chunk = 8192;
     uint32_t search_batch = 4; /* chain candidates per search step: 2, 4 or 8 (which LZ kernel) */
     uint32_t rans_nt = 64;     /* coder threads per workgroup: 64 or 128 (which coder kernel) */
+    bool s3_prefix = false;    /* Section 3 as PackageOffsetsSection chooses it (else flat LEB128) */
     bool use_graph = true;
     Ctx c;
     std::mutex mu;
@@ -266,8 +269,9 @@ void build_blocks(const przm_span *bufs, uint32_t nbufs, std::vector<przm_block_
     first[nbufs] = blocks.size();
 }
 
-/* device buffers for nblocks blocks processed in chunks of `chunk` */
-int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg) {
+/* device buffers for nblocks blocks processed in chunks of `chunk`; offs: the per-block
+ * workspace of the Section-3 prefix option too */
+int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg, bool offs = false) {
     const uint32_t ch = std::min<uint32_t>(std::max<uint32_t>(nblocks, 1), chunk);
     CHK(c.blocks.ensure((size_t)std::max<uint32_t>(nblocks, 1) * sizeof(przm_block_t), &c.gen));
     CHK(c.ws.ensure((size_t)ch * PRZM_WS_BYTES, &c.gen));
@@ -280,6 +284,7 @@ int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg)
     CHK(c.run.ensure(64 + 4ull * ((std::max<uint32_t>(nblocks, 1) + ch - 1) / ch + 2), &c.gen));
     CHK(c.scratch.ensure((size_t)grid * LZ_SCRATCH_STRIDE, &c.gen));
     if (dbg) CHK(c.dbg.ensure((size_t)std::max<uint32_t>(nblocks, 1) * PRZM_BLOCK * 2, &c.gen));
+    if (offs) CHK(c.ows.ensure((size_t)ch * PRZM_OWS_BYTES, &c.gen));
     if (!c.st) HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
     return 0;
 }
@@ -309,6 +314,13 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         }
         return 0;
     };
+    /* the Section-3 prefix option needs the offsets workspace of a chunk */
+    const bool offs = g->s3_prefix && !stop_after_lz;
+    uint8_t *ows = offs ? (uint8_t *)c.ows.p : nullptr;
+    if (offs && (!ows || c.ows.cap < (size_t)std::min(chunk, std::max<uint32_t>(nblocks, 1)) * PRZM_OWS_BYTES)) {
+        snprintf(g_err, sizeof g_err, "the prefix offsets option needs its workspace (set it before the buffers)");
+        return PRZM_GPU_ERR_ARG;
+    }
     CHK(rec());
     if (check) hipLaunchKernelGGL(przm_check_kernel, dim3(nblocks), dim3(64), 0, st, d_src, blocks, nblocks);
     const uint32_t nchunks = (nblocks + chunk - 1) / chunk;
@@ -347,14 +359,21 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         hipLaunchKernelGGL(g->rans_nt == 128 ? przm_rans_kernel_128 : przm_rans_kernel,
                            dim3(PRZM_J_N * ((nb + jobs - 1) / jobs)), dim3(g->rans_nt), 0, st,
                            (uint8_t *)c.ws.p, meta + c0, (const przm_ftab_t *)c.ftab.p, nb, (const uint32_t *)c.perm.p);
+        if (offs) { /* Section 3 as a distance-prefix container: codes, extra bits, code streams */
+            hipLaunchKernelGGL(przm_offs_prep_kernel, dim3(nb), dim3(OFFS_NT), 0, st, (const uint8_t *)c.ws.p,
+                               (const przm_meta_t *)(meta + c0), ows, nb);
+            hipLaunchKernelGGL(przm_offs_m12_kernel, dim3((nb + 63) / 64), dim3(64), 0, st, ows, nb);
+            hipLaunchKernelGGL(przm_offs_vans_kernel, dim3((nb + 15) / 16), dim3(64), 0, st, ows, nb);
+        }
         CHK(rec());
         hipLaunchKernelGGL(przm_frame_size_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, blocks + c0, meta + c0,
-                           (const uint8_t *)c.ws.p, nb, flen + c0);
+                           (const uint8_t *)c.ws.p, nb, flen + c0, (const uint8_t *)ows);
         CHK(rec());
         hipLaunchKernelGGL(przm_scan_kernel, dim3(1), dim3(1024), 0, st, flen + c0, nb, foff + c0, run);
         CHK(rec());
         hipLaunchKernelGGL(przm_frame_write_kernel, dim3(nb), dim3(256), 0, st, d_src, blocks + c0, meta + c0,
-                           (const uint8_t *)c.ws.p, (const przm_ftab_t *)c.ftab.p, foff + c0, d_out, nb);
+                           (const uint8_t *)c.ws.p, (const przm_ftab_t *)c.ftab.p, foff + c0, d_out, nb,
+                           (const uint8_t *)ows);
         HIPCHK(hipGetLastError());
     }
     CHK(rec());
@@ -391,7 +410,7 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
     const uint32_t depth = (uint32_t)depth_of_level(level);
     const int grid = g->lz_grid;
     const uint32_t chunk = g->chunk;
-    CHK(ensure_ctx(c, nblocks, chunk, grid, false));
+    CHK(ensure_ctx(c, nblocks, chunk, grid, false, g->s3_prefix));
     HIPCHK(hipMemcpyAsync(c.blocks.p, blocks.data(), blocks.size() * sizeof(przm_block_t), hipMemcpyHostToDevice, c.st));
     if (c.ev.size() < 2) {
         for (int i = 0; i < 2; i++) {
@@ -407,7 +426,8 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
     if (use_graph) {
         const bool hit = c.gexec && c.gkey.src == d_src && c.gkey.out == d_out && c.gkey.nblocks == nblocks &&
                          c.gkey.depth == depth && c.gkey.grid == (uint32_t)grid && c.gkey.chunk == chunk &&
-                         c.gkey.sb == g->search_batch && c.gkey.rnt == g->rans_nt && c.gkey.gen == c.gen;
+                         c.gkey.sb == g->search_batch && c.gkey.rnt == g->rans_nt &&
+                         c.gkey.s3 == (uint32_t)g->s3_prefix && c.gkey.gen == c.gen;
         if (!hit) {
             if (c.gexec) { (void)hipGraphExecDestroy(c.gexec); c.gexec = nullptr; }
             if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
@@ -419,7 +439,8 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
             HIPCHK(e2);
             c.graph = gr;
             HIPCHK(hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
-            c.gkey = {d_src, d_out, nblocks, depth, (uint32_t)grid, chunk, g->search_batch, g->rans_nt, c.gen};
+            c.gkey = {d_src, d_out, nblocks, depth, (uint32_t)grid, chunk, g->search_batch, g->rans_nt,
+                      (uint32_t)g->s3_prefix, c.gen};
         }
         HIPCHK(hipGraphLaunch(c.gexec, c.st));
     } else {
@@ -613,6 +634,13 @@ int32_t przm_gpu_set_kernel_shape(przm_gpu *g, int32_t search_batch, int32_t ran
     return 0;
 }
 
+int32_t przm_gpu_set_offsets_mode(przm_gpu *g, int32_t mode) {
+    if (!g || (mode != PRZM_OFFSETS_FLAT && mode != PRZM_OFFSETS_PREFIX)) return PRZM_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->s3_prefix = mode == PRZM_OFFSETS_PREFIX;
+    return 0;
+}
+
 int32_t przm_gpu_get_tuning(przm_gpu *g, int32_t tuned[4]) {
     if (!g || !tuned) return PRZM_GPU_ERR_ARG;
     std::lock_guard<std::mutex> lk(g->mu);
@@ -764,6 +792,8 @@ int32_t przm_gpu_encode_device_noalloc(przm_gpu *g, const void *d_src, uint64_t 
     v.run.p = base + NA_OFF_RUN;
     v.ws.p = base + NA_OFF_WS;
     v.scratch.p = base + NA_OFF_LZ;
+    v.ows.p = base + NA_OFF_OWS;
+    v.ows.cap = (size_t)NA_BLOCKS * PRZM_OWS_BYTES;
     HIPCHK(hipMemcpyAsync(v.blocks.p, blocks, nb * sizeof(przm_block_t), hipMemcpyHostToDevice, v.st));
     const int grid = std::min<int>(std::min<int>(g->lz_grid, (int)NA_GRID), (int)nb);
     /* every counter the kernels start from zero is cleared by a fill kernel inside enqueue */
@@ -1194,7 +1224,7 @@ int32_t host_grow(uint8_t **p, uint64_t *cap, uint64_t need) {
 uint64_t ctx_bytes(const Ctx &c) {
     uint64_t t = 0;
     for (const DevBuf *b : {&c.in, &c.out, &c.blocks, &c.ws, &c.meta, &c.ftab, &c.flen, &c.foff, &c.run, &c.scratch,
-                            &c.dbg, &c.perm, &c.tok, &c.tmp})
+                            &c.dbg, &c.perm, &c.tok, &c.tmp, &c.ows})
         t += b->cap;
     return t;
 }
@@ -1209,7 +1239,7 @@ int32_t slot_reserve(przm_server *s, Slot &sl, uint64_t arena, uint32_t nb) {
     CHK(host_grow(&sl.h_out, &sl.h_out_cap, bound));
     CHK(sl.c.in.ensure(arena + 64, &sl.c.gen));
     CHK(sl.c.out.ensure(bound, &sl.c.gen));
-    CHK(ensure_ctx(sl.c, std::max<uint32_t>(nb, 1), std::max<uint32_t>(nb, 1), g->lz_grid, false));
+    CHK(ensure_ctx(sl.c, std::max<uint32_t>(nb, 1), std::max<uint32_t>(nb, 1), g->lz_grid, false, g->s3_prefix));
     return 0;
 }
 

@@ -27,6 +27,9 @@ MaxOffset32 = 1048575
 MaxMatchLen64 = 65535
 # frame block of this implementation (one self-contained frame per block)
 BlockSize = 64 * 1024
+# Section 3 of the frames (GPUCompressor.SetOffsetsMode)
+OffsetsFlat = 0    # 0x09 + the LEB128 varints
+OffsetsPrefix = 1  # the distance-prefix container when shorter
 # bridge.go:148-153, the reference tokenizer's tile and per-tile stream slots.  Here the tile is
 # the 64 KiB block (window and extension bounded by it), so the reference's GPU tests that
 # expect a 1 MiB window and a 4095 length cap (gpu_tier2_boundaries_test.go) see 64 KiB and 1024.
@@ -287,6 +290,12 @@ class GPUCompressor:
     @property
     def handle(self):
         return self._h
+
+    def SetOffsetsMode(self, mode: int) -> None:
+        """Section 3 of the frames written from now on: OffsetsFlat (LEB128, the default) or
+        OffsetsPrefix (the distance-prefix container when shorter, PackageOffsetsSection of
+        core/distance/distance.go:825-849, as the reference CPU path writes it)."""
+        self._check(self._lib.przm_gpu_set_offsets_mode(self._h, int(mode)))
 
     def _check(self, rc):
         if rc:
