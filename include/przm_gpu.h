@@ -64,7 +64,7 @@ typedef struct {
     double pcie_overhead_percent;
     int32_t pipelined_chunks;
     int32_t graph_launch;
-    double stage_ms[8]; /* check, lz, hist, rans, frame-size, scan, frame-write, spare */
+    double stage_ms[8]; /* check, lz, hist, rans, frame-size, scan, frame-write, offsets (prefix option) */
     /* filled by profiled (non-graph) runs: bytes of the blocks the LZ kernel
      * tokenized, token-stream bytes it wrote, frame bytes, uncompressed frames */
     uint64_t lz_in_bytes;

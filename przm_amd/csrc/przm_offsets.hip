@@ -13,13 +13,14 @@
  *                         positions a second scan gives (BitWriter,
  *                         distance.go:217-247); the codes' histogram feeds
  *                         NormalizeFrequencies for the V-ANS alternative.
- *  przm_offs_m12_kernel   one lane per block: the codes' Mode-12 stream
- *                         (distance.go:323-376), one rANS state whose table is
- *                         picked by the class of the previous code, written
- *                         backward from the end of the block's slot.
- *  przm_offs_vans_kernel  one lane quad per block with 64 or more distances:
- *                         the codes' adaptive V-ANS stream (the 4-state coder of
- *                         the literal sections, literals.go:368-384).
+ *  przm_offs_code_kernel  both code streams of the codes, in one launch:
+ *                         the Mode-12 stream (distance.go:323-376), one rANS
+ *                         state whose table is picked by the class of the
+ *                         previous code, one lane per block, written backward
+ *                         from the end of the block's slot; and, for blocks
+ *                         with 64 or more distances, the adaptive V-ANS stream
+ *                         (the 4-state coder of the literal sections,
+ *                         literals.go:368-384), one lane quad per block.
  *
  * The frame kernels then compare the code streams, pick the container mode
  * (SelectPrefixMode, distance.go:750-758) and take the container only when it
@@ -71,10 +72,21 @@ __device__ __forceinline__ uint32_t wg_scan_excl(uint32_t v, uint32_t *part, uin
     return before + inc - v;
 }
 
+/* bytes of a 32-bit word, little-endian: byte j of w */
+#define WBYTE(w, j) (((w) >> (8u * (j))) & 0xFFu)
+
+/*
+ * One workgroup per block; each thread takes 4 bytes of the offset stream per round (1024
+ * bytes a round).  A byte without its top bit ends a varint: its thread reads the varint's
+ * continuation bytes from the 8-byte window of its own word and the word before it (a varint
+ * of a distance below 2^28 has at most 4 bytes).  One workgroup scan per round numbers the
+ * varints and places their extra bits (count << 20 | bits: a round has at most 1024 varints
+ * and 1024 * 14 extra bits).
+ */
 extern "C" __global__ void __launch_bounds__(OFFS_NT)
 przm_offs_prep_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__restrict__ meta, uint8_t *__restrict__ ows,
                       uint32_t nblocks) {
-    __shared__ uint32_t ext[OFFS_EXT_WORDS];
+    __shared__ __align__(16) uint32_t ext[OFFS_EXT_WORDS];
     __shared__ uint32_t cnt[256];
     __shared__ uint32_t key[256];
     __shared__ uint32_t frq[256];
@@ -82,58 +94,86 @@ przm_offs_prep_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__restr
     const uint32_t blk = blockIdx.x, tid = threadIdx.x;
     if (blk >= nblocks) return;
     const uint8_t *off = ws + (uint64_t)blk * PRZM_WS_BYTES + PRZM_WS_OFF;
+    const uint32_t *offw = (const uint32_t *)off;
     const uint32_t noff = meta[blk].len[PRZM_S_OFF];
     uint8_t *ob = ows + (uint64_t)blk * PRZM_OWS_BYTES;
     przm_offs_t *H = (przm_offs_t *)(ob + PRZM_OWS_HDR);
     uint8_t *codes = ob + PRZM_OWS_CODES;
-    for (uint32_t i = tid; i < OFFS_EXT_WORDS; i += OFFS_NT) ext[i] = 0;
+    for (uint32_t i = tid; i < OFFS_EXT_WORDS / 4u; i += OFFS_NT) ((uint4 *)ext)[i] = make_uint4(0, 0, 0, 0);
     for (uint32_t i = tid; i < 256; i += OFFS_NT) cnt[i] = 0;
     __syncthreads();
     uint32_t nd = 0, nbits = 0;
-    for (uint32_t base = 0; base < noff; base += OFFS_NT) {
-        const uint32_t i = base + tid;
-        const uint32_t b = i < noff ? off[i] : 0x80u;
-        const bool term = (b & 0x80u) == 0;
-        uint32_t d = 0, code = 0, extra = 0, nb = 0;
-        if (term) {
-            /* this byte ends a varint: its continuation bytes are the ones just before it */
-            d = b;
-            for (uint32_t k = 1; k <= 4u && k <= i; k++) {
-                const uint32_t c = off[i - k];
+    /* own word (bytes at or past noff read as 0x80: no varint ends there) and the word before it
+     * (none before the stream: 0, so a walk back stops); the next round's are loaded a round ahead */
+    auto load_cur = [&](uint32_t p0) -> uint32_t { return p0 < noff ? offw[p0 >> 2] : 0x80808080u; };
+    auto load_prev = [&](uint32_t p0) -> uint32_t { return p0 >= 4u && p0 < noff ? offw[(p0 >> 2) - 1u] : 0u; };
+    uint32_t ncur = load_cur(4u * tid), nprev = load_prev(4u * tid);
+    for (uint32_t base = 0; base < noff; base += 4u * OFFS_NT) {
+        const uint32_t p0 = base + 4u * tid;
+        uint32_t cur = ncur;
+        const uint32_t prevw = nprev;
+        ncur = load_cur(p0 + 4u * OFFS_NT);
+        nprev = load_prev(p0 + 4u * OFFS_NT);
+        if (p0 < noff && noff - p0 < 4u) {
+            const uint32_t valid = noff - p0;
+            cur = (cur & ((1u << (8u * valid)) - 1u)) | (0x80808080u << (8u * valid));
+        }
+        const uint64_t win = ((uint64_t)cur << 32) | prevw; /* bytes p0 - 4 .. p0 + 3 */
+        uint32_t code[4], extra[4], nbk[4], term = 0, nt = 0, nbt = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) {
+            code[j] = 0; extra[j] = 0; nbk[j] = 0;
+            const uint32_t b = WBYTE(cur, j);
+            if (b & 0x80u) continue;
+            uint32_t d = b;
+#pragma unroll
+            for (uint32_t k = 1; k <= 4; k++) { /* window byte 4 + j - k */
+                const uint32_t c = (uint32_t)(win >> (8u * (4u + j - k))) & 0xFFu;
                 if (!(c & 0x80u)) break;
                 d = (d << 7) | (c & 0x7Fu);
             }
             if (d <= 4u) {
-                code = d - 1u;
+                code[j] = d - 1u;
             } else {
                 const uint32_t u = d - 1u;
-                nb = 30u - (uint32_t)__builtin_clz(u);
+                const uint32_t nb = 30u - (uint32_t)__builtin_clz(u);
                 const uint32_t half = (u >> nb) & 1u;
-                code = 2u * nb + 2u + half;
-                extra = d - (((2u + half) << nb) + 1u);
+                code[j] = 2u * nb + 2u + half;
+                extra[j] = d - (((2u + half) << nb) + 1u);
+                nbk[j] = nb;
             }
+            term |= 1u << j;
+            nt++;
+            nbt += nbk[j];
         }
-        uint32_t nt, bt;
-        const uint32_t idx = nd + wg_scan_excl(term ? 1u : 0u, part, &nt);
-        const uint32_t bit = nbits + wg_scan_excl(term ? nb : 0u, part, &bt);
-        /* a block holds at most PRZM_OFFS_MAX distances; a longer stream keeps its flat section
-         * (the coder kernels skip it), so nothing past the slots is written */
-        if (term && idx < PRZM_OFFS_MAX && (bit + nb) <= 32u * (OFFS_EXT_WORDS - 1u)) {
-            codes[idx] = (uint8_t)code;
-            atomicAdd(&cnt[code], 1u);
-            if (nb) {
-                /* bit k of the stream is bit 31 - (k & 31) of word k >> 5 (bytes big-endian) */
-                const uint32_t w = bit >> 5, o = bit & 31u;
-                if (o + nb <= 32u) {
-                    atomicOr(&ext[w], extra << (32u - o - nb));
-                } else {
-                    atomicOr(&ext[w], extra >> (o + nb - 32u));
-                    atomicOr(&ext[w + 1], extra << (64u - o - nb));
+        uint32_t tot;
+        const uint32_t ex = wg_scan_excl((nt << 20) | nbt, part, &tot);
+        uint32_t idx = nd + (ex >> 20), bit = nbits + (ex & 0xFFFFFu);
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) {
+            if (!(term >> j & 1u)) continue;
+            const uint32_t nb = nbk[j];
+            /* a block holds at most PRZM_OFFS_MAX distances; a longer stream keeps its flat
+             * section (the coder kernels skip it), so nothing past the slots is written */
+            if (idx < PRZM_OFFS_MAX && bit + nb <= 32u * (OFFS_EXT_WORDS - 1u)) {
+                codes[idx] = (uint8_t)code[j];
+                atomicAdd(&cnt[code[j] & 255u], 1u); /* codes are <= 48 for the distances of a block */
+                if (nb) {
+                    /* bit k of the stream is bit 31 - (k & 31) of word k >> 5 (bytes big-endian) */
+                    const uint32_t w = bit >> 5, o = bit & 31u;
+                    if (o + nb <= 32u) {
+                        atomicOr(&ext[w], extra[j] << (32u - o - nb));
+                    } else {
+                        atomicOr(&ext[w], extra[j] >> (o + nb - 32u));
+                        atomicOr(&ext[w + 1], extra[j] << (64u - o - nb));
+                    }
                 }
             }
+            idx++;
+            bit += nb;
         }
-        nd += nt;
-        nbits += bt;
+        nd += tot >> 20;
+        nbits += tot & 0xFFFFFu;
     }
     __syncthreads();
     const uint32_t eb = nd <= PRZM_OFFS_MAX ? (nbits + 7u) >> 3 : 0u;
@@ -151,50 +191,77 @@ przm_offs_prep_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__restr
         normalize_wave(cnt, key, frq, (przm_ftab_t *)(ob + PRZM_OWS_FTAB), &H->mat_len);
 }
 
-/* the Mode-12 stream of block blockIdx.x * 64 + lane's codes (distance.go:323-376) */
-extern "C" __global__ void __launch_bounds__(64)
-przm_offs_m12_kernel(uint8_t *__restrict__ ows, uint32_t nblocks) {
-    __shared__ uint32_t tab[256];
+/*
+ * The Mode-12 stream of block blockIdx.x * 64 + lane's codes (distance.go:323-376), one lane
+ * per block.  The codes are read 16 at a time with the next two chunks in flight, so a step
+ * waits on its table entry (LDS) and its own arithmetic only.  Emitted byte k belongs at
+ * end[-1 - k]: four bytes are gathered into one aligned 32-bit store (end is 16-byte aligned).
+ * The renormalisation emits at most two bytes per step (x < 2^24 and 4096 f - 1 >= 4095).
+ */
+__device__ void offs_m12(uint8_t *__restrict__ ows, uint32_t nblocks, uint32_t group, uint32_t *tab) {
     for (uint32_t i = threadIdx.x; i < 256; i += 64) tab[i] = (&przm_dt.fc[0][0])[i];
     __syncthreads();
-    const uint32_t blk = blockIdx.x * 64u + threadIdx.x;
+    const uint32_t blk = group * 64u + threadIdx.x;
     if (blk >= nblocks) return;
     uint8_t *ob = ows + (uint64_t)blk * PRZM_OWS_BYTES;
     przm_offs_t *H = (przm_offs_t *)(ob + PRZM_OWS_HDR);
     if (H->n > PRZM_OFFS_MAX) return; /* m12_len stays PRZM_ENC_OVERFLOW: the section stays flat */
     const uint32_t n = H->n;
-    const uint8_t *codes = ob + PRZM_OWS_CODES;
-    uint8_t *const end = ob + PRZM_OWS_M12 + PRZM_OWS_M12_CAP;
-    const uint32_t cap = PRZM_OWS_M12_CAP - 16u;
-    uint32_t x = 65536u, K = 0;
+    const uint4 *c4 = (const uint4 *)(ob + PRZM_OWS_CODES);
+    uint32_t *const endw = (uint32_t *)(ob + PRZM_OWS_M12 + PRZM_OWS_M12_CAP);
+    /* a chunk emits at most 32 bytes, so stopping once past cap writes at most cap + 32 */
+    const uint32_t cap = PRZM_OWS_M12_CAP - 48u;
+    const uint4 z = make_uint4(0, 0, 0, 0);
+    const int32_t nc = (int32_t)((n + 15u) >> 4);
+    uint4 cur = nc > 0 ? c4[nc - 1] : z, nxt = nc > 1 ? c4[nc - 2] : z;
+    uint32_t x = 65536u, K = 0, acc = 0;
     bool over = false;
-    uint32_t sym = n ? codes[n - 1] : 0u;
-    for (uint32_t i = n; i-- > 0;) {
-        const uint32_t prev = i ? codes[i - 1] : 0u;
-        const uint32_t e = tab[dist_ctx(prev) * 64u + sym];
-        const uint32_t f = e & 0xFFFFu;
-        while (x > 4096u * f - 1u) {
-            end[-1 - (int64_t)K] = (uint8_t)x;
-            K++;
-            x >>= 8;
+#define M12_EMIT(b)                                              \
+    do {                                                         \
+        acc |= (b) << (24u - 8u * (K & 3u));                     \
+        if ((K & 3u) == 3u) { endw[-1 - (int32_t)(K >> 2)] = acc; acc = 0; } \
+        K++;                                                     \
+    } while (0)
+    for (int32_t c = nc - 1; c >= 0; c--) {
+        const uint4 pre = c > 1 ? c4[c - 2] : z;
+        const uint32_t w[4] = {cur.x, cur.y, cur.z, cur.w};
+        const uint32_t below = c > 0 ? nxt.w >> 24 : 0u; /* code 16 c - 1 (0 before the first) */
+#pragma unroll
+        for (int32_t k = 15; k >= 0; k--) {
+            if ((uint32_t)(16 * c + k) >= n) continue;
+            const uint32_t sym = WBYTE(w[k >> 2], (uint32_t)k & 3u);
+            const uint32_t prev = k ? WBYTE(w[(k - 1) >> 2], (uint32_t)(k - 1) & 3u) : below;
+            const uint32_t e = tab[dist_ctx(prev) * 64u + sym];
+            const uint32_t f = e & 0xFFFFu, lim = 4096u * f - 1u;
+            if (x > lim) {
+                M12_EMIT(x & 0xFFu);
+                x >>= 8;
+                if (x > lim) {
+                    M12_EMIT(x & 0xFFu);
+                    x >>= 8;
+                }
+            }
+            uint32_t q, r;
+            divmod(x, f, &q, &r);
+            x = (q << 12) + (e >> 16) + r;
         }
-        uint32_t q, r;
-        divmod(x, f, &q, &r);
-        x = (q << 12) + (e >> 16) + r;
-        sym = prev;
+        cur = nxt;
+        nxt = pre;
         if (K > cap) { over = true; break; }
     }
+#undef M12_EMIT
+    /* the last, partial word: its bytes sit at the word's high addresses; the rest of the word
+     * lies below the stream and is not part of it */
+    if (!over && (K & 3u)) endw[-1 - (int32_t)(K >> 2)] = acc;
     H->m12_len = over ? PRZM_ENC_OVERFLOW : K;
     H->m12_state = x;
 }
 
-/* the adaptive V-ANS stream of the codes of block blockIdx.x * 16 + quad (64+ distances) */
-extern "C" __global__ void __launch_bounds__(64)
-przm_offs_vans_kernel(uint8_t *__restrict__ ows, uint32_t nblocks) {
-    __shared__ uint32_t vt[16][256];
-    __shared__ __align__(16) uint8_t stg[16][RANS_STAGE + 16];
+/* the adaptive V-ANS stream of the codes of block group * 16 + quad (64+ distances) */
+__device__ void offs_vans(uint8_t *__restrict__ ows, uint32_t nblocks, uint32_t group, uint32_t (*vt)[256],
+                          uint8_t (*stg)[RANS_STAGE + 16]) {
     const uint32_t q = threadIdx.x >> 2, j = threadIdx.x & 3u;
-    const uint32_t blk = blockIdx.x * 16u + q;
+    const uint32_t blk = group * 16u + q;
     const bool has = blk < nblocks;
     uint8_t *ob = ows + (uint64_t)(has ? blk : 0u) * PRZM_OWS_BYTES;
     przm_offs_t *H = (przm_offs_t *)(ob + PRZM_OWS_HDR);
@@ -218,4 +285,18 @@ przm_offs_vans_kernel(uint8_t *__restrict__ ows, uint32_t nblocks) {
     if (!live) return;
     if (j == 0) H->v_len = K;
     H->v_state[j] = x;
+}
+
+/* both code streams in one launch, so that the few long-running Mode-12 workgroups (one lane
+ * per block) and the V-ANS workgroups (one quad per block) overlap: workgroups below m12_groups
+ * run the Mode-12 coder over 64 blocks each, the rest the V-ANS coder over 16 blocks each */
+struct OffsCoderShared {
+    uint32_t vt[16][256];
+    __align__(16) uint8_t stg[16][RANS_STAGE + 16];
+};
+extern "C" __global__ void __launch_bounds__(64)
+przm_offs_code_kernel(uint8_t *__restrict__ ows, uint32_t nblocks, uint32_t m12_groups) {
+    __shared__ OffsCoderShared S;
+    if (blockIdx.x < m12_groups) offs_m12(ows, nblocks, blockIdx.x, S.vt[0]);
+    else offs_vans(ows, nblocks, blockIdx.x - m12_groups, S.vt, S.stg);
 }

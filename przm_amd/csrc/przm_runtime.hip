@@ -360,10 +360,12 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
                            dim3(PRZM_J_N * ((nb + jobs - 1) / jobs)), dim3(g->rans_nt), 0, st,
                            (uint8_t *)c.ws.p, meta + c0, (const przm_ftab_t *)c.ftab.p, nb, (const uint32_t *)c.perm.p);
         if (offs) { /* Section 3 as a distance-prefix container: codes, extra bits, code streams */
+            CHK(rec());
             hipLaunchKernelGGL(przm_offs_prep_kernel, dim3(nb), dim3(OFFS_NT), 0, st, (const uint8_t *)c.ws.p,
                                (const przm_meta_t *)(meta + c0), ows, nb);
-            hipLaunchKernelGGL(przm_offs_m12_kernel, dim3((nb + 63) / 64), dim3(64), 0, st, ows, nb);
-            hipLaunchKernelGGL(przm_offs_vans_kernel, dim3((nb + 15) / 16), dim3(64), 0, st, ows, nb);
+            const uint32_t m12_groups = (nb + 63) / 64;
+            hipLaunchKernelGGL(przm_offs_code_kernel, dim3(m12_groups + (nb + 15) / 16), dim3(64), 0, st, ows, nb,
+                               m12_groups);
         }
         CHK(rec());
         hipLaunchKernelGGL(przm_frame_size_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, blocks + c0, meta + c0,
@@ -466,17 +468,21 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
          * event-record node of a captured graph records are not accepted by hipEventElapsedTime) */
         const std::vector<hipEvent_t> &sev = stage_ev;
         if (want_stages && sev.size() >= 2) {
-            /* layout: [start][check+zero] then per chunk [lz][hist][rans][size][scan][write] ... [end] */
+            /* layout: [start][check+zero] then per chunk [lz][hist][rans]([offsets])[size][scan][write]
+             * ... [end]; the offsets kernels of the prefix option are stage 7 */
+            static const int flat_map[6] = {1, 2, 3, 4, 5, 6}, pre_map[7] = {1, 2, 3, 7, 4, 5, 6};
+            const int per = g->s3_prefix ? 7 : 6;
+            const int *map = g->s3_prefix ? pre_map : flat_map;
             float t;
             HIPCHK(hipEventElapsedTime(&t, sev[0], sev[1]));
             prof->stage_ms[0] += t;
             size_t i = 1;
-            while (i + 6 < sev.size()) {
-                for (int s = 0; s < 6; s++) {
+            while (i + per < sev.size()) {
+                for (int s = 0; s < per; s++) {
                     HIPCHK(hipEventElapsedTime(&t, sev[i + s], sev[i + s + 1]));
-                    prof->stage_ms[1 + s] += t;
+                    prof->stage_ms[map[s]] += t;
                 }
-                i += 6;
+                i += per;
             }
         }
         for (auto e : stage_ev) (void)hipEventDestroy(e);
