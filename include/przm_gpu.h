@@ -216,7 +216,7 @@ int32_t przm_gpu_encode_device(przm_gpu *g, const void *d_src, uint64_t n, int32
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc); it has no counterpart in the reference.  Allocates
  * nothing: every intermediate buffer lives in one static device array of the
- * library (about 48 MB per GPU, held until the process ends), so n is at most
+ * library (about 50 MB per GPU, held until the process ends), so n is at most
  * 1 MiB (16 frames; PRZM_GPU_ERR_CAPACITY above), out_cap must hold
  * przm_gpu_frames_bound(n) and frame_sizes is required.  All calls on one GPU
  * share that array: calls that overlap on one GPU (from two handles or two
@@ -242,8 +242,10 @@ uint64_t przm_gpu_frames_bound(uint64_t n);
 /* ---- autotuner ---- */
 /* Times candidate launch shapes on a sample (up to ~4 GiB from the front) of
  * the given device arena, one stage at a time, and keeps the fastest for this
- * handle: the match kernel's workgroups and the blocks per launch, then its
- * chain candidates per search step, then the coder's jobs per workgroup.
+ * handle: the blocks per launch (16384, 32768, 65536, or the whole batch when
+ * its workspace fits) on two match workgroups per CU, then one per CU (taken
+ * only when more than 1 % faster), then the match kernel's chain candidates
+ * per search step, then the coder's jobs per workgroup.
  * tuned[0..3] = lz grid, chunk blocks, coder jobs per workgroup, search batch. */
 int32_t przm_gpu_autotune(przm_gpu *g, const void *d_src, const przm_span *bufs, uint32_t nbufs, int32_t level,
                           int32_t tuned[4]);

@@ -217,6 +217,11 @@ def IsAvailable() -> bool:  # bridge.go:61
     return lib.przm_gpu_probe(ctypes.byref(info)) == 0
 
 
+def NewGPUCompressor(deviceID: int = 0) -> "GPUCompressor":  # bridge.go:291
+    """A compressor bound to one device (GPUError when it cannot be created)."""
+    return GPUCompressor(deviceID)
+
+
 def GetGlobalCompressor() -> "GPUCompressor":  # bridge.go:96
     global _global
     with _global_mu:
