@@ -11,13 +11,15 @@
  *   - Section 3 as 0x09 + LEB128 offsets, 0x00 + u32 length, or a distance-prefix
  *     container 0x06 / 0x0A / 0x0B whose code stream is raw, Mode 12 (0x0C), adaptive
  *     V-ANS or flat (core/distance/distance.go:563-690);
- *   - Section 4 as 4-lane V-ANS (docs/PRZM_SPEC.md:582-647) or raw literals;
+ *   - Section 4 as 4-lane V-ANS (docs/PRZM_SPEC.md:582-647), a sheaf container of
+ *     four context fibers, each raw or V-ANS (core/literals/literals.go:601-664,
+ *     sheaf.go:20-202), or raw literals;
  *   - the opcodes 0x00-0x7F (fused repeat, registers 0..2), 0x80-0xBF (explicit
  *     offset), 0xC0-0xC2 (long repeat), 0xF0 (literal run), 0xF1 (byte run) and
  *     0xF3-0xFA (short literal runs), with the ring registers {1, 4, 8}
  *     (docs/PRZM_SPEC.md:468-526, decompress.go).
- * Anything else (stride, dictionaries, normalisation, delta masks, sheaf and
- * nibble-plane literal sections, the other opcodes) is refused with
+ * Anything else (stride, dictionaries, normalisation, delta masks, nibble-plane
+ * literal sections, the other opcodes) is refused with
  * PRZM_DEC_ERR_UNSUPPORTED rather than guessed.
  * Decoded bytes equal the reference decoders' on every frame it accepts.
  */

@@ -29,6 +29,8 @@
  *                             323-376, 692-849 (PackageOffsetsSection)
  *   Section 4                 core/literals/literals.go:664-693 without the
  *                             sheaf trials
+ *   Section 4 sheaf option    core/literals/literals.go:344-355, 601-693,
+ *                             sheaf.go:20-152 (PackageLiteralsSection)
  *   frame assembly            compress.go:113-132, 183-202, 962-1033
  *
  * Mode-G rule added on top of the reference: a raw Section 4 that either
@@ -606,6 +608,96 @@ uint32_t orc_package_offsets(const uint8_t *off, uint32_t noff, uint8_t *out, in
     return 1 + noff;
 }
 
+/* ------------------------------------------------------------------ */
+/* Section 4 with the sheaf trials (the encoder option that brings     */
+/* PackageLiteralsSection of the reference CPU path to mode G).        */
+
+static int sheaf_ws(uint8_t b) { return b == ' ' || b == '\t' || b == '\r' || b == '\n'; }
+static int sheaf_syntax(uint8_t b) { /* sheaf.go:21-25 */
+    return b && strchr("{}[]()<>:;,.\"'`=+-*/%&|^!~?\\#", b) != NULL;
+}
+static int sheaf_ident(uint8_t b) { return (b >= 'a' && b <= 'z') || b == '_'; }
+static int sheaf_digit(uint8_t b) { return b >= '0' && b <= '9'; }
+
+/* the fiber of a literal whose two previous literals are p2, p1 (0 before the first):
+ * ContextClassify modes 0-2 (sheaf.go:36-83) and ContextClassifyOrder2 mode 3 (86-102) */
+int orc_sheaf_class(uint8_t p2, uint8_t p1, int mode) {
+    switch (mode) {
+    case 0: return sheaf_ws(p1) ? 0 : sheaf_syntax(p1) ? 1 : sheaf_ident(p1) ? 2 : 3;
+    case 1: return (p1 == '"' || p1 == ':') ? 0 : sheaf_ident(p1) ? 1 : sheaf_digit(p1) ? 2 : 3;
+    case 2: return (sheaf_ws(p1) || sheaf_syntax(p1)) ? 0 : 1;
+    default:
+        if (sheaf_digit(p1)) return (sheaf_digit(p2) || p2 == '.' || p2 == '-') ? 0 : 1;
+        if (sheaf_ident(p1)) return sheaf_ident(p2) ? 2 : 3;
+        return sheaf_ws(p1) ? 0 : 1;
+    }
+}
+
+/* PackEtaleSheaf (literals.go:601-664) of lits[0, n) in one mode: DecomposeFibers
+ * (sheaf.go:125-152), each fiber as EncodeVANS4x (the container without its 8-byte head,
+ * literals.go:344-355) with its own normalised table when shorter than the fiber, and the
+ * 40-byte header.  out holds 40 + 2n + 4 * 600 bytes.  Returns the container length. */
+uint32_t orc_pack_sheaf(const uint8_t *lits, uint32_t n, int mode, uint8_t *out) {
+    uint8_t *fib = (uint8_t *)malloc((size_t)n + 16);
+    uint8_t *enc = (uint8_t *)malloc(2 * (size_t)n + 700);
+    uint32_t l[4] = {0, 0, 0, 0}, start[4], e[4];
+    uint8_t p1 = 0, p2 = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        l[orc_sheaf_class(p2, p1, mode)]++;
+        p2 = p1;
+        p1 = lits[i];
+    }
+    start[0] = 0;
+    for (int k = 1; k < 4; k++) start[k] = start[k - 1] + l[k - 1];
+    uint32_t fill[4] = {start[0], start[1], start[2], start[3]};
+    p1 = p2 = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        fib[fill[orc_sheaf_class(p2, p1, mode)]++] = lits[i];
+        p2 = p1;
+        p1 = lits[i];
+    }
+    memcpy(out, "VA\x01", 3);
+    out[3] = (uint8_t)(0x01 | 0x04 | ((mode & 0x0F) << 4));
+    wr32(out + 4, n);
+    uint32_t pos = 40;
+    for (int k = 0; k < 4; k++) {
+        e[k] = 0;
+        if (!l[k]) continue;
+        uint32_t counts[256];
+        uint16_t freq[256];
+        orc_histogram(fib + start[k], l[k], counts);
+        orc_normalize(counts, freq);
+        const uint32_t cl = orc_vans4x_encode(fib + start[k], l[k], freq, enc) - 8;
+        if (cl < l[k]) { memcpy(out + pos, enc + 8, cl); e[k] = cl; }
+        else { memcpy(out + pos, fib + start[k], l[k]); e[k] = l[k]; }
+        pos += e[k];
+    }
+    for (int k = 0; k < 4; k++) { wr32(out + 8 + 4 * k, l[k]); wr32(out + 24 + 4 * k, e[k]); }
+    free(fib);
+    free(enc);
+    return pos;
+}
+
+/* PackageLiteralsSection (literals.go:667-693): the shortest of the adaptive V-ANS container
+ * and the four sheaf containers, tried in that order with strict <, when shorter than the
+ * literals; else the literals.  out holds 40 + 2n + 4 * 600 bytes.  *kind: 0 raw, 1 V-ANS,
+ * 2 + mode for a sheaf container (the frame then sets flag 0x80). */
+uint32_t orc_package_literals(const uint8_t *lits, uint32_t n, uint8_t *out, int *kind) {
+    *kind = 0;
+    if (!n) return 0;
+    uint8_t *tmp = (uint8_t *)malloc(40 + 2 * (size_t)n + 2400);
+    uint32_t best = n;
+    const uint32_t lv = orc_vans4x_adaptive(lits, n, tmp);
+    if (lv < best) { best = lv; *kind = 1; memcpy(out, tmp, lv); }
+    for (int m = 0; m < 4; m++) {
+        const uint32_t ls = orc_pack_sheaf(lits, n, m, tmp);
+        if (ls < best) { best = ls; *kind = 2 + m; memcpy(out, tmp, ls); }
+    }
+    if (*kind == 0) memcpy(out, lits, n);
+    free(tmp);
+    return best;
+}
+
 /* compress.go:113-132 */
 int orc_check_high_entropy(const uint8_t *src, uint32_t n) {
     uint32_t sample = n < 512 ? n : 512;
@@ -646,14 +738,18 @@ typedef struct {
     uint32_t s4_vans;
     uint32_t s3_mode;     /* first byte of Section 3: 0x09 flat, 0x06 / 0x0A / 0x0B prefix; 0 none */
     int32_t s3_code;      /* prefix code stream: 0 Mode-12, 1 adaptive V-ANS; -1 flat or none */
+    uint32_t s4_kind;     /* Section 4: 0 raw, 1 adaptive V-ANS, 2 + mode a sheaf container */
 } orc_frame_info;
+
+#define ORC_OPT_PREFIX 1u /* Section 3 as PackageOffsetsSection chooses it */
+#define ORC_OPT_SHEAF 2u  /* Section 4 as PackageLiteralsSection chooses it (sheaf trials) */
 
 /*
  * One block (n <= 65536) -> one frame.  out holds 8 + n bytes.
  * compress.go:962-1033 with the mode-G sections described above.
  */
-uint32_t orc_encode_block_s3(const uint8_t *src, uint32_t n, int level, int s3_prefix, uint8_t *out,
-                             orc_frame_info *info) {
+uint32_t orc_encode_block_opts(const uint8_t *src, uint32_t n, int level, unsigned opts, uint8_t *out,
+                               orc_frame_info *info) {
     orc_frame_info tmp_info;
     if (!info) info = &tmp_info;
     memset(info, 0, sizeof(*info));
@@ -676,10 +772,10 @@ uint32_t orc_encode_block_s3(const uint8_t *src, uint32_t n, int level, int s3_p
 
     uint8_t *s2 = (uint8_t *)malloc(16 + 2 * (size_t)s.ncmd + 3 * 640);
     uint32_t l2 = orc_package_commands(&s, s2, info->sub_choice);
-    /* Section 3: flat 0x09, or with s3_prefix PackageOffsetsSection's choice */
+    /* Section 3: flat 0x09, or with ORC_OPT_PREFIX PackageOffsetsSection's choice */
     uint8_t *s3 = (uint8_t *)malloc(16 + 3 * (size_t)s.noff + 700);
     uint32_t l3;
-    if (s3_prefix) {
+    if (opts & ORC_OPT_PREFIX) {
         int mode, ck;
         l3 = orc_package_offsets(s.off, s.noff, s3, &mode, &ck);
         info->s3_mode = (uint32_t)mode;
@@ -689,13 +785,20 @@ uint32_t orc_encode_block_s3(const uint8_t *src, uint32_t n, int level, int s3_p
         if (l3) { s3[0] = 0x09; memcpy(s3 + 1, s.off, s.noff); }
         info->s3_mode = l3 ? 0x09 : 0;
     }
-    uint8_t *s4 = (uint8_t *)malloc(8 + 600 + 16 + 2 * (size_t)s.nlit);
+    /* Section 4: adaptive V-ANS or raw, or with ORC_OPT_SHEAF PackageLiteralsSection's choice */
+    uint8_t *s4 = (uint8_t *)malloc(40 + 2400 + 16 + 2 * (size_t)s.nlit);
     uint32_t l4 = 0;
     int vans = 0;
-    if (s.nlit) {
+    if (s.nlit && (opts & ORC_OPT_SHEAF)) {
+        int k;
+        l4 = orc_package_literals(s.lit, s.nlit, s4, &k);
+        vans = k != 0;
+        info->s4_kind = (uint32_t)k;
+    } else if (s.nlit) {
         uint32_t lv = orc_vans4x_adaptive(s.lit, s.nlit, s4);
         if (lv < s.nlit) { l4 = lv; vans = 1; }
         else { memcpy(s4, s.lit, s.nlit); l4 = s.nlit; }
+        info->s4_kind = (uint32_t)vans;
     }
     info->s2 = l2; info->s3 = l3; info->s4 = l4; info->s4_vans = (uint32_t)vans;
     int ext = l2 > 65535 || l3 > 65535;
@@ -733,15 +836,15 @@ uint32_t orc_encode_block_s3(const uint8_t *src, uint32_t n, int level, int s3_p
 }
 
 uint32_t orc_encode_block(const uint8_t *src, uint32_t n, int level, uint8_t *out, orc_frame_info *info) {
-    return orc_encode_block_s3(src, n, level, 0, out, info);
+    return orc_encode_block_opts(src, n, level, 0, out, info);
 }
 
 /* A buffer -> one frame per 64 KiB block (an empty buffer -> one empty
  * uncompressed frame, as compress.go:1036-1114 produces for u == 0).
  * frame_sizes gets one entry per frame; returns total bytes, or 0 when
  * out_cap / frames_cap is too small. */
-uint64_t orc_encode_buffer_s3(const uint8_t *src, uint64_t n, int level, int s3_prefix, uint8_t *out,
-                              uint64_t out_cap, uint32_t *frame_sizes, uint64_t frames_cap, uint64_t *nframes) {
+uint64_t orc_encode_buffer_opts(const uint8_t *src, uint64_t n, int level, unsigned opts, uint8_t *out,
+                                uint64_t out_cap, uint32_t *frame_sizes, uint64_t frames_cap, uint64_t *nframes) {
     uint64_t nb = n ? (n + ORC_BLOCK - 1) / ORC_BLOCK : 1;
     if (nb > frames_cap) return 0;
     uint64_t total = 0;
@@ -750,7 +853,7 @@ uint64_t orc_encode_buffer_s3(const uint8_t *src, uint64_t n, int level, int s3_
         uint64_t off = b * ORC_BLOCK;
         uint32_t len = (uint32_t)(n - off < ORC_BLOCK ? n - off : ORC_BLOCK);
         if (!n) len = 0;
-        uint32_t fl = orc_encode_block_s3(src + off, len, level, s3_prefix, tmp, NULL);
+        uint32_t fl = orc_encode_block_opts(src + off, len, level, opts, tmp, NULL);
         if (total + fl > out_cap) { free(tmp); return 0; }
         memcpy(out + total, tmp, fl);
         frame_sizes[b] = fl;
@@ -763,7 +866,7 @@ uint64_t orc_encode_buffer_s3(const uint8_t *src, uint64_t n, int level, int s3_
 
 uint64_t orc_encode_buffer(const uint8_t *src, uint64_t n, int level, uint8_t *out, uint64_t out_cap,
                            uint32_t *frame_sizes, uint64_t frames_cap, uint64_t *nframes) {
-    return orc_encode_buffer_s3(src, n, level, 0, out, out_cap, frame_sizes, frames_cap, nframes);
+    return orc_encode_buffer_opts(src, n, level, 0, out, out_cap, frame_sizes, frames_cap, nframes);
 }
 
 /* Stream-level entry point for tests: tokenize one block into caller

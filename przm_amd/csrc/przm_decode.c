@@ -5,7 +5,8 @@
  * 457-647 commands, offsets and V-ANS; decompress.go:113-260 header and
  * section framing; core/commands/commands.go:205-258, 474-520, 755-789
  * Section 2; core/distance/distance.go:262-321, 525-690 Section 3;
- * core/literals/literals.go:696-736 Section 4; core/commands/vans_meta.go:45-68
+ * core/literals/literals.go:601-664, 696-736 and sheaf.go Section 4;
+ * core/commands/vans_meta.go:45-68
  * Meta8 contexts) for the features listed in the header.  Every read is
  * bounds-checked; a frame either decodes completely or returns an error.
  */
@@ -107,18 +108,14 @@ static int build_table(const uint16_t *freq, uint16_t *cdf, uint8_t *slot) {
     return c == 4096u ? 0 : PRZM_DEC_ERR_CORRUPT;
 }
 
-/* 'V','A',1,flags, u32 n, then the joint-nibble matrix (flags 0) or nothing (Meta8, flags & 0x40),
- * four states and the stream: n symbols into dst (cap bytes) */
-static int64_t vans_container(const uint8_t *c, size_t cn, uint8_t *dst, size_t cap, int meta8) {
-    if (cn < 8) return PRZM_DEC_ERR_CORRUPT;
-    const uint32_t n = le32(c + 4);
-    if (n > cap) return PRZM_DEC_ERR_CAPACITY;
+/* from c[p]: the joint-nibble matrix (adaptive) or nothing (Meta8), four states and the stream:
+ * n symbols into dst */
+static int64_t vans_body(const uint8_t *c, size_t cn, size_t p, uint8_t *dst, uint32_t n, int meta8) {
     rans_tables t;
     memset(&t, 0, sizeof t);
     uint16_t freq[256], cdf[256];
     uint16_t (*mcdf)[256] = NULL;
     uint8_t *slots = NULL;
-    size_t p = 8;
     int rc = 0;
     if (meta8) {
         mcdf = (uint16_t (*)[256])malloc(8 * 256 * sizeof(uint16_t));
@@ -167,6 +164,92 @@ static int64_t vans_container(const uint8_t *c, size_t cn, uint8_t *dst, size_t 
 out:
     free(mcdf);
     free(slots);
+    return rc ? rc : (int64_t)n;
+}
+
+/* 'V','A',1,flags, u32 n, then the body above: n symbols into dst (cap bytes) */
+static int64_t vans_container(const uint8_t *c, size_t cn, uint8_t *dst, size_t cap, int meta8) {
+    if (cn < 8) return PRZM_DEC_ERR_CORRUPT;
+    const uint32_t n = le32(c + 4);
+    if (n > cap) return PRZM_DEC_ERR_CAPACITY;
+    return vans_body(c, cn, 8, dst, n, meta8);
+}
+
+/* ---------------- sheaf literal containers ---------------- */
+
+static int sh_ws(uint8_t b) { return b == ' ' || b == '\t' || b == '\r' || b == '\n'; }
+static int sh_syntax(uint8_t b) {
+    switch (b) {
+    case '{': case '}': case '[': case ']': case '(': case ')': case '<': case '>': case ':': case ';':
+    case ',': case '.': case '"': case '\'': case '`': case '=': case '+': case '-': case '*': case '/':
+    case '%': case '&': case '|': case '^': case '!': case '~': case '?': case '\\': case '#':
+        return 1;
+    default:
+        return 0;
+    }
+}
+static int sh_ident(uint8_t b) { return (b >= 'a' && b <= 'z') || b == '_'; }
+static int sh_digit(uint8_t b) { return b >= '0' && b <= '9'; }
+
+/* the fiber of the next literal after p2, p1 (sheaf.go:36-102) */
+static int sh_class(uint8_t p2, uint8_t p1, int mode) {
+    switch (mode) {
+    case 0: return sh_ws(p1) ? 0 : sh_syntax(p1) ? 1 : sh_ident(p1) ? 2 : 3;
+    case 1: return (p1 == '"' || p1 == ':') ? 0 : sh_ident(p1) ? 1 : sh_digit(p1) ? 2 : 3;
+    case 2: return (sh_ws(p1) || sh_syntax(p1)) ? 0 : 1;
+    default:
+        if (sh_digit(p1)) return (sh_digit(p2) || p2 == '.' || p2 == '-') ? 0 : 1;
+        if (sh_ident(p1)) return sh_ident(p2) ? 2 : 3;
+        return sh_ws(p1) ? 0 : 1;
+    }
+}
+
+/* 'V','A',1, 0x05 | mode << 4, u32 n, u32 fiber lengths l[4], u32 coded lengths e[4], then each
+ * fiber raw (e == l) or as a V-ANS body of e bytes; the literals come back by walking the
+ * contexts and taking each next byte from its fiber (literals.go:601-664, sheaf.go:125-202) */
+static int64_t sheaf_container(const uint8_t *c, size_t cn, uint8_t *dst, uint32_t n) {
+    if (cn < 40) return PRZM_DEC_ERR_CORRUPT;
+    const int mode = c[3] >> 4;
+    if (mode > 3) return PRZM_DEC_ERR_CORRUPT;
+    uint32_t l[4], e[4];
+    uint64_t sum = 0, pos = 40;
+    for (int k = 0; k < 4; k++) {
+        l[k] = le32(c + 8 + 4 * k);
+        e[k] = le32(c + 24 + 4 * k);
+        sum += l[k];
+        if ((l[k] == 0 && e[k] != 0) || e[k] > l[k]) return PRZM_DEC_ERR_CORRUPT;
+        pos += e[k];
+    }
+    if (sum != n || pos > cn || (mode == 2 && (l[2] || l[3]))) return PRZM_DEC_ERR_CORRUPT;
+    uint8_t *fib = (uint8_t *)malloc((size_t)n + 16);
+    if (!fib) return PRZM_DEC_ERR_CAPACITY;
+    const uint8_t *f[4];
+    size_t at = 40, fo = 0;
+    int rc = 0;
+    for (int k = 0; k < 4 && !rc; k++) {
+        f[k] = fib + fo;
+        if (l[k] == 0) continue;
+        if (e[k] == l[k]) {
+            memcpy(fib + fo, c + at, l[k]);
+        } else {
+            const int64_t r = vans_body(c + at, e[k], 0, fib + fo, l[k], 0);
+            if (r < 0) rc = (int)r;
+        }
+        at += e[k];
+        fo += l[k];
+    }
+    uint32_t used[4] = {0, 0, 0, 0};
+    uint8_t p1 = 0, p2 = 0;
+    for (uint32_t i = 0; i < n && !rc; i++) {
+        const int k = sh_class(p2, p1, mode);
+        if (used[k] >= l[k]) { rc = PRZM_DEC_ERR_CORRUPT; break; }
+        dst[i] = f[k][used[k]++];
+        p2 = p1;
+        p1 = dst[i];
+    }
+    for (int k = 0; k < 4 && !rc; k++)
+        if (used[k] != l[k]) rc = PRZM_DEC_ERR_CORRUPT;
+    free(fib);
     return rc ? rc : (int64_t)n;
 }
 
@@ -422,13 +505,12 @@ static int section4(const uint8_t *s4, size_t n, uint8_t **buf, const uint8_t **
     *out = s4;
     *on = n;
     if (n >= 8 && s4[0] == 'V' && s4[1] == 'A' && s4[2] == 0x01) {
-        if (s4[3] & 0x04) return PRZM_DEC_ERR_UNSUPPORTED; /* sheaf container */
         const uint32_t l = le32(s4 + 4);
         if (l > MAX_FRAME) return PRZM_DEC_ERR_CORRUPT;
         uint8_t *m = (uint8_t *)malloc((size_t)l + 1);
         if (!m) return PRZM_DEC_ERR_CAPACITY;
         /* the flag byte is not read past the sheaf bit: every other container is adaptive V-ANS */
-        const int64_t r = vans_container(s4, n, m, l, 0);
+        const int64_t r = (s4[3] & 0x04) ? sheaf_container(s4, n, m, l) : vans_container(s4, n, m, l, 0);
         if (r < 0) { free(m); return (int)r; }
         *buf = m;
         *out = m;

@@ -17,7 +17,7 @@ U64 = ctypes.c_uint64
 class FrameInfo(ctypes.Structure):
     _fields_ = [("kind", U32), ("ncmd", U32), ("noff", U32), ("nlit", U32), ("nop", U32), ("nmeta", U32),
                 ("nvar", U32), ("ntok", U32), ("s2", U32), ("s3", U32), ("s4", U32), ("sub", ctypes.c_int32 * 3),
-                ("s4v", U32), ("s3_mode", U32), ("s3_code", ctypes.c_int32)]
+                ("s4v", U32), ("s3_mode", U32), ("s3_code", ctypes.c_int32), ("s4_kind", U32)]
 
 
 def ptr(a):
@@ -67,10 +67,16 @@ class Oracle:
         l.orc_encode_buffer.restype = U64
         l.orc_package_cmd_stream.argtypes = [P, U32, P, P]
         l.orc_package_cmd_stream.restype = ctypes.c_int64
-        l.orc_encode_block_s3.argtypes = [P, U32, ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(FrameInfo)]
-        l.orc_encode_block_s3.restype = U32
-        l.orc_encode_buffer_s3.argtypes = [P, U64, ctypes.c_int, ctypes.c_int, P, U64, P, U64, ctypes.POINTER(U64)]
-        l.orc_encode_buffer_s3.restype = U64
+        l.orc_encode_block_opts.argtypes = [P, U32, ctypes.c_int, ctypes.c_uint, P, ctypes.POINTER(FrameInfo)]
+        l.orc_encode_block_opts.restype = U32
+        l.orc_encode_buffer_opts.argtypes = [P, U64, ctypes.c_int, ctypes.c_uint, P, U64, P, U64, ctypes.POINTER(U64)]
+        l.orc_encode_buffer_opts.restype = U64
+        l.orc_sheaf_class.argtypes = [ctypes.c_uint8, ctypes.c_uint8, ctypes.c_int]
+        l.orc_sheaf_class.restype = ctypes.c_int
+        l.orc_pack_sheaf.argtypes = [P, U32, ctypes.c_int, P]
+        l.orc_pack_sheaf.restype = U32
+        l.orc_package_literals.argtypes = [P, U32, P, ctypes.POINTER(ctypes.c_int)]
+        l.orc_package_literals.restype = U32
         l.orc_dist_code.argtypes = [U32, P, P, P]
         l.orc_dist_code.restype = None
         l.orc_dist_m12_encode.argtypes = [P, U32, P]
@@ -189,23 +195,43 @@ class Oracle:
             raise ValueError("command stream does not split")
         return out[:n].tobytes(), list(ch)
 
-    def encode_block(self, src, level=6, s3_prefix=False):
-        """One frame; s3_prefix: Section 3 as PackageOffsetsSection chooses it."""
+    def encode_block(self, src, level=6, s3_prefix=False, sheaf=False):
+        """One frame; s3_prefix: Section 3 as PackageOffsetsSection chooses it; sheaf: Section 4
+        as PackageLiteralsSection chooses it."""
         a = u8(src)
         out = np.zeros(a.size + 64, np.uint8)
         info = FrameInfo()
-        n = self.l.orc_encode_block_s3(ptr(a), a.size, level, int(s3_prefix), ptr(out), ctypes.byref(info))
+        n = self.l.orc_encode_block_opts(ptr(a), a.size, level, int(s3_prefix) | 2 * int(sheaf), ptr(out),
+                                         ctypes.byref(info))
         return out[:n].tobytes(), info
 
-    def encode_buffer(self, src, level=6, s3_prefix=False):
+    def encode_buffer(self, src, level=6, s3_prefix=False, sheaf=False):
         a = u8(src)
         nb = max((a.size + 65535) // 65536, 1)
         out = np.zeros(a.size + 8 * nb + 64, np.uint8)
         sizes = np.zeros(nb, np.uint32)
         nf = U64()
-        tot = self.l.orc_encode_buffer_s3(ptr(a), a.size, level, int(s3_prefix), ptr(out), out.size, ptr(sizes), nb,
-                                          ctypes.byref(nf))
+        tot = self.l.orc_encode_buffer_opts(ptr(a), a.size, level, int(s3_prefix) | 2 * int(sheaf), ptr(out),
+                                            out.size, ptr(sizes), nb, ctypes.byref(nf))
         return out[:tot].tobytes(), sizes[: nf.value].copy()
+
+    def sheaf_class(self, p2, p1, mode):
+        return self.l.orc_sheaf_class(p2, p1, mode)
+
+    def pack_sheaf(self, lits, mode):
+        """PackEtaleSheaf's container of lits in one mode."""
+        a = u8(lits)
+        out = np.zeros(40 + 2 * a.size + 2400, np.uint8)
+        n = self.l.orc_pack_sheaf(ptr(a), a.size, mode, ptr(out))
+        return out[:n].tobytes()
+
+    def package_literals(self, lits):
+        """PackageLiteralsSection: (section, kind: 0 raw, 1 V-ANS, 2 + mode sheaf)."""
+        a = u8(lits)
+        out = np.zeros(40 + 2 * a.size + 2400, np.uint8)
+        k = ctypes.c_int()
+        n = self.l.orc_package_literals(ptr(a), a.size, ptr(out), ctypes.byref(k))
+        return out[:n].tobytes(), k.value
 
     def dist_code(self, d):
         """(code, extra, extra bit count) of distance d (DistanceToCode)."""

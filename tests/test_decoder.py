@@ -16,6 +16,7 @@ import pytest
 
 from oracle_lib import oracle, refdec, have_refdec
 from offsets_cases import copy_block
+from sheaf_cases import lits_frame
 from tokenizer_kats import KATS
 
 from przm_amd import datagen as D
@@ -87,13 +88,33 @@ def test_roundtrip_golden_fixtures():
         assert X.DecompressFrames(frames, case["frame_sizes"]) == src, name
 
 
+def test_roundtrip_sheaf_literal_frames():
+    """Frames whose Section 4 the oracle writes as PackageLiteralsSection chooses it (sheaf
+    containers of every mode among them), and containers of each mode on their own."""
+    O = oracle()
+    kinds = set()
+    rng = np.random.default_rng(88)
+    text = np.frombuffer(bytes(rng.choice(list(b'{"name": "value_1", "n": 12.5}, \n'), 400000).tolist()),
+                         np.uint8).copy()
+    for name, a in list(_inputs()) + [("json", text)]:
+        frames, sizes = O.encode_buffer(a, 6, True, True)
+        assert X.DecompressFrames(frames, sizes) == a.tobytes(), name
+        for b in range(max(1, (a.size + 65535) // 65536)):
+            kinds.add(O.encode_block(a[b * 65536:(b + 1) * 65536], 6, False, True)[1].s4_kind)
+    assert kinds & {2, 3, 4, 5}, kinds
+    for m in range(4):
+        for n in (1, 7, 300, 5000):
+            lits = bytes(rng.choice(list(b'{"k": "v_1", "n": 1.5}, \n abc'), n).tolist())
+            assert X.DecompressFrame(lits_frame(lits, O.pack_sheaf(lits, m))) == lits, (m, n)
+
+
 @needs_ref
-@pytest.mark.parametrize("prefix", [False, True])
-def test_equals_reference_decoder(prefix):
+@pytest.mark.parametrize("prefix,sheaf", [(False, False), (True, False), (True, True)])
+def test_equals_reference_decoder(prefix, sheaf):
     """Frame by frame, the same bytes as the reference C99 decoder."""
     O, R = oracle(), refdec()
     for name, a in list(_inputs()) + list(_prefix_blocks()):
-        frames, sizes = O.encode_buffer(a, 6, prefix)
+        frames, sizes = O.encode_buffer(a, 6, prefix, sheaf)
         f = np.frombuffer(frames, np.uint8)
         pos = 0
         for s in sizes:
@@ -194,10 +215,11 @@ def test_corrupted_frames_fail_cleanly():
     frames = []
     T = json.load(open(os.path.join(HERE, "golden", "offsets_ties.json")))
     rb = copy_block(T["renorm_block"]["distances"], T["renorm_block"]["seed"])
+    js = np.frombuffer(bytes(rng.choice(list(b'{"name": "value_1", "n": 12.5}, \n'), 65536).tolist()), np.uint8)
     for a, pre in ((D.mixture(65536, 84), False), (D.markov(30000, 85), False),
                    (np.frombuffer(b"abcabcabd" * 2000, np.uint8).copy(), False), (D.mixture(65536, 86), True),
-                   (rb, True)):
-        f, s = O.encode_buffer(a, 6, pre)
+                   (rb, True), (js.copy(), True)):
+        f, s = O.encode_buffer(a, 6, pre, pre)
         frames.append((np.frombuffer(f, np.uint8), a))
     for f, a in frames:
         for cut in range(0, f.size, max(1, f.size // 300)):

@@ -1,6 +1,6 @@
 """Section 3 as a distance-prefix container on the GPU (przm_gpu_set_offsets_mode,
 PRZM_OFFSETS_PREFIX), through the C ABI: every frame equals the oracle's frame with
-the same option (orc_encode_block_s3, PackageOffsetsSection of
+the same option (orc_encode_block_opts, PackageOffsetsSection of
 core/distance/distance.go:825-849), on
 
   * edge inputs, the mixture and Markov content at several levels;

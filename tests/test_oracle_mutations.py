@@ -4,7 +4,7 @@ Every case below plants one plausible slip in a copy of oracle/przm_oracle.c
 (a threshold off by one, a cap moved, a comparison made non-strict, two
 candidates tried in the other order, a register update dropped), compiles it
 with gcc into a temporary directory and runs the known-answer checks of
-tests/test_oracle_kats.py and tests/offsets_cases.py against it.  A mutant that passes every check means
+tests/test_oracle_kats.py, tests/offsets_cases.py and tests/sheaf_cases.py against it.  A mutant that passes every check means
 the checks do not pin that rule.  CPU only; the real oracle is not touched.
 
 One written rule cannot change an output and has no mutant here: the run
@@ -21,6 +21,7 @@ import pytest
 
 from oracle_lib import Oracle, have_refdec, refdec
 from offsets_cases import offsets_failures
+from sheaf_cases import sheaf_failures
 from tokenizer_kats import KATS, run_kat
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -82,6 +83,27 @@ MUTANTS = [
      "} else if (orig < 65535 && n <= 65535 && cl <= 65535) {"),
     ("prefix_non_strict", "if (pl < raw && info[1] != n) {", "if (pl <= raw && info[1] != n) {"),
     ("code_length_rule_dropped", "if (pl < raw && info[1] != n) {", "if (pl < raw) {"),
+    # Section 4 with the sheaf trials
+    ("sheaf_syntax_without_hash", 'strchr("{}[]()<>:;,.\\"\'`=+-*/%&|^!~?\\\\#", b)',
+     'strchr("{}[]()<>:;,.\\"\'`=+-*/%&|^!~?\\\\", b)'),
+    ("sheaf_ident_without_underscore", "static int sheaf_ident(uint8_t b) { return (b >= 'a' && b <= 'z') || b == '_'; }",
+     "static int sheaf_ident(uint8_t b) { return (b >= 'a' && b <= 'z'); }"),
+    ("sheaf_mode1_without_colon", "case 1: return (p1 == '\"' || p1 == ':')", "case 1: return (p1 == '\"')"),
+    ("sheaf_order2_without_dot", "(sheaf_digit(p2) || p2 == '.' || p2 == '-')", "(sheaf_digit(p2) || p2 == '-')"),
+    ("sheaf_first_context", "uint32_t l[4] = {0, 0, 0, 0}, start[4], e[4];\n    uint8_t p1 = 0, p2 = 0;",
+     "uint32_t l[4] = {0, 0, 0, 0}, start[4], e[4];\n    uint8_t p1 = ' ', p2 = 0;"),
+    ("sheaf_flags_without_4x", "out[3] = (uint8_t)(0x01 | 0x04 | ((mode & 0x0F) << 4));",
+     "out[3] = (uint8_t)(0x04 | ((mode & 0x0F) << 4));"),
+    ("sheaf_fiber_non_strict", "if (cl < l[k]) { memcpy(out + pos, enc + 8, cl); e[k] = cl; }",
+     "if (cl <= l[k]) { memcpy(out + pos, enc + 8, cl); e[k] = cl; }"),
+    ("sheaf_lengths_swapped", "wr32(out + 8 + 4 * k, l[k]); wr32(out + 24 + 4 * k, e[k]);",
+     "wr32(out + 8 + 4 * k, e[k]); wr32(out + 24 + 4 * k, l[k]);"),
+    ("literals_vans_non_strict", "if (lv < best) { best = lv; *kind = 1; memcpy(out, tmp, lv); }",
+     "if (lv <= best) { best = lv; *kind = 1; memcpy(out, tmp, lv); }"),
+    ("literals_sheaf_non_strict", "if (ls < best) { best = ls; *kind = 2 + m; memcpy(out, tmp, ls); }",
+     "if (ls <= best) { best = ls; *kind = 2 + m; memcpy(out, tmp, ls); }"),
+    ("literals_modes_reversed", "for (int m = 0; m < 4; m++) {\n        const uint32_t ls",
+     "for (int m = 3; m >= 0; m--) {\n        const uint32_t ls"),
 ]
 
 
@@ -111,7 +133,8 @@ def decision_failures(O):
     a = np.concatenate([lit, np.frombuffer(b"q" * 5000, np.uint8)])
     if O.encode_block(a)[1].kind != 3:
         bad.append("hazard")
-    return bad + offsets_failures(O, refdec() if have_refdec() else None)
+    R = refdec() if have_refdec() else None
+    return bad + offsets_failures(O, R) + sheaf_failures(O, R)
 
 
 def test_real_oracle_passes_every_check():
