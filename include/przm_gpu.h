@@ -64,7 +64,7 @@ typedef struct {
     double pcie_overhead_percent;
     int32_t pipelined_chunks;
     int32_t graph_launch;
-    double stage_ms[8]; /* check, lz, hist, rans, frame-size, scan, frame-write, offsets (prefix option) */
+    double stage_ms[8]; /* check, lz, hist, rans, frame-size, scan, frame-write, section options */
     /* filled by profiled (non-graph) runs: bytes of the blocks the LZ kernel
      * tokenized, token-stream bytes it wrote, frame bytes, uncompressed frames */
     uint64_t lz_in_bytes;
@@ -216,7 +216,7 @@ int32_t przm_gpu_encode_device(przm_gpu *g, const void *d_src, uint64_t n, int32
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc); it has no counterpart in the reference.  Allocates
  * nothing: every intermediate buffer lives in one static device array of the
- * library (about 50 MB per GPU, held until the process ends), so n is at most
+ * library (about 56 MB per GPU, held until the process ends), so n is at most
  * 1 MiB (16 frames; PRZM_GPU_ERR_CAPACITY above), out_cap must hold
  * przm_gpu_frames_bound(n) and frame_sizes is required.  All calls on one GPU
  * share that array: calls that overlap on one GPU (from two handles or two
@@ -275,6 +275,22 @@ int32_t przm_gpu_set_options(przm_gpu *g, int32_t use_graph);
  * block of a launch.  Set it while no call or server is running on the handle;
  * przm_gpu_encode_device_noalloc supports it within its fixed limit. */
 int32_t przm_gpu_set_offsets_mode(przm_gpu *g, int32_t mode);
+
+/* ---- Section 4 of the frames ---- */
+#define PRZM_LITERALS_VANS 0  /* adaptive V-ANS when shorter, else raw: the reference GPU path */
+#define PRZM_LITERALS_SHEAF 1 /* also the four sheaf containers (the reference CPU path) */
+/* How every frame this handle writes from now on codes its literals.
+ * PRZM_LITERALS_SHEAF writes Section 4 as the reference CPU path does
+ * (PackageLiteralsSection, core/literals/literals.go:667-693): the shortest
+ * of the adaptive V-ANS container and the four sheaf containers
+ * (PackEtaleSheaf, literals.go:601-664: the literals split into four fibers by
+ * the class of the one or two literals before each, core/literals/sheaf.go,
+ * each fiber V-ANS-coded with its own table when shorter), tried in that order
+ * with strict <, when shorter than the literals.  Every decoder of the format
+ * reads both.  The option adds about 346 KB of device workspace per block of a
+ * launch.  Set it while no call or server is running on the handle;
+ * przm_gpu_encode_device_noalloc supports it within its fixed limit. */
+int32_t przm_gpu_set_literals_mode(przm_gpu *g, int32_t mode);
 
 /* ---- device memory for callers without an allocator ---- */
 int32_t przm_gpu_malloc(przm_gpu *g, uint64_t bytes, void **d);

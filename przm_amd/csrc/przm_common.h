@@ -114,4 +114,31 @@ typedef struct {
 #define PRZM_OWS_VANS_CAP (2u * PRZM_OFFS_MAX + 64u)
 #define PRZM_OWS_BYTES (PRZM_OWS_VANS + PRZM_OWS_VANS_CAP)
 
+/*
+ * Section 4 with the sheaf trials (an encoder option): per-block sheaf
+ * workspace.  For each of the four modes the literals are split into four
+ * fibers, laid out back to back with each fiber starting 16-byte aligned; the
+ * winning mode's coded fibers go to slots of align16(l + 64) bytes each,
+ * filled backward from their ends.
+ */
+typedef struct {
+    uint32_t n;             /* literals */
+    uint32_t l[4][4];       /* fiber lengths per mode */
+    uint32_t k[4][4];       /* coded stream bytes per fiber from the counting pass, or PRZM_ENC_OVERFLOW
+                               (the fiber stays raw) */
+    uint32_t mat[4][4];     /* matrix bytes of each fiber's table */
+    uint32_t choice;        /* Section 4: 0 raw, 1 adaptive V-ANS, 2 + mode a sheaf container */
+    uint32_t len;           /* Section 4 bytes */
+    uint32_t ek[4];         /* the chosen mode's coded stream bytes per fiber */
+    uint32_t st[4][4];      /* and their final states */
+} przm_sheaf_t;
+
+#define PRZM_SWS_HDR 0u                                 /* przm_sheaf_t */
+#define PRZM_SWS_FTAB 512u                              /* przm_ftab_t [mode][fiber] */
+#define PRZM_SWS_FIB (PRZM_SWS_FTAB + 16u * 1576u)      /* fibers of mode m at FIB + m * FIB_CAP */
+#define PRZM_SWS_FIB_CAP (65536u + 64u)
+#define PRZM_SWS_OUT (PRZM_SWS_FIB + 4u * PRZM_SWS_FIB_CAP)
+#define PRZM_SWS_OUT_CAP (65536u + 4u * 80u + 64u)
+#define PRZM_SWS_BYTES (PRZM_SWS_OUT + PRZM_SWS_OUT_CAP)
+
 #endif

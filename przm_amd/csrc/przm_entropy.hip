@@ -625,7 +625,7 @@ __device__ __forceinline__ uint32_t rd32g(const uint8_t *p) {
 }
 
 __device__ Plan plan_frame(const przm_block_t &bd, const przm_meta_t *mt, const uint8_t *wsb,
-                           const przm_offs_t *of) {
+                           const przm_offs_t *of, const przm_sheaf_t *sh) {
     Plan P = {};
     const uint32_t n = bd.len;
     if ((bd.flags & PRZM_BF_RAW) || n == 0) { P.kind = 1; return P; }
@@ -671,11 +671,15 @@ __device__ Plan plan_frame(const przm_block_t &bd, const przm_meta_t *mt, const 
             if (l3 < noff && cl != nd) { P.l3 = l3; P.s3m = m; P.s3c = ck; P.s3cl = cl; }
         }
     }
-    /* Section 4: literals.go:368-384 / 664-693 without the sheaf trials */
+    /* Section 4: literals.go:368-384 / 664-693 without the sheaf trials, or with the sheaf option
+     * the choice of PackageLiteralsSection (przm_sheaf_encode_kernel) */
     const uint32_t nlit = mt->len[PRZM_S_LIT];
     P.s4v = 0;
     P.l4 = nlit;
-    if (nlit) {
+    if (sh && nlit) {
+        P.l4 = sh->len;
+        P.s4v = sh->choice != 0 ? 1u : 0u;
+    } else if (nlit) {
         const uint32_t ev = mt->enc_len[PRZM_J_LIT];
         if (ev != PRZM_ENC_OVERFLOW) {
             const uint32_t tv = 8 + mt->mat_len[0] + 16 + ev;
@@ -701,17 +705,18 @@ __device__ __forceinline__ uint32_t frame_bytes(const przm_block_t &bd, const Pl
     return 8 + (P.ext ? 6 : 4) + 8 + P.l2 + P.l3 + P.l4;
 }
 
-/* ows: the blocks' offsets workspaces with the prefix option, nullptr without */
+/* ows / sws: the blocks' offsets / sheaf workspaces with those options, nullptr without */
 extern "C" __global__ void __launch_bounds__(256)
 przm_frame_size_kernel(const przm_block_t *__restrict__ blocks, przm_meta_t *__restrict__ meta,
                        const uint8_t *__restrict__ ws, uint32_t nblocks, uint32_t *__restrict__ frame_len,
-                       const uint8_t *__restrict__ ows) {
+                       const uint8_t *__restrict__ ows, const uint8_t *__restrict__ sws) {
     const uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x;
     if (blk >= nblocks) return;
     const przm_block_t bd = blocks[blk];
     przm_meta_t *mt = meta + blk;
     const przm_offs_t *of = ows ? (const przm_offs_t *)(ows + (uint64_t)blk * PRZM_OWS_BYTES + PRZM_OWS_HDR) : nullptr;
-    const Plan P = plan_frame(bd, mt, ws + (uint64_t)blk * PRZM_WS_BYTES, of);
+    const przm_sheaf_t *sh = sws ? (const przm_sheaf_t *)(sws + (uint64_t)blk * PRZM_SWS_BYTES + PRZM_SWS_HDR) : nullptr;
+    const Plan P = plan_frame(bd, mt, ws + (uint64_t)blk * PRZM_WS_BYTES, of, sh);
     const uint32_t fl = frame_bytes(bd, P);
     mt->frame_len = fl;
     mt->frame_kind = P.kind;
@@ -813,7 +818,8 @@ extern "C" __global__ void __launch_bounds__(256)
 przm_frame_write_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restrict__ blocks,
                         const przm_meta_t *__restrict__ meta, const uint8_t *__restrict__ ws,
                         const przm_ftab_t *__restrict__ ftab, const uint64_t *__restrict__ frame_off,
-                        uint8_t *__restrict__ out, uint32_t nblocks, const uint8_t *__restrict__ ows) {
+                        uint8_t *__restrict__ out, uint32_t nblocks, const uint8_t *__restrict__ ows,
+                        const uint8_t *__restrict__ sws) {
     const uint32_t blk = blockIdx.x;
     if (blk >= nblocks) return;
     const przm_block_t bd = blocks[blk];
@@ -928,10 +934,47 @@ przm_frame_write_kernel(const uint8_t *__restrict__ src, const przm_block_t *__r
         p += l3;
     }
     /* Section 4 */
-    if (l4) {
-        if (mt->s4_vans) put_vans(o + p, wsb, mt, ft, PRZM_J_LIT, mt->len[PRZM_S_LIT], 0,
-                                  wsb + PRZM_WS_ENC_LIT + PRZM_ENC_LIT_CAP);
-        else wr_bytes(o + p, wsb + PRZM_WS_LIT, l4);
+    const uint8_t *sb = sws ? sws + (uint64_t)blk * PRZM_SWS_BYTES : nullptr;
+    const uint32_t ch = sb ? ((const przm_sheaf_t *)(sb + PRZM_SWS_HDR))->choice : (mt->s4_vans ? 1u : 0u);
+    if (l4 && ch == 1) {
+        put_vans(o + p, wsb, mt, ft, PRZM_J_LIT, mt->len[PRZM_S_LIT], 0, wsb + PRZM_WS_ENC_LIT + PRZM_ENC_LIT_CAP);
+    } else if (l4 && ch >= 2) { /* assembleSheafContainer (literals.go:645-664) */
+        const przm_sheaf_t *H = (const przm_sheaf_t *)(sb + PRZM_SWS_HDR);
+        const uint32_t m = ch - 2;
+        uint8_t *s4 = o + p;
+        if (threadIdx.x == 0) {
+            s4[0] = 'V'; s4[1] = 'A'; s4[2] = 0x01; s4[3] = (uint8_t)(0x05u | (m << 4));
+            wr_le32(s4 + 4, H->n);
+            for (uint32_t k = 0; k < 4; k++) {
+                const bool coded = H->l[m][k] && H->k[m][k] != PRZM_ENC_OVERFLOW;
+                wr_le32(s4 + 8 + 4 * k, H->l[m][k]);
+                wr_le32(s4 + 24 + 4 * k, coded ? H->mat[m][k] + 16u + H->k[m][k] : H->l[m][k]);
+            }
+        }
+        const uint8_t *fib = sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP;
+        const przm_ftab_t *sft = (const przm_ftab_t *)(sb + PRZM_SWS_FTAB) + 4u * m;
+        uint32_t at = 40, fs = 0, end = 0;
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t lk = H->l[m][k];
+            end += (lk + 64u + 15u) & ~15u;
+            if (!lk) continue;
+            if (H->k[m][k] != PRZM_ENC_OVERFLOW) { /* matrix, four states, stream (EncodeVANS4x) */
+                /* the counted length, which the frame's size was planned with (the final coding
+                 * of the fiber emits the same bytes: the tests compare them) */
+                const uint32_t ml = H->mat[m][k], ek = H->k[m][k];
+                if (threadIdx.x == 0)
+                    for (int j = 0; j < 4; j++) wr_le32(s4 + at + ml + 4 * j, H->st[k][j]);
+                wr_bytes(s4 + at, sft[k].mat, ml);
+                wr_bytes(s4 + at + ml + 16, sb + PRZM_SWS_OUT + end - ek, ek);
+                at += ml + 16 + ek;
+            } else {
+                wr_bytes(s4 + at, fib + fs, lk);
+                at += lk;
+            }
+            fs += (lk + 15u) & ~15u;
+        }
+    } else if (l4) {
+        wr_bytes(o + p, wsb + PRZM_WS_LIT, l4);
     }
 }
 

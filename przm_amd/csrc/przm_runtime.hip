@@ -27,6 +27,7 @@
 #include "przm_entropy.hip"
 #include "przm_lz.hip"
 #include "przm_offsets.hip"
+#include "przm_sheaf.hip"
 
 namespace {
 
@@ -82,18 +83,19 @@ struct DevBuf {
 /* One set of device buffers + stream: the handle owns one, the server one per slot. */
 struct Ctx {
     hipStream_t st = nullptr;
-    DevBuf in, out, blocks, ws, meta, ftab, flen, foff, run, scratch, dbg, perm, tok, tmp, ows;
+    DevBuf in, out, blocks, ws, meta, ftab, flen, foff, run, scratch, dbg, perm, tok, tmp, ows, sws;
     uint64_t gen = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    struct { const void *src; void *out; uint32_t nblocks, depth, grid, chunk, sb, rnt, s3; uint64_t gen; } gkey{};
+    struct { const void *src; void *out; uint32_t nblocks, depth, grid, chunk, sb, rnt, s3, s4; uint64_t gen; } gkey{};
     std::vector<hipEvent_t> ev;
     void release() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
-        for (DevBuf *b : {&in, &out, &blocks, &ws, &meta, &ftab, &flen, &foff, &run, &scratch, &dbg, &perm, &tok, &tmp, &ows})
+        for (DevBuf *b : {&in, &out, &blocks, &ws, &meta, &ftab, &flen, &foff, &run, &scratch, &dbg, &perm, &tok, &tmp, &ows,
+                          &sws})
             b->release();
         for (auto e : ev) (void)hipEventDestroy(e);
         ev.clear();
@@ -159,7 +161,8 @@ constexpr uint64_t NA_OFF_RUN = NA_OFF_FOFF + na_al(NA_BLOCKS * 8ull);
 constexpr uint64_t NA_OFF_WS = NA_OFF_RUN + 256;
 constexpr uint64_t NA_OFF_LZ = NA_OFF_WS + na_al((uint64_t)NA_BLOCKS * PRZM_WS_BYTES);
 constexpr uint64_t NA_OFF_OWS = NA_OFF_LZ + (uint64_t)NA_GRID * LZ_SCRATCH_STRIDE;
-constexpr uint64_t NA_BYTES = NA_OFF_OWS + (uint64_t)NA_BLOCKS * PRZM_OWS_BYTES;
+constexpr uint64_t NA_OFF_SWS = NA_OFF_OWS + (uint64_t)NA_BLOCKS * PRZM_OWS_BYTES;
+constexpr uint64_t NA_BYTES = NA_OFF_SWS + (uint64_t)NA_BLOCKS * PRZM_SWS_BYTES;
 
 __device__ __align__(256) uint8_t scratch[NA_BYTES];
 
@@ -210,6 +213,7 @@ struct przm_gpu {
     uint32_t search_batch = 4; /* chain candidates per search step: 2, 4 or 8 (which LZ kernel) */
     uint32_t rans_nt = 64;     /* coder threads per workgroup: 64 or 128 (which coder kernel) */
     bool s3_prefix = false;    /* Section 3 as PackageOffsetsSection chooses it (else flat LEB128) */
+    bool s4_sheaf = false;     /* Section 4 as PackageLiteralsSection chooses it (else V-ANS or raw) */
     bool use_graph = true;
     Ctx c;
     std::mutex mu;
@@ -269,9 +273,10 @@ void build_blocks(const przm_span *bufs, uint32_t nbufs, std::vector<przm_block_
     first[nbufs] = blocks.size();
 }
 
-/* device buffers for nblocks blocks processed in chunks of `chunk`; offs: the per-block
- * workspace of the Section-3 prefix option too */
-int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg, bool offs = false) {
+/* device buffers for nblocks blocks processed in chunks of `chunk`; offs / sheaf: the per-block
+ * workspaces of the Section-3 prefix and Section-4 sheaf options too */
+int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg, bool offs = false,
+                   bool sheaf = false) {
     const uint32_t ch = std::min<uint32_t>(std::max<uint32_t>(nblocks, 1), chunk);
     CHK(c.blocks.ensure((size_t)std::max<uint32_t>(nblocks, 1) * sizeof(przm_block_t), &c.gen));
     CHK(c.ws.ensure((size_t)ch * PRZM_WS_BYTES, &c.gen));
@@ -285,6 +290,7 @@ int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg,
     CHK(c.scratch.ensure((size_t)grid * LZ_SCRATCH_STRIDE, &c.gen));
     if (dbg) CHK(c.dbg.ensure((size_t)std::max<uint32_t>(nblocks, 1) * PRZM_BLOCK * 2, &c.gen));
     if (offs) CHK(c.ows.ensure((size_t)ch * PRZM_OWS_BYTES, &c.gen));
+    if (sheaf) CHK(c.sws.ensure((size_t)ch * PRZM_SWS_BYTES, &c.gen));
     if (!c.st) HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
     return 0;
 }
@@ -314,11 +320,13 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         }
         return 0;
     };
-    /* the Section-3 prefix option needs the offsets workspace of a chunk */
-    const bool offs = g->s3_prefix && !stop_after_lz;
+    /* the Section-3 prefix and Section-4 sheaf options need their workspaces for a chunk */
+    const bool offs = g->s3_prefix && !stop_after_lz, sheaf = g->s4_sheaf && !stop_after_lz;
     uint8_t *ows = offs ? (uint8_t *)c.ows.p : nullptr;
-    if (offs && (!ows || c.ows.cap < (size_t)std::min(chunk, std::max<uint32_t>(nblocks, 1)) * PRZM_OWS_BYTES)) {
-        snprintf(g_err, sizeof g_err, "the prefix offsets option needs its workspace (set it before the buffers)");
+    uint8_t *sws = sheaf ? (uint8_t *)c.sws.p : nullptr;
+    const size_t chb = (size_t)std::min(chunk, std::max<uint32_t>(nblocks, 1));
+    if ((offs && (!ows || c.ows.cap < chb * PRZM_OWS_BYTES)) || (sheaf && (!sws || c.sws.cap < chb * PRZM_SWS_BYTES))) {
+        snprintf(g_err, sizeof g_err, "a section option needs its workspace (set the option before the call)");
         return PRZM_GPU_ERR_ARG;
     }
     CHK(rec());
@@ -359,23 +367,30 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
         hipLaunchKernelGGL(g->rans_nt == 128 ? przm_rans_kernel_128 : przm_rans_kernel,
                            dim3(PRZM_J_N * ((nb + jobs - 1) / jobs)), dim3(g->rans_nt), 0, st,
                            (uint8_t *)c.ws.p, meta + c0, (const przm_ftab_t *)c.ftab.p, nb, (const uint32_t *)c.perm.p);
+        if (offs || sheaf) CHK(rec());
         if (offs) { /* Section 3 as a distance-prefix container: codes, extra bits, code streams */
-            CHK(rec());
             hipLaunchKernelGGL(przm_offs_prep_kernel, dim3(nb), dim3(OFFS_NT), 0, st, (const uint8_t *)c.ws.p,
                                (const przm_meta_t *)(meta + c0), ows, nb);
             const uint32_t m12_groups = (nb + 63) / 64;
             hipLaunchKernelGGL(przm_offs_code_kernel, dim3(m12_groups + (nb + 15) / 16), dim3(64), 0, st, ows, nb,
                                m12_groups);
         }
+        if (sheaf) { /* Section 4 with the sheaf trials: fibers, their coded lengths, the choice */
+            hipLaunchKernelGGL(przm_sheaf_split_kernel, dim3(4 * nb), dim3(SHEAF_NT), 0, st, (const uint8_t *)c.ws.p,
+                               (const przm_meta_t *)(meta + c0), sws, nb);
+            hipLaunchKernelGGL(przm_sheaf_count_kernel, dim3(nb), dim3(64), 0, st, sws, nb);
+            hipLaunchKernelGGL(przm_sheaf_encode_kernel, dim3((nb + 3) / 4), dim3(64), 0, st,
+                               (const przm_meta_t *)(meta + c0), sws, nb);
+        }
         CHK(rec());
         hipLaunchKernelGGL(przm_frame_size_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, blocks + c0, meta + c0,
-                           (const uint8_t *)c.ws.p, nb, flen + c0, (const uint8_t *)ows);
+                           (const uint8_t *)c.ws.p, nb, flen + c0, (const uint8_t *)ows, (const uint8_t *)sws);
         CHK(rec());
         hipLaunchKernelGGL(przm_scan_kernel, dim3(1), dim3(1024), 0, st, flen + c0, nb, foff + c0, run);
         CHK(rec());
         hipLaunchKernelGGL(przm_frame_write_kernel, dim3(nb), dim3(256), 0, st, d_src, blocks + c0, meta + c0,
                            (const uint8_t *)c.ws.p, (const przm_ftab_t *)c.ftab.p, foff + c0, d_out, nb,
-                           (const uint8_t *)ows);
+                           (const uint8_t *)ows, (const uint8_t *)sws);
         HIPCHK(hipGetLastError());
     }
     CHK(rec());
@@ -412,7 +427,7 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
     const uint32_t depth = (uint32_t)depth_of_level(level);
     const int grid = g->lz_grid;
     const uint32_t chunk = g->chunk;
-    CHK(ensure_ctx(c, nblocks, chunk, grid, false, g->s3_prefix));
+    CHK(ensure_ctx(c, nblocks, chunk, grid, false, g->s3_prefix, g->s4_sheaf));
     HIPCHK(hipMemcpyAsync(c.blocks.p, blocks.data(), blocks.size() * sizeof(przm_block_t), hipMemcpyHostToDevice, c.st));
     if (c.ev.size() < 2) {
         for (int i = 0; i < 2; i++) {
@@ -429,7 +444,8 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
         const bool hit = c.gexec && c.gkey.src == d_src && c.gkey.out == d_out && c.gkey.nblocks == nblocks &&
                          c.gkey.depth == depth && c.gkey.grid == (uint32_t)grid && c.gkey.chunk == chunk &&
                          c.gkey.sb == g->search_batch && c.gkey.rnt == g->rans_nt &&
-                         c.gkey.s3 == (uint32_t)g->s3_prefix && c.gkey.gen == c.gen;
+                         c.gkey.s3 == (uint32_t)g->s3_prefix && c.gkey.s4 == (uint32_t)g->s4_sheaf &&
+                         c.gkey.gen == c.gen;
         if (!hit) {
             if (c.gexec) { (void)hipGraphExecDestroy(c.gexec); c.gexec = nullptr; }
             if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
@@ -442,7 +458,7 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
             c.graph = gr;
             HIPCHK(hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
             c.gkey = {d_src, d_out, nblocks, depth, (uint32_t)grid, chunk, g->search_batch, g->rans_nt,
-                      (uint32_t)g->s3_prefix, c.gen};
+                      (uint32_t)g->s3_prefix, (uint32_t)g->s4_sheaf, c.gen};
         }
         HIPCHK(hipGraphLaunch(c.gexec, c.st));
     } else {
@@ -468,11 +484,12 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
          * event-record node of a captured graph records are not accepted by hipEventElapsedTime) */
         const std::vector<hipEvent_t> &sev = stage_ev;
         if (want_stages && sev.size() >= 2) {
-            /* layout: [start][check+zero] then per chunk [lz][hist][rans]([offsets])[size][scan][write]
-             * ... [end]; the offsets kernels of the prefix option are stage 7 */
+            /* layout: [start][check+zero] then per chunk [lz][hist][rans]([options])[size][scan][write]
+             * ... [end]; the kernels of the section options (prefix offsets, sheaf) are stage 7 */
             static const int flat_map[6] = {1, 2, 3, 4, 5, 6}, pre_map[7] = {1, 2, 3, 7, 4, 5, 6};
-            const int per = g->s3_prefix ? 7 : 6;
-            const int *map = g->s3_prefix ? pre_map : flat_map;
+            const bool opt = g->s3_prefix || g->s4_sheaf;
+            const int per = opt ? 7 : 6;
+            const int *map = opt ? pre_map : flat_map;
             float t;
             HIPCHK(hipEventElapsedTime(&t, sev[0], sev[1]));
             prof->stage_ms[0] += t;
@@ -647,6 +664,13 @@ int32_t przm_gpu_set_offsets_mode(przm_gpu *g, int32_t mode) {
     return 0;
 }
 
+int32_t przm_gpu_set_literals_mode(przm_gpu *g, int32_t mode) {
+    if (!g || (mode != PRZM_LITERALS_VANS && mode != PRZM_LITERALS_SHEAF)) return PRZM_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->s4_sheaf = mode == PRZM_LITERALS_SHEAF;
+    return 0;
+}
+
 int32_t przm_gpu_get_tuning(przm_gpu *g, int32_t tuned[4]) {
     if (!g || !tuned) return PRZM_GPU_ERR_ARG;
     std::lock_guard<std::mutex> lk(g->mu);
@@ -800,6 +824,8 @@ int32_t przm_gpu_encode_device_noalloc(przm_gpu *g, const void *d_src, uint64_t 
     v.scratch.p = base + NA_OFF_LZ;
     v.ows.p = base + NA_OFF_OWS;
     v.ows.cap = (size_t)NA_BLOCKS * PRZM_OWS_BYTES;
+    v.sws.p = base + NA_OFF_SWS;
+    v.sws.cap = (size_t)NA_BLOCKS * PRZM_SWS_BYTES;
     HIPCHK(hipMemcpyAsync(v.blocks.p, blocks, nb * sizeof(przm_block_t), hipMemcpyHostToDevice, v.st));
     const int grid = std::min<int>(std::min<int>(g->lz_grid, (int)NA_GRID), (int)nb);
     /* every counter the kernels start from zero is cleared by a fill kernel inside enqueue */
@@ -1235,7 +1261,7 @@ int32_t host_grow(uint8_t **p, uint64_t *cap, uint64_t need) {
 uint64_t ctx_bytes(const Ctx &c) {
     uint64_t t = 0;
     for (const DevBuf *b : {&c.in, &c.out, &c.blocks, &c.ws, &c.meta, &c.ftab, &c.flen, &c.foff, &c.run, &c.scratch,
-                            &c.dbg, &c.perm, &c.tok, &c.tmp, &c.ows})
+                            &c.dbg, &c.perm, &c.tok, &c.tmp, &c.ows, &c.sws})
         t += b->cap;
     return t;
 }
@@ -1250,7 +1276,8 @@ int32_t slot_reserve(przm_server *s, Slot &sl, uint64_t arena, uint32_t nb) {
     CHK(host_grow(&sl.h_out, &sl.h_out_cap, bound));
     CHK(sl.c.in.ensure(arena + 64, &sl.c.gen));
     CHK(sl.c.out.ensure(bound, &sl.c.gen));
-    CHK(ensure_ctx(sl.c, std::max<uint32_t>(nb, 1), std::max<uint32_t>(nb, 1), g->lz_grid, false, g->s3_prefix));
+    CHK(ensure_ctx(sl.c, std::max<uint32_t>(nb, 1), std::max<uint32_t>(nb, 1), g->lz_grid, false, g->s3_prefix,
+                   g->s4_sheaf));
     return 0;
 }
 

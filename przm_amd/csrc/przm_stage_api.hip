@@ -97,7 +97,7 @@ int32_t tokenize_device_impl(przm_gpu *g, const uint8_t *d_src, uint64_t n, int 
     build_blocks(&sp, 1, blocks, first);
     for (auto &b : blocks) b.flags = 0; /* every block is tokenized, high-entropy or not */
     const uint32_t nb = (uint32_t)blocks.size();
-    CHK(ensure_ctx(c, nb, nb, g->lz_grid, false, g->s3_prefix));
+    CHK(ensure_ctx(c, nb, nb, g->lz_grid, false, g->s3_prefix, g->s4_sheaf));
     HIPCHK(hipMemcpyAsync(c.blocks.p, blocks.data(), nb * sizeof(przm_block_t), hipMemcpyHostToDevice, c.st));
     CHK(ensure_events(c, 4));
     HIPCHK(hipEventRecord(c.ev[0], c.st));

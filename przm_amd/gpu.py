@@ -30,6 +30,9 @@ BlockSize = 64 * 1024
 # Section 3 of the frames (GPUCompressor.SetOffsetsMode)
 OffsetsFlat = 0    # 0x09 + the LEB128 varints
 OffsetsPrefix = 1  # the distance-prefix container when shorter
+# Section 4 of the frames (GPUCompressor.SetLiteralsMode)
+LiteralsVANS = 0   # adaptive V-ANS when shorter, else raw
+LiteralsSheaf = 1  # also the four sheaf containers
 # bridge.go:148-153, the reference tokenizer's tile and per-tile stream slots.  Here the tile is
 # the 64 KiB block (window and extension bounded by it), so the reference's GPU tests that
 # expect a 1 MiB window and a 4095 length cap (gpu_tier2_boundaries_test.go) see 64 KiB and 1024.
@@ -301,6 +304,12 @@ class GPUCompressor:
         OffsetsPrefix (the distance-prefix container when shorter, PackageOffsetsSection of
         core/distance/distance.go:825-849, as the reference CPU path writes it)."""
         self._check(self._lib.przm_gpu_set_offsets_mode(self._h, int(mode)))
+
+    def SetLiteralsMode(self, mode: int) -> None:
+        """Section 4 of the frames written from now on: LiteralsVANS (the default) or LiteralsSheaf
+        (PackageLiteralsSection of core/literals/literals.go:667-693, with the four sheaf
+        containers, as the reference CPU path writes it)."""
+        self._check(self._lib.przm_gpu_set_literals_mode(self._h, int(mode)))
 
     def _check(self, rc):
         if rc:

@@ -120,3 +120,49 @@ def sheaf_failures(O, R=None):
             if not ok:
                 bad.append(f"frame_mode_{m}")
     return bad
+
+
+_LOW = b"abcdefghijklmnopqrstuvwxyz"
+_DIG = b"0123456789"
+_UP = b"ABCDEFGHIJKLMNOPQRSTUVWXYZ"
+_SYN = b' \t\r\n{}[]()<>:;,."\'`=+-*/%&|^!~?\\#'
+
+
+def _next_mode1(rng, p2, p1):
+    # the next byte's class follows mode 1's classes of the previous one: '"' / ':' -> digit,
+    # identifier -> upper case or space, digit -> identifier, anything else -> '"' / ':'
+    if p1 in b'":':
+        return _DIG[rng.integers(0, 10)]
+    if p1 in _LOW or p1 == ord("_"):
+        return (_UP + b" ")[rng.integers(0, 27)]
+    if p1 in _DIG:
+        return _LOW[rng.integers(0, 26)]
+    return b'":'[rng.integers(0, 2)]
+
+
+def _next_mode2(rng, p2, p1):
+    # white space / syntax -> content, content -> white space / syntax
+    if p1 in _SYN:
+        return (_LOW + _DIG + _UP)[rng.integers(0, 62)]
+    return b" {}[](),;.:"[rng.integers(0, 11)]
+
+
+def _next_mode3(rng, p2, p1):
+    # the class of the two previous bytes (digit after digit, identifier after identifier, ...)
+    if p1 in _DIG:
+        return _LOW[rng.integers(0, 26)] if p2 in _DIG else _DIG[rng.integers(0, 10)]
+    if p1 in _LOW or p1 == ord("_"):
+        return _DIG[rng.integers(0, 10)] if (p2 in _LOW or p2 == ord("_")) else _LOW[rng.integers(0, 26)]
+    return _LOW[rng.integers(0, 26)]
+
+
+def modal_text(mode, n, seed):
+    """n bytes whose literal streams the option codes as sheaf containers of `mode` (1, 2 or 3):
+    each next byte is drawn from a byte class that the mode's context of the byte(s) before it
+    predicts; the mixture content takes mode 0."""
+    rng = np.random.default_rng(seed)
+    step = {1: _next_mode1, 2: _next_mode2, 3: _next_mode3}[mode]
+    out = bytearray(b"a")
+    while len(out) < n:
+        out.append(step(rng, out[-2] if len(out) > 1 else 0, out[-1]))
+    return np.frombuffer(bytes(out[:n]), np.uint8).copy()

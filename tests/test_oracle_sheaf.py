@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle_lib import oracle, refdec, have_refdec
-from sheaf_cases import HAND, TIES_PATH, container, lits_frame, sheaf_failures
+from sheaf_cases import HAND, TIES_PATH, container, lits_frame, modal_text, sheaf_failures
 
 from przm_amd import datagen as D
 
@@ -82,3 +82,15 @@ def test_sheaf_frames_decode_and_never_grow():
         for b in range(0, max(1, (a.size + 65535) // 65536)):
             kinds.add(O.encode_block(a[b * 65536:(b + 1) * 65536], 6, False, True)[1].s4_kind)
     assert {0, 1} <= kinds and kinds & {2, 3, 4, 5}, kinds
+
+
+def test_modal_inputs_take_their_mode():
+    """The inputs the GPU tests use to reach sheaf modes 1-3 on the device (the mixture reaches 0)."""
+    O = oracle()
+    for mode in (1, 2, 3):
+        a = modal_text(mode, 200000, 40 + mode)
+        kinds = {O.encode_block(a[b * 65536:(b + 1) * 65536], 6, False, True)[1].s4_kind for b in range(4)}
+        assert kinds == {2 + mode}, (mode, kinds)
+    a = D.mixture(4 << 20, 44)
+    kinds = {O.encode_block(a[b * 65536:(b + 1) * 65536], 6, False, True)[1].s4_kind for b in range(64)}
+    assert 2 in kinds, kinds
