@@ -13,9 +13,11 @@ generator (JSON-like records, int32 tables, motif-random, uniform random),
             frames compared byte for byte with the batched ones.
   plain_256MiB  one 256 MiB Markov-text buffer, 4096 blocks, one launch
             per stage.
-  prefix_offsets  the same batch with Section 3 written as a distance-prefix
-            container where shorter (przm_gpu_set_offsets_mode): GB/s, ratio,
-            bytes saved, the offsets kernels' time, a decoded sample.
+  section_options  the same batch with the frame options of the reference CPU
+            path: Section 3 as a distance-prefix container where shorter
+            (przm_gpu_set_offsets_mode), Section 4 with the sheaf trials
+            (przm_gpu_set_literals_mode), and both: GB/s, ratio, bytes saved,
+            the options' kernel time, a decoded sample.
   serving   open-loop Poisson arrivals at 2,000 req/s, sizes log-uniform in
             [4 KiB, 16 MiB], 64 GiB HBM admission budget: p50/p99 latency.
   roofline  the dominant kernel (przm_lz_kernel) against the 8 TB/s HBM
@@ -150,7 +152,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-prefix", action="store_true", help="skip the prefix-offsets leg")
+    ap.add_argument("--no-options", action="store_true", help="skip the section-option legs")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--lz-grid", type=int, default=0, help="fixed LZ workgroups (skips the autotuner with --chunk)")
     ap.add_argument("--chunk", type=int, default=0, help="fixed blocks per launch (skips the autotuner with --lz-grid)")
@@ -292,43 +294,59 @@ def main():
                  "launches_per_buffer": 8, "bit_exact_vs_batched": exact}
         log(f"plain: {plain['value']:.3f} GB/s, bit-exact vs batched: {exact}")
 
-    # the same batch with Section 3 as a distance-prefix container (the encoder option of
-    # przm_gpu_set_offsets_mode): timed like the headline form, one profiled pass for the offsets
-    # kernels' time, and the first buffers decoded by the library's decoder
-    prefix = None
-    if not args.no_prefix:
+    # the same batch with the section options (przm_gpu_set_offsets_mode / _literals_mode: Section 3
+    # as a distance-prefix container, Section 4 with the sheaf trials, as the reference CPU path
+    # writes them): timed like the headline form, one profiled pass for the options' kernels, and
+    # the first buffers decoded by the library's decoder
+    def option_leg(offsets, literals):
         from przm_amd import decode as X
-        lib.przm_gpu_set_offsets_mode(g.handle, 1)
-        batched()
-        lib.przm_gpu_sync(g.handle)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
+        lib.przm_gpu_set_offsets_mode(g.handle, offsets)
+        lib.przm_gpu_set_literals_mode(g.handle, literals)
+        try:
             batched()
-        lib.przm_gpu_sync(g.handle)
-        pw = time.perf_counter() - t0
-        pre_frames = int(out_off[-1])
-        lib.przm_gpu_set_options(g.handle, 0)
-        pprof = L.GpuProfile()
-        batched(pprof)
-        lib.przm_gpu_set_options(g.handle, 1)
-        nchk = min(nbuf, 200)
-        blob = np.zeros(int(out_off[nchk]), np.uint8)
-        lib.przm_gpu_d2h(g.handle, ctypes.c_void_p(blob.ctypes.data), d_out, blob.size)
-        ok, fi = True, 0
-        for i in range(nchk):
-            k = max((int(sizes[i]) + 65535) // 65536, 1)
-            got = X.DecompressFrames(blob[int(out_off[i]): int(out_off[i + 1])], fsz[fi: fi + k])
-            ok = ok and got == arena[int(offs[i]): int(offs[i]) + int(sizes[i])].tobytes()
-            fi += k
-        lib.przm_gpu_set_offsets_mode(g.handle, 0)
-        pst = {k: round(float(v), 3) for k, v in zip(stage_names + ["offsets"], list(pprof.stage_ms[:7]) +
+            lib.przm_gpu_sync(g.handle)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                batched()
+            lib.przm_gpu_sync(g.handle)
+            pw = time.perf_counter() - t0
+            fb = int(out_off[-1])
+            lib.przm_gpu_set_options(g.handle, 0)
+            pprof = L.GpuProfile()
+            batched(pprof)
+            lib.przm_gpu_set_options(g.handle, 1)
+            nchk = min(nbuf, 200)
+            blob = np.zeros(int(out_off[nchk]), np.uint8)
+            lib.przm_gpu_d2h(g.handle, ctypes.c_void_p(blob.ctypes.data), d_out, blob.size)
+            ok, fi = True, 0
+            for i in range(nchk):
+                k = max((int(sizes[i]) + 65535) // 65536, 1)
+                got = X.DecompressFrames(blob[int(out_off[i]): int(out_off[i + 1])], fsz[fi: fi + k])
+                ok = ok and got == arena[int(offs[i]): int(offs[i]) + int(sizes[i])].tobytes()
+                fi += k
+        finally:
+            lib.przm_gpu_set_offsets_mode(g.handle, 0)
+            lib.przm_gpu_set_literals_mode(g.handle, 0)
+        pst = {k: round(float(v), 3) for k, v in zip(stage_names + ["options"], list(pprof.stage_ms[:7]) +
                                                      [pprof.stage_ms[7]])}
-        prefix = {"value": round(total_in * args.steps / pw / 1e9, 4), "unit": "GB/s",
-                  "ms_per_step": round(pw * 1000 / args.steps, 3), "compression_ratio": round(total_in / pre_frames, 4),
-                  "frame_bytes": pre_frames, "bytes_saved_vs_flat": total_frames - pre_frames,
-                  "stages_ms": pst, "decoded_sample_buffers": nchk, "decoded_sample_ok": ok}
-        log(f"prefix offsets: {prefix['value']:.2f} GB/s, ratio {prefix['compression_ratio']}, "
-            f"saved {prefix['bytes_saved_vs_flat']} bytes, offsets kernels {pst['offsets']} ms, decode ok {ok}")
+        return {"value": round(total_in * args.steps / pw / 1e9, 4), "unit": "GB/s",
+                "ms_per_step": round(pw * 1000 / args.steps, 3), "compression_ratio": round(total_in / fb, 4),
+                "frame_bytes": fb, "bytes_saved_vs_default": total_frames - fb, "stages_ms": pst,
+                "decoded_sample_buffers": nchk, "decoded_sample_ok": ok}
+
+    options = None
+    if not args.no_options:
+        options = {}
+        for name, o, l in (("prefix_offsets", 1, 0), ("sheaf_literals", 0, 1), ("both", 1, 1)):
+            try:
+                options[name] = option_leg(o, l)
+                r = options[name]
+                log(f"{name}: {r['value']:.2f} GB/s, ratio {r['compression_ratio']}, saved "
+                    f"{r['bytes_saved_vs_default']} bytes, option kernels {r['stages_ms']['options']} ms, "
+                    f"decode ok {r['decoded_sample_ok']}")
+            except G.GPUError as e:
+                options[name] = {"error": str(e)}
+                log(f"{name}: {e}")
 
     # roofline of the dominant kernel: algorithmic bytes (block bytes read once + token-stream bytes
     # written once) per launch / average launch duration, from the profiled batched pass
@@ -445,7 +463,7 @@ def main():
         "plain": plain,
         "batched_vs_plain": round(value / plain["value"], 3) if plain else None,
         "plain_256MiB_markov": plain256,
-        "prefix_offsets": prefix,
+        "section_options": options,
         "serving": serving,
         "roofline": roofline,
         "copy_kernel_gbps": copy_gbps,

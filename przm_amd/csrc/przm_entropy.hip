@@ -5,9 +5,10 @@
  *  przm_check_kernel    checkHighEntropy (compress.go:113-132) per block.
  *  przm_hist_kernel     one wave per (block, stream): 256-bin histogram in
  *                       LDS, NormalizeFrequencies (literals.go:78-140) with
- *                       the deficit handed out by a bitonic sort on
- *                       (count desc, symbol asc), CDF, and the joint-nibble
- *                       matrix (literals.go:143-184).
+ *                       the deficit handed out to the largest (count,
+ *                       -symbol) keys, found by a bitwise search for the
+ *                       threshold key, CDF, and the joint-nibble matrix
+ *                       (literals.go:143-184).
  *  przm_rans_kernel     one lane quad per (block, coder job).  The format
  *                       fixes four interleaved states per stream
  *                       (literals.go:302-325, vans_meta.go:104-132): quad
@@ -95,13 +96,13 @@ __device__ __forceinline__ void stream_of(uint32_t k, uint8_t *wsb, const przm_m
 
 /*
  * NormalizeFrequencies (literals.go:78-140) + CDF + joint-nibble matrix
- * (literals.go:143-184) for the 256 counts in cnt[], by one wave.  key[] and
- * frq[] are LDS scratch of 256 words each.
+ * (literals.go:143-184) for the 256 counts in cnt[], by one wave.  frq[] is LDS
+ * scratch of 256 words.
  */
-__device__ void normalize_wave(const uint32_t *cnt, uint32_t *key, uint32_t *frq, przm_ftab_t *ft,
+__device__ void normalize_wave(const uint32_t *cnt, uint32_t *frq, przm_ftab_t *ft,
                                uint32_t *mat_len) {
     const uint32_t lane = threadIdx.x & 63;
-    uint32_t c4[4], f4[4];
+    uint32_t c4[4], f4[4], k4[4];
     uint32_t tot = 0, uni = 0;
     for (int i = 0; i < 4; i++) { c4[i] = cnt[lane * 4 + i]; tot += c4[i]; uni += c4[i] ? 1u : 0u; }
     tot = wave_sum(tot);
@@ -113,32 +114,27 @@ __device__ void normalize_wave(const uint32_t *cnt, uint32_t *key, uint32_t *frq
         f4[i] = c4[i] ? 1u + (uint32_t)(((uint64_t)c4[i] * rem) / (tot ? tot : 1u)) : 0u;
         sum += f4[i];
         const uint32_t s = lane * 4 + i;
-        key[s] = c4[i] >= (1u << 24) ? 0xFFFFFF00u | (255u - s) : (c4[i] << 8) | (255u - s);
+        k4[i] = c4[i] >= (1u << 24) ? 0xFFFFFF00u | (255u - s) : (c4[i] << 8) | (255u - s);
     }
     sum = wave_sum(sum);
+    /* the deficit goes +1 each to the `diff` largest keys (count, then the lower symbol): the
+     * diff-th largest key T, found bit by bit from the top (the keys are distinct, so exactly
+     * diff keys are >= T); diff <= the symbols present, so only present symbols qualify */
     const uint32_t diff = 4096u - sum;
-    __builtin_amdgcn_wave_barrier();
-    /* bitonic sort, descending, of (count, -symbol) keys */
-    for (uint32_t size = 2; size <= 256; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            for (uint32_t t = lane; t < 128; t += 64) {
-                const uint32_t i = 2 * t - (t & (stride - 1));
-                const uint32_t j = i + stride;
-                const bool desc = ((i & size) == 0);
-                const uint32_t ki = key[i], kj = key[j];
-                if (desc ? (ki < kj) : (ki > kj)) { key[i] = kj; key[j] = ki; }
-            }
-            __builtin_amdgcn_wave_barrier();
+    if (diff) {
+        uint32_t T = 0;
+        for (int b = 31; b >= 0; b--) {
+            const uint32_t cand = T | (1u << b);
+            uint32_t c = 0;
+            for (int i = 0; i < 4; i++) c += k4[i] >= cand ? 1u : 0u;
+            if (wave_sum(c) >= diff) T = cand;
         }
+        for (int i = 0; i < 4; i++) f4[i] += k4[i] >= T ? 1u : 0u;
     }
     for (int i = 0; i < 4; i++) frq[lane * 4 + i] = f4[i];
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t r = lane; r < 256; r += 64) {
-        if (r < diff) frq[255u - (key[r] & 255u)] += 1u;
-    }
-    __builtin_amdgcn_wave_barrier();
     uint32_t loc[4], acc = 0;
-    for (int i = 0; i < 4; i++) { loc[i] = frq[lane * 4 + i]; acc += loc[i]; }
+    for (int i = 0; i < 4; i++) { loc[i] = f4[i]; acc += loc[i]; }
     uint32_t cum = wave_scan_incl(acc) - acc;
     for (int i = 0; i < 4; i++) {
         ft->fc[lane * 4 + i] = loc[i] | (cum << 16);
@@ -172,7 +168,6 @@ extern "C" __global__ void __launch_bounds__(256)
 przm_hist_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, przm_ftab_t *__restrict__ ftab,
                  uint32_t nblocks) {
     __shared__ uint32_t cnt[4][256];
-    __shared__ uint32_t key[4][256];
     __shared__ uint32_t frq[4][256];
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t job = blockIdx.x * 4 + w;
@@ -216,7 +211,7 @@ przm_hist_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, przm_
     }
     __syncthreads();
     if (!run) return;
-    normalize_wave(cnt[w], key[w], frq[w], ftab + (uint64_t)blk * 4 + k, &meta[blk].mat_len[k]);
+    normalize_wave(cnt[w], frq[w], ftab + (uint64_t)blk * 4 + k, &meta[blk].mat_len[k]);
 }
 
 /* whole-buffer histogram (one workgroup): counts_out, and the normalised table */
@@ -224,7 +219,6 @@ extern "C" __global__ void __launch_bounds__(256)
 przm_hist1_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__restrict__ counts_out,
                   przm_ftab_t *__restrict__ ft, uint32_t *__restrict__ mat_len) {
     __shared__ uint32_t cnt[256];
-    __shared__ uint32_t key[256];
     __shared__ uint32_t frq[256];
     const uint32_t tid = threadIdx.x;
     cnt[tid] = 0;
@@ -237,7 +231,7 @@ przm_hist1_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__restr
     for (; i < n; i += 256) atomicAdd(&cnt[src[i]], 1u);
     __syncthreads();
     counts_out[tid] = cnt[tid];
-    if (tid < 64 && ft) normalize_wave(cnt, key, frq, ft, mat_len);
+    if (tid < 64 && ft) normalize_wave(cnt, frq, ft, mat_len);
 }
 
 /* ---------------- rANS ---------------- */
@@ -1045,11 +1039,10 @@ przm_hist_grid_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__r
 extern "C" __global__ void __launch_bounds__(64)
 przm_normalize_kernel(const uint32_t *__restrict__ counts, przm_ftab_t *__restrict__ ft, uint32_t *__restrict__ mat_len) {
     __shared__ uint32_t cnt[256];
-    __shared__ uint32_t key[256];
     __shared__ uint32_t frq[256];
     for (uint32_t s = threadIdx.x; s < 256; s += 64) cnt[s] = counts[s];
     __syncthreads();
-    normalize_wave(cnt, key, frq, ft, mat_len);
+    normalize_wave(cnt, frq, ft, mat_len);
 }
 
 /* ModeFlat section [0x00][u32 n][payload] (PackStreamFlat, gpu/kernel/przm_ans.cu:202-234) */

@@ -4,15 +4,17 @@
  * them (core/literals/literals.go:667-693): the shortest of the raw literals,
  * the adaptive V-ANS container and the four sheaf containers.
  *
- *  przm_sheaf_split_kernel   one workgroup per (block, mode): each literal's
- *                            fiber from the class of the literals before it
+ *  przm_sheaf_split_kernel   one workgroup per block: each literal's fiber in
+ *                            each mode from the class of the literals before it
  *                            (ContextClassify / ContextClassifyOrder2,
  *                            core/literals/sheaf.go:36-102); a counting pass
- *                            builds the fiber lengths and histograms, one wave
- *                            per fiber normalises its table (NormalizeFrequencies
- *                            and the matrix, literals.go:78-184), and a scatter
- *                            pass writes the fibers (DecomposeFibers,
+ *                            builds the 16 fiber lengths and histograms, the
+ *                            waves normalise the fibers' tables
+ *                            (NormalizeFrequencies and the matrix,
+ *                            literals.go:78-184), and a scatter pass per mode
+ *                            writes the fibers (DecomposeFibers,
  *                            sheaf.go:125-152), each 16-byte aligned.
+ *  przm_sheaf_order_kernel   the counting jobs by fiber length.
  *  przm_sheaf_count_kernel   one lane quad per (block, mode, fiber): the exact
  *                            length of the fiber's V-ANS stream (the coder of
  *                            literals.go:289-341 without its stores), each lane
@@ -65,91 +67,146 @@ __device__ __forceinline__ uint32_t al16(uint32_t v) { return (v + 15u) & ~15u; 
 /* byte i of the literals, 0 before the first */
 __device__ __forceinline__ uint32_t lit_at(const uint8_t *L, int64_t i) { return i >= 0 ? L[i] : 0u; }
 
+/* a fiber shorter than this is never coded: its coded form has a matrix of at least 5 bytes
+ * (row mask, one column mask, one weight) and 16 bytes of states (literals.go:622) */
+#define SHEAF_MIN_CODED 22u
+
+/* the four modes' fibers of the 16 literals at i0 (b: those literals, byte t of word t >> 2) */
+__device__ __forceinline__ void sheaf_classes16(const uint8_t (*lut)[256], uint32_t p2, uint32_t p1, const uint32_t *w,
+                                                uint32_t cls[16]) {
+#pragma unroll
+    for (uint32_t t = 0; t < 16u; t++) {
+        const uint32_t b = (w[t >> 2] >> (8u * (t & 3u))) & 0xFFu;
+        cls[t] = lut[0][p1] | (lut[1][p1] << 2) | (lut[2][p1] << 4) | (sheaf_class(lut, p2, p1, 3) << 6);
+        p2 = p1;
+        p1 = b;
+    }
+}
+
+/*
+ * One workgroup per block, all four modes: a counting pass over the literals (16 a thread and
+ * round, loaded as one 16-byte word) fills 16 histograms (mode, fiber); the fiber lengths come
+ * from them, and each wave normalises the tables of four of them (the fibers long enough to be
+ * coded); a scatter pass per mode then writes the fibers.
+ */
 extern "C" __global__ void __launch_bounds__(SHEAF_NT)
 przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__restrict__ meta,
                         uint8_t *__restrict__ sws, uint32_t nblocks) {
     __shared__ uint8_t lut[3][256];
-    __shared__ uint32_t cnt[4][256];
-    __shared__ uint32_t key[4][256];
-    __shared__ uint32_t frq[4][256];
+    __shared__ uint32_t cnt[16][256];
+    __shared__ uint32_t frq[SHEAF_NT / 64u][256];
     __shared__ uint32_t part[SHEAF_NT / 64u][2];
-    const uint32_t blk = blockIdx.x >> 2, m = blockIdx.x & 3u, tid = threadIdx.x, w = tid >> 6;
+    __shared__ uint32_t flen[16];
+    const uint32_t blk = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63u;
     if (blk >= nblocks) return;
-    const uint8_t *L = ws + (uint64_t)blk * PRZM_WS_BYTES + PRZM_WS_LIT;
+    const uint8_t *L = ws + (uint64_t)blk * PRZM_WS_BYTES + PRZM_WS_LIT; /* 16-byte aligned */
     const uint32_t n = meta[blk].len[PRZM_S_LIT];
     uint8_t *sb = sws + (uint64_t)blk * PRZM_SWS_BYTES;
     przm_sheaf_t *H = (przm_sheaf_t *)(sb + PRZM_SWS_HDR);
-    uint8_t *fib = sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP;
     for (uint32_t i = tid; i < 3 * 256; i += SHEAF_NT) (&lut[0][0])[i] = (&przm_sheaf_lut.c[0][0])[i];
-    for (uint32_t i = tid; i < 4 * 256; i += SHEAF_NT) (&cnt[0][0])[i] = 0;
+    for (uint32_t i = tid; i < 16 * 256 / 4; i += SHEAF_NT) ((uint4 *)&cnt[0][0])[i] = make_uint4(0, 0, 0, 0);
     __syncthreads();
-    /* counting pass: each thread 16 literals a round */
+    /* counting pass */
     for (uint32_t base = 0; base < n; base += 16u * SHEAF_NT) {
         const uint32_t i0 = base + 16u * tid;
-        uint32_t p2 = lit_at(L, (int64_t)i0 - 2), p1 = lit_at(L, (int64_t)i0 - 1);
+        if (i0 >= n) continue;
+        const uint4 v = *(const uint4 *)(L + i0);
+        const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+        uint32_t cls[16];
+        sheaf_classes16(lut, lit_at(L, (int64_t)i0 - 2), lit_at(L, (int64_t)i0 - 1), wd, cls);
         for (uint32_t t = 0; t < 16u && i0 + t < n; t++) {
-            const uint32_t b = L[i0 + t];
-            atomicAdd(&cnt[sheaf_class(lut, p2, p1, m)][b], 1u);
-            p2 = p1;
-            p1 = b;
+            const uint32_t b = (wd[t >> 2] >> (8u * (t & 3u))) & 0xFFu, c = cls[t];
+#pragma unroll
+            for (uint32_t m = 0; m < 4; m++) atomicAdd(&cnt[4u * m + ((c >> (2u * m)) & 3u)][b], 1u);
         }
     }
     __syncthreads();
-    /* fiber lengths (wave k sums fiber k's histogram) and tables */
-    uint32_t lk = 0;
-    for (uint32_t s = tid & 63u; s < 256; s += 64) lk += cnt[w][s];
-    lk = wave_sum(lk);
-    if ((tid & 63u) == 0) part[w][0] = lk;
+    /* fiber lengths: wave w sums rows w, w + 4, w + 8, w + 12 */
+    for (uint32_t r = w; r < 16; r += SHEAF_NT / 64u) {
+        uint32_t l = 0;
+        for (uint32_t s = lane; s < 256; s += 64) l += cnt[r][s];
+        l = wave_sum(l);
+        if (lane == 0) flen[r] = l;
+    }
     __syncthreads();
-    uint32_t l[4], fs[4];
-    for (uint32_t k = 0; k < 4; k++) l[k] = part[k][0];
-    fs[0] = 0;
-    for (uint32_t k = 1; k < 4; k++) fs[k] = fs[k - 1] + al16(l[k - 1]);
-    if (tid == 0) {
-        if (m == 0) H->n = n;
-        for (uint32_t k = 0; k < 4; k++) {
-            H->l[m][k] = l[k];
-            if (!l[k]) H->mat[m][k] = 0; /* a fiber's own wave writes it otherwise */
+    if (tid < 16) {
+        if (tid == 0) H->n = n;
+        H->l[tid >> 2][tid & 3u] = flen[tid];
+        if (flen[tid] < SHEAF_MIN_CODED) H->mat[tid >> 2][tid & 3u] = 0; /* the fiber's wave writes it otherwise */
+    }
+    for (uint32_t r = w; r < 16; r += SHEAF_NT / 64u)
+        if (flen[r] >= SHEAF_MIN_CODED)
+            normalize_wave(cnt[r], frq[w], (przm_ftab_t *)(sb + PRZM_SWS_FTAB) + r, &H->mat[r >> 2][r & 3u]);
+    /* scatter pass, one mode at a time: a literal's place in its fiber is the number of earlier
+     * literals of that fiber; one workgroup scan per round of the four per-thread counts (16 bits
+     * each: a round holds at most 4096 literals) */
+    for (uint32_t m = 0; m < 4; m++) {
+        uint8_t *fib = sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP;
+        uint32_t fs[4];
+        fs[0] = 0;
+        for (uint32_t k = 1; k < 4; k++) fs[k] = fs[k - 1] + al16(flen[4u * m + k - 1]);
+        uint32_t run[4] = {0, 0, 0, 0};
+        for (uint32_t base = 0; base < n; base += 16u * SHEAF_NT) {
+            const uint32_t i0 = base + 16u * tid;
+            uint32_t wd[4] = {0, 0, 0, 0}, cls[16], mine[4] = {0, 0, 0, 0};
+            if (i0 < n) {
+                const uint4 v = *(const uint4 *)(L + i0);
+                wd[0] = v.x; wd[1] = v.y; wd[2] = v.z; wd[3] = v.w;
+                sheaf_classes16(lut, lit_at(L, (int64_t)i0 - 2), lit_at(L, (int64_t)i0 - 1), wd, cls);
+                for (uint32_t t = 0; t < 16u && i0 + t < n; t++) mine[(cls[t] >> (2u * m)) & 3u]++;
+            }
+            const uint32_t lo = mine[0] | (mine[1] << 16), hi = mine[2] | (mine[3] << 16);
+            const uint32_t lin = wave_scan_incl(lo), hin = wave_scan_incl(hi);
+            if (lane == 63u) { part[w][0] = lin; part[w][1] = hin; }
+            __syncthreads();
+            uint32_t blo = 0, bhi = 0, tlo = 0, thi = 0;
+            for (uint32_t k = 0; k < SHEAF_NT / 64u; k++) {
+                if (k < w) { blo += part[k][0]; bhi += part[k][1]; }
+                tlo += part[k][0];
+                thi += part[k][1];
+            }
+            __syncthreads();
+            const uint32_t elo = blo + lin - lo, ehi = bhi + hin - hi;
+            uint32_t pos[4] = {fs[0] + run[0] + (elo & 0xFFFFu), fs[1] + run[1] + (elo >> 16),
+                               fs[2] + run[2] + (ehi & 0xFFFFu), fs[3] + run[3] + (ehi >> 16)};
+            for (uint32_t t = 0; t < 16u && i0 + t < n; t++)
+                fib[pos[(cls[t] >> (2u * m)) & 3u]++] = (uint8_t)(wd[t >> 2] >> (8u * (t & 3u)));
+            run[0] += tlo & 0xFFFFu;
+            run[1] += tlo >> 16;
+            run[2] += thi & 0xFFFFu;
+            run[3] += thi >> 16;
         }
     }
-    if (l[w] > 0)
-        normalize_wave(cnt[w], key[w], frq[w], (przm_ftab_t *)(sb + PRZM_SWS_FTAB) + 4u * m + w, &H->mat[m][w]);
+}
+
+/* the counting jobs (block, mode, fiber) by fiber length in 256-byte classes, longest first
+ * (one workgroup): a wave of przm_sheaf_count_kernel steps as often as its longest job */
+extern "C" __global__ void __launch_bounds__(1024)
+przm_sheaf_order_kernel(const uint8_t *__restrict__ sws, uint32_t nblocks, uint32_t *__restrict__ perm) {
+    __shared__ uint32_t cnt[256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, njobs = 16u * nblocks;
+    auto cls = [&](uint32_t j) -> uint32_t {
+        const przm_sheaf_t *H = (const przm_sheaf_t *)(sws + (uint64_t)(j >> 4) * PRZM_SWS_BYTES + PRZM_SWS_HDR);
+        const uint32_t c = H->l[(j >> 2) & 3u][j & 3u] >> 8;
+        return 255u - (c < 255u ? c : 255u);
+    };
+    if (tid < 256) cnt[tid] = 0;
     __syncthreads();
-    /* scatter pass: a literal's place in its fiber is the number of earlier literals of that
-     * fiber; per round one workgroup scan of the four per-thread counts (16 bits each: a round
-     * holds at most 4096 literals) */
-    uint32_t run[4] = {0, 0, 0, 0};
-    for (uint32_t base = 0; base < n; base += 16u * SHEAF_NT) {
-        const uint32_t i0 = base + 16u * tid;
-        uint32_t p2 = lit_at(L, (int64_t)i0 - 2), p1 = lit_at(L, (int64_t)i0 - 1);
-        uint32_t cls[16], mine[4] = {0, 0, 0, 0};
-        for (uint32_t t = 0; t < 16u; t++) {
-            const uint32_t b = i0 + t < n ? L[i0 + t] : 0u;
-            cls[t] = sheaf_class(lut, p2, p1, m);
-            if (i0 + t < n) mine[cls[t]]++;
-            p2 = p1;
-            p1 = b;
+    for (uint32_t j = tid; j < njobs; j += blockDim.x) atomicAdd(&cnt[cls(j)], 1u);
+    __syncthreads();
+    if (tid < 64) { /* exclusive scan of the 256 counts, four per lane */
+        uint32_t c4[4], t = 0;
+        for (int k = 0; k < 4; k++) { c4[k] = cnt[lane * 4 + k]; t += c4[k]; }
+        uint32_t x = t;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= (uint32_t)o) x += y;
         }
-        const uint32_t lo = mine[0] | (mine[1] << 16), hi = mine[2] | (mine[3] << 16);
-        const uint32_t lin = wave_scan_incl(lo), hin = wave_scan_incl(hi);
-        if ((tid & 63u) == 63u) { part[w][0] = lin; part[w][1] = hin; }
-        __syncthreads();
-        uint32_t blo = 0, bhi = 0, tlo = 0, thi = 0;
-        for (uint32_t k = 0; k < SHEAF_NT / 64u; k++) {
-            if (k < w) { blo += part[k][0]; bhi += part[k][1]; }
-            tlo += part[k][0];
-            thi += part[k][1];
-        }
-        __syncthreads();
-        const uint32_t elo = blo + lin - lo, ehi = bhi + hin - hi;
-        uint32_t pos[4] = {fs[0] + run[0] + (elo & 0xFFFFu), fs[1] + run[1] + (elo >> 16),
-                           fs[2] + run[2] + (ehi & 0xFFFFu), fs[3] + run[3] + (ehi >> 16)};
-        for (uint32_t t = 0; t < 16u && i0 + t < n; t++) fib[pos[cls[t]]++] = L[i0 + t];
-        run[0] += tlo & 0xFFFFu;
-        run[1] += tlo >> 16;
-        run[2] += thi & 0xFFFFu;
-        run[3] += thi >> 16;
+        uint32_t run = x - t;
+        for (int k = 0; k < 4; k++) { cnt[lane * 4 + k] = run; run += c4[k]; }
     }
+    __syncthreads();
+    for (uint32_t j = tid; j < njobs; j += blockDim.x) perm[atomicAdd(&cnt[cls(j)], 1u)] = j;
 }
 
 /*
@@ -194,13 +251,15 @@ __device__ uint32_t vans_count_quad(const uint8_t *__restrict__ src, uint32_t n,
     return tot;
 }
 
-/* one quad per (block, mode, fiber): job = blockIdx.x * 16 + quad, block = job >> 4 */
+/* one quad per (block, mode, fiber) job, 16 jobs a workgroup in the order perm gives
+ * (przm_sheaf_order_kernel); job = block * 16 + mode * 4 + fiber */
 extern "C" __global__ void __launch_bounds__(64)
-przm_sheaf_count_kernel(uint8_t *__restrict__ sws, uint32_t nblocks) {
+przm_sheaf_count_kernel(uint8_t *__restrict__ sws, uint32_t nblocks, const uint32_t *__restrict__ perm) {
     __shared__ uint32_t vt[16][256];
     const uint32_t q = threadIdx.x >> 2, j = threadIdx.x & 3u;
-    const uint32_t job = blockIdx.x * 16u + q, blk = job >> 4, m = (job >> 2) & 3u, k = job & 3u;
-    const bool has = blk < nblocks;
+    const uint32_t slot = blockIdx.x * 16u + q;
+    const bool has = slot < 16u * nblocks;
+    const uint32_t job = has ? perm[slot] : 0u, blk = job >> 4, m = (job >> 2) & 3u, k = job & 3u;
     uint8_t *sb = sws + (uint64_t)(has ? blk : 0u) * PRZM_SWS_BYTES;
     przm_sheaf_t *H = (przm_sheaf_t *)(sb + PRZM_SWS_HDR);
     uint32_t n = 0, fs = 0, mat = 0;
@@ -208,7 +267,7 @@ przm_sheaf_count_kernel(uint8_t *__restrict__ sws, uint32_t nblocks) {
         n = H->l[m][k];
         for (uint32_t i = 0; i < k; i++) fs += al16(H->l[m][i]);
         mat = H->mat[m][k];
-        if (n) {
+        if (n >= SHEAF_MIN_CODED) {
             const przm_ftab_t *ft = (const przm_ftab_t *)(sb + PRZM_SWS_FTAB) + 4u * m + k;
             for (uint32_t s = j; s < 256; s += 4) vt[q][s] = ft->fc[s];
         }
@@ -217,7 +276,7 @@ przm_sheaf_count_kernel(uint8_t *__restrict__ sws, uint32_t nblocks) {
     if (!has) return;
     /* coded = matrix + 4 states + stream, taken only when shorter than the fiber */
     const uint32_t head = mat + 16u;
-    const uint32_t K = n > head ? vans_count_quad(sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP + fs, n, vt[q],
+    const uint32_t K = n >= SHEAF_MIN_CODED && n > head ? vans_count_quad(sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP + fs, n, vt[q],
                                                   n - head)
                                 : PRZM_ENC_OVERFLOW;
     if (j == 0) H->k[m][k] = K;
