@@ -31,9 +31,10 @@
 
 #define SHEAF_NT 256u
 
-/* the class tables of modes 0-2, by the previous literal (sheaf.go:36-83) */
+/* each byte's kind for the context signature below: 0 white space, 1 '"' or ':', 2 other syntax,
+ * 3 identifier, 4 digit, 5 anything else (the byte classes of sheaf.go:36-102) */
 struct SheafTable {
-    uint8_t c[3][256];
+    uint8_t kind[256];
     static constexpr bool ws(int b) { return b == ' ' || b == '\t' || b == '\r' || b == '\n'; }
     static constexpr bool syntax(int b) {
         return b == '{' || b == '}' || b == '[' || b == ']' || b == '(' || b == ')' || b == '<' || b == '>' ||
@@ -43,24 +44,12 @@ struct SheafTable {
     }
     static constexpr bool ident(int b) { return (b >= 'a' && b <= 'z') || b == '_'; }
     static constexpr bool digit(int b) { return b >= '0' && b <= '9'; }
-    constexpr SheafTable() : c() {
-        for (int b = 0; b < 256; b++) {
-            c[0][b] = ws(b) ? 0 : syntax(b) ? 1 : ident(b) ? 2 : 3;
-            c[1][b] = (b == '"' || b == ':') ? 0 : ident(b) ? 1 : digit(b) ? 2 : 3;
-            c[2][b] = (ws(b) || syntax(b)) ? 0 : 1;
-        }
+    constexpr SheafTable() : kind() {
+        for (int b = 0; b < 256; b++)
+            kind[b] = ws(b) ? 0 : (b == '"' || b == ':') ? 1 : syntax(b) ? 2 : ident(b) ? 3 : digit(b) ? 4 : 5;
     }
 };
 __constant__ SheafTable przm_sheaf_lut = SheafTable();
-
-/* the fiber of a literal after p2, p1 in mode m (mode 3: ContextClassifyOrder2, sheaf.go:86-102) */
-__device__ __forceinline__ uint32_t sheaf_class(const uint8_t (*lut)[256], uint32_t p2, uint32_t p1, uint32_t m) {
-    if (m < 3) return lut[m][p1];
-    const bool d1 = p1 - '0' < 10u, i1 = (p1 - 'a' < 26u) || p1 == '_';
-    if (d1) return ((p2 - '0' < 10u) || p2 == '.' || p2 == '-') ? 0u : 1u;
-    if (i1) return ((p2 - 'a' < 26u) || p2 == '_') ? 2u : 3u;
-    return (p1 == ' ' || p1 == '\t' || p1 == '\r' || p1 == '\n') ? 0u : 1u;
-}
 
 __device__ __forceinline__ uint32_t al16(uint32_t v) { return (v + 15u) & ~15u; }
 
@@ -71,13 +60,38 @@ __device__ __forceinline__ uint32_t lit_at(const uint8_t *L, int64_t i) { return
  * (row mask, one column mask, one weight) and 16 bytes of states (literals.go:622) */
 #define SHEAF_MIN_CODED 22u
 
-/* the four modes' fibers of the 16 literals at i0 (b: those literals, byte t of word t >> 2) */
-__device__ __forceinline__ void sheaf_classes16(const uint8_t (*lut)[256], uint32_t p2, uint32_t p1, const uint32_t *w,
-                                                uint32_t cls[16]) {
+/*
+ * A literal's fibers in all four modes depend on its two previous literals only through one of
+ * eight context signatures: 0 after white space, 1 after '"' or ':', 2 after other syntax, 3 / 4
+ * after an identifier byte preceded by one / by anything else, 5 / 6 after a digit preceded by a
+ * digit, '.' or '-' / by anything else, 7 after anything else.  Its fiber in mode m is
+ * sheaf_fiber(m, signature): 2 bits per signature of the mode's table (mode 0 ws 0, syntax 1,
+ * identifier 2, other 3; mode 1 '"' ':' 0, identifier 1, digit 2, other 3; mode 2 ws and syntax 0,
+ * other 1; mode 3 ContextClassifyOrder2, sheaf.go:86-102).
+ */
+#define SHEAF_T0 0xFE94u /* 3 3 3 2 2 1 1 0 (signatures 7..0) */
+#define SHEAF_T1 0xE973u /* 3 2 2 1 1 3 0 3 */
+#define SHEAF_T2 0x5540u /* 1 1 1 1 1 0 0 0 */
+#define SHEAF_T3 0x5394u /* 1 1 0 3 2 1 1 0 */
+__device__ __forceinline__ uint32_t sheaf_fiber(uint32_t m, uint32_t sig) {
+    const uint32_t t = m == 0 ? SHEAF_T0 : m == 1 ? SHEAF_T1 : m == 2 ? SHEAF_T2 : SHEAF_T3;
+    return (t >> (2u * sig)) & 3u;
+}
+__device__ __forceinline__ uint32_t sheaf_sig(const uint8_t *kind, uint32_t p2, uint32_t p1) {
+    const uint32_t k1 = kind[p1], k2 = kind[p2];
+    if (k1 < 3u) return k1;
+    if (k1 == 3u) return k2 == 3u ? 3u : 4u;
+    if (k1 == 4u) return (k2 == 4u || p2 == '.' || p2 == '-') ? 5u : 6u;
+    return 7u;
+}
+
+/* the context signatures of the 16 literals at i0 (w: those literals, byte t of word t >> 2) */
+__device__ __forceinline__ void sheaf_sigs16(const uint8_t *kind, uint32_t p2, uint32_t p1, const uint32_t *w,
+                                             uint32_t sig[16]) {
 #pragma unroll
     for (uint32_t t = 0; t < 16u; t++) {
         const uint32_t b = (w[t >> 2] >> (8u * (t & 3u))) & 0xFFu;
-        cls[t] = lut[0][p1] | (lut[1][p1] << 2) | (lut[2][p1] << 4) | (sheaf_class(lut, p2, p1, 3) << 6);
+        sig[t] = sheaf_sig(kind, p2, p1);
         p2 = p1;
         p1 = b;
     }
@@ -85,14 +99,16 @@ __device__ __forceinline__ void sheaf_classes16(const uint8_t (*lut)[256], uint3
 
 /*
  * One workgroup per block, all four modes: a counting pass over the literals (16 a thread and
- * round, loaded as one 16-byte word) fills 16 histograms (mode, fiber); the fiber lengths come
- * from them, and each wave normalises the tables of four of them (the fibers long enough to be
- * coded); a scatter pass per mode then writes the fibers.
+ * round, loaded as one 16-byte word) fills one histogram per context signature; the 16
+ * histograms (mode, fiber) are sums of those rows, the fiber lengths come from them, and each
+ * wave normalises the tables of four of them (the fibers long enough to be coded); a scatter
+ * pass per mode then writes the fibers.
  */
 extern "C" __global__ void __launch_bounds__(SHEAF_NT)
 przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__restrict__ meta,
                         uint8_t *__restrict__ sws, uint32_t nblocks) {
-    __shared__ uint8_t lut[3][256];
+    __shared__ uint8_t kind[256];
+    __shared__ uint32_t hs[8][256];
     __shared__ uint32_t cnt[16][256];
     __shared__ uint32_t frq[SHEAF_NT / 64u][256];
     __shared__ uint32_t part[SHEAF_NT / 64u][2];
@@ -103,22 +119,32 @@ przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__res
     const uint32_t n = meta[blk].len[PRZM_S_LIT];
     uint8_t *sb = sws + (uint64_t)blk * PRZM_SWS_BYTES;
     przm_sheaf_t *H = (przm_sheaf_t *)(sb + PRZM_SWS_HDR);
-    for (uint32_t i = tid; i < 3 * 256; i += SHEAF_NT) (&lut[0][0])[i] = (&przm_sheaf_lut.c[0][0])[i];
-    for (uint32_t i = tid; i < 16 * 256 / 4; i += SHEAF_NT) ((uint4 *)&cnt[0][0])[i] = make_uint4(0, 0, 0, 0);
+    for (uint32_t i = tid; i < 256; i += SHEAF_NT) kind[i] = przm_sheaf_lut.kind[i];
+    for (uint32_t i = tid; i < 8 * 256 / 4; i += SHEAF_NT) ((uint4 *)&hs[0][0])[i] = make_uint4(0, 0, 0, 0);
     __syncthreads();
-    /* counting pass */
+    /* counting pass: one histogram per context signature */
     for (uint32_t base = 0; base < n; base += 16u * SHEAF_NT) {
         const uint32_t i0 = base + 16u * tid;
         if (i0 >= n) continue;
         const uint4 v = *(const uint4 *)(L + i0);
         const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
-        uint32_t cls[16];
-        sheaf_classes16(lut, lit_at(L, (int64_t)i0 - 2), lit_at(L, (int64_t)i0 - 1), wd, cls);
-        for (uint32_t t = 0; t < 16u && i0 + t < n; t++) {
-            const uint32_t b = (wd[t >> 2] >> (8u * (t & 3u))) & 0xFFu, c = cls[t];
+        uint32_t sig[16];
+        sheaf_sigs16(kind, lit_at(L, (int64_t)i0 - 2), lit_at(L, (int64_t)i0 - 1), wd, sig);
+        for (uint32_t t = 0; t < 16u && i0 + t < n; t++)
+            atomicAdd(&hs[sig[t]][(wd[t >> 2] >> (8u * (t & 3u))) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    /* the (mode, fiber) histograms: thread s sums bin s over the signatures of each fiber */
+    for (uint32_t sb_ = tid; sb_ < 256; sb_ += SHEAF_NT) {
+        uint32_t h[8], acc[16];
+        for (uint32_t g = 0; g < 8; g++) h[g] = hs[g][sb_];
 #pragma unroll
-            for (uint32_t m = 0; m < 4; m++) atomicAdd(&cnt[4u * m + ((c >> (2u * m)) & 3u)][b], 1u);
-        }
+        for (uint32_t r = 0; r < 16; r++) acc[r] = 0;
+#pragma unroll
+        for (uint32_t m = 0; m < 4; m++)
+#pragma unroll
+            for (uint32_t g = 0; g < 8; g++) acc[4u * m + sheaf_fiber(m, g)] += h[g];
+        for (uint32_t r = 0; r < 16; r++) cnt[r][sb_] = acc[r];
     }
     __syncthreads();
     /* fiber lengths: wave w sums rows w, w + 4, w + 8, w + 12 */
@@ -152,8 +178,9 @@ przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__res
             if (i0 < n) {
                 const uint4 v = *(const uint4 *)(L + i0);
                 wd[0] = v.x; wd[1] = v.y; wd[2] = v.z; wd[3] = v.w;
-                sheaf_classes16(lut, lit_at(L, (int64_t)i0 - 2), lit_at(L, (int64_t)i0 - 1), wd, cls);
-                for (uint32_t t = 0; t < 16u && i0 + t < n; t++) mine[(cls[t] >> (2u * m)) & 3u]++;
+                sheaf_sigs16(kind, lit_at(L, (int64_t)i0 - 2), lit_at(L, (int64_t)i0 - 1), wd, cls);
+                for (uint32_t t = 0; t < 16u; t++) cls[t] = sheaf_fiber(m, cls[t]);
+                for (uint32_t t = 0; t < 16u && i0 + t < n; t++) mine[cls[t]]++;
             }
             const uint32_t lo = mine[0] | (mine[1] << 16), hi = mine[2] | (mine[3] << 16);
             const uint32_t lin = wave_scan_incl(lo), hin = wave_scan_incl(hi);
@@ -170,7 +197,7 @@ przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__res
             uint32_t pos[4] = {fs[0] + run[0] + (elo & 0xFFFFu), fs[1] + run[1] + (elo >> 16),
                                fs[2] + run[2] + (ehi & 0xFFFFu), fs[3] + run[3] + (ehi >> 16)};
             for (uint32_t t = 0; t < 16u && i0 + t < n; t++)
-                fib[pos[(cls[t] >> (2u * m)) & 3u]++] = (uint8_t)(wd[t >> 2] >> (8u * (t & 3u)));
+                fib[pos[cls[t]]++] = (uint8_t)(wd[t >> 2] >> (8u * (t & 3u)));
             run[0] += tlo & 0xFFFFu;
             run[1] += tlo >> 16;
             run[2] += thi & 0xFFFFu;
