@@ -49,6 +49,18 @@ int64_t przm_decode_frame(const uint8_t *src, size_t n, uint8_t *dst, size_t cap
 int64_t przm_decode_frames(const uint8_t *src, size_t n, const uint32_t *sizes, size_t nframes, uint8_t *dst,
                            size_t cap);
 
+/* A PRZS multi-frame stream (decompress.go:842-866: 'P','R','Z','S', flags u8,
+ * u32 chunk count, u64 total uncompressed size; then per chunk its u32 length
+ * and one frame, cmd/przm-viz/analyzer.go:242-256), or a single frame: the
+ * total uncompressed size from the stream header (the frame's size for a frame),
+ * or an error. */
+int64_t przm_decode_stream_size(const uint8_t *src, size_t n);
+/* The whole stream (exactly n bytes: header, then every chunk) or a single
+ * frame into dst; the bytes written, or an error: PRZM_DEC_ERR_SIZE when the
+ * chunks do not fill n or do not sum to the header's total, and
+ * PRZM_DEC_ERR_UNSUPPORTED for a stream flag other than 0 (none is defined). */
+int64_t przm_decode_stream(const uint8_t *src, size_t n, uint8_t *dst, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -292,6 +292,26 @@ int32_t przm_gpu_set_offsets_mode(przm_gpu *g, int32_t mode);
  * przm_gpu_encode_device_noalloc supports it within its fixed limit. */
 int32_t przm_gpu_set_literals_mode(przm_gpu *g, int32_t mode);
 
+/* ---- the PRZS multi-frame stream (decompress.go:842-866 ParseStreamHeader) ---- */
+#define PRZM_CONTAINER_FRAMES 0 /* frames back to back, sizes beside them (the default) */
+#define PRZM_CONTAINER_STREAM 1 /* a buffer of more than one block as one PRZS stream */
+/* How the encode entry points of this handle lay out a buffer from now on.
+ * PRZM_CONTAINER_STREAM writes every buffer longer than one 64 KiB block as the
+ * reference's multi-chunk stream (what CompressWithOptions writes with
+ * Threads > 1 and ChunkSize = 64 KiB): the 17-byte header 'P','R','Z','S',
+ * flags 0, u32 chunk count, u64 total uncompressed size (decompress.go:849-866),
+ * then per block its u32 frame length and the frame (the chunk walk of
+ * cmd/przm-viz/analyzer.go:242-256); the frames are those of the default layout.
+ * A buffer of one block or none stays a bare frame, as the reference writes a
+ * single frame for input that fits one chunk.  frame_sizes then give the bytes
+ * each block occupies (the first block of a stream includes the header, every
+ * block its prefix) and out_off the start of each buffer's stream.  The
+ * serving loop always writes frames.  Buffers above 2^24 - 1 blocks are
+ * refused.  Set it while no call is running on the handle. */
+int32_t przm_gpu_set_container(przm_gpu *g, int32_t mode);
+/* Upper bound of the output bytes of an n-byte buffer in the stream layout. */
+uint64_t przm_gpu_stream_bound(uint64_t n);
+
 /* ---- device memory for callers without an allocator ---- */
 int32_t przm_gpu_malloc(przm_gpu *g, uint64_t bytes, void **d);
 int32_t przm_gpu_free(przm_gpu *g, void *d);

@@ -105,6 +105,8 @@ SIGNATURES = {
     "przm_gpu_set_kernel_shape": (I32, [P, I32, I32]),
     "przm_gpu_set_offsets_mode": (I32, [P, I32]),
     "przm_gpu_set_literals_mode": (I32, [P, I32]),
+    "przm_gpu_set_container": (I32, [P, I32]),
+    "przm_gpu_stream_bound": (U64, [U64]),
     "przm_gpu_get_tuning": (I32, [P, ctypes.POINTER(I32)]),
     "przm_gpu_set_options": (I32, [P, I32]),
     "przm_gpu_malloc": (I32, [P, U64, ctypes.POINTER(P)]),

@@ -32,6 +32,15 @@ typedef struct {
 } przm_block_t;
 
 #define PRZM_BF_RAW 1u      /* emit an uncompressed frame (high-entropy sample or empty) */
+/* PRZS multi-frame stream of a buffer of more than one block (decompress.go:842-866 header,
+ * cmd/przm-viz/analyzer.go:242-256 chunk walk): every block's frame is a chunk behind its u32
+ * length, and the buffer's first block also carries the 17-byte stream header before it,
+ * with the buffer's chunk count in the flag word's high bits */
+#define PRZM_BF_CHUNK 2u    /* the frame is a length-prefixed chunk of a PRZS stream */
+#define PRZM_BF_HEAD 4u     /* the stream header precedes this chunk */
+#define PRZM_BF_COUNT_SHIFT 8u
+#define PRZM_PRZS_HDR 17u   /* 'P','R','Z','S', flags u8, chunk count u32, total size u64 */
+#define PRZM_PRZS_PREFIX 4u /* u32 chunk length */
 
 /* Byte offsets of the per-block stream slots inside one block workspace. */
 #define PRZM_WS_CMD 0u

@@ -19,6 +19,8 @@
 #include "przm_meta8_freq.h"
 
 #define MAGIC 0x4D5A5250u    /* "PRZM" little-endian */
+#define MAGIC_S 0x535A5250u  /* "PRZS": a multi-frame stream (decompress.go:25) */
+#define STREAM_HDR 17u        /* decompress.go:849-866 */
 #define MAX_FRAME (16u << 20) /* MaxBlockSize (api.go:22) */
 #define F_STRIDE 0x01u
 #define F_DELTA 0x02u
@@ -658,4 +660,39 @@ int64_t przm_decode_frames(const uint8_t *src, size_t n, const uint32_t *sizes, 
         dp += (size_t)r;
     }
     return sp == n ? (int64_t)dp : PRZM_DEC_ERR_CORRUPT;
+}
+
+/* ---------------- PRZS stream ---------------- */
+
+int64_t przm_decode_stream_size(const uint8_t *src, size_t n) {
+    if (!src || n < 8) return PRZM_DEC_ERR_CORRUPT;
+    if (le32(src) == MAGIC) return przm_decode_size(src, n);
+    if (le32(src) != MAGIC_S) return PRZM_DEC_ERR_MAGIC;
+    if (n < STREAM_HDR) return PRZM_DEC_ERR_CORRUPT; /* ParseStreamHeader, decompress.go:851-853 */
+    const uint64_t total = (uint64_t)le32(src + 9) | (uint64_t)le32(src + 13) << 32;
+    return total > (uint64_t)INT64_MAX ? PRZM_DEC_ERR_CORRUPT : (int64_t)total;
+}
+
+int64_t przm_decode_stream(const uint8_t *src, size_t n, uint8_t *dst, size_t cap) {
+    if (!src || n < 8) return PRZM_DEC_ERR_CORRUPT;
+    if (le32(src) == MAGIC) return przm_decode_frame(src, n, dst, cap);
+    const int64_t total = przm_decode_stream_size(src, n);
+    if (total < 0) return total;
+    if (src[4] != 0) return PRZM_DEC_ERR_UNSUPPORTED; /* no stream flag is defined */
+    if ((uint64_t)total > cap) return PRZM_DEC_ERR_CAPACITY;
+    const uint32_t count = le32(src + 5);
+    /* the chunks: u32 length, then that many bytes of one frame (cmd/przm-viz/analyzer.go:242-256) */
+    size_t sp = STREAM_HDR, dp = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        if (n - sp < 4) return PRZM_DEC_ERR_CORRUPT;
+        const uint32_t len = le32(src + sp);
+        sp += 4;
+        if (len > n - sp) return PRZM_DEC_ERR_CORRUPT;
+        const int64_t r = przm_decode_frame(src + sp, len, dst ? dst + dp : NULL, cap - dp);
+        if (r < 0) return r;
+        sp += len;
+        dp += (size_t)r;
+    }
+    if (sp != n || dp != (uint64_t)total) return PRZM_DEC_ERR_SIZE;
+    return (int64_t)dp;
 }

@@ -114,6 +114,8 @@ __device__ void normalize_wave(const uint32_t *cnt, uint32_t *frq, przm_ftab_t *
         f4[i] = c4[i] ? 1u + (uint32_t)(((uint64_t)c4[i] * rem) / (tot ? tot : 1u)) : 0u;
         sum += f4[i];
         const uint32_t s = lane * 4 + i;
+        /* a count reaches 2^24 only in a one-symbol table of exactly 16 MiB (the largest input of
+         * the stage entry points), whose deficit is 0 */
         k4[i] = c4[i] >= (1u << 24) ? 0xFFFFFF00u | (255u - s) : (c4[i] << 8) | (255u - s);
     }
     sum = wave_sum(sum);
@@ -720,7 +722,9 @@ przm_frame_size_kernel(const przm_block_t *__restrict__ blocks, przm_meta_t *__r
     mt->s3_mode = P.s3m;
     mt->s3_code = P.s3c;
     mt->s3_code_len = P.s3cl;
-    frame_len[blk] = fl;
+    /* a PRZS chunk also takes its length prefix, and the stream's first chunk the header */
+    frame_len[blk] = fl + ((bd.flags & PRZM_BF_CHUNK) ? PRZM_PRZS_PREFIX : 0u) +
+                     ((bd.flags & PRZM_BF_HEAD) ? PRZM_PRZS_HDR : 0u);
 }
 
 /* Exclusive scan of frame lengths into 64-bit offsets; base/total carried in *run
@@ -822,6 +826,22 @@ przm_frame_write_kernel(const uint8_t *__restrict__ src, const przm_block_t *__r
     const przm_ftab_t *ft = ftab + (uint64_t)blk * 4;
     uint8_t *o = out + frame_off[blk];
     const uint32_t n = bd.len;
+    if (bd.flags & PRZM_BF_HEAD) { /* the PRZS stream header (decompress.go:849-866) */
+        if (threadIdx.x == 0) {
+            /* the buffer's blocks follow this one in the table, all full but the last */
+            const uint32_t cnt = bd.flags >> PRZM_BF_COUNT_SHIFT;
+            const uint64_t total = (uint64_t)(cnt - 1u) * PRZM_BLOCK + blocks[blk + cnt - 1u].len;
+            o[0] = 'P'; o[1] = 'R'; o[2] = 'Z'; o[3] = 'S'; o[4] = 0;
+            wr_le32(o + 5, cnt);
+            wr_le32(o + 9, (uint32_t)total);
+            wr_le32(o + 13, (uint32_t)(total >> 32));
+        }
+        o += PRZM_PRZS_HDR;
+    }
+    if (bd.flags & PRZM_BF_CHUNK) { /* the chunk's length (cmd/przm-viz/analyzer.go:243-246) */
+        if (threadIdx.x == 0) wr_le32(o, mt->frame_len);
+        o += PRZM_PRZS_PREFIX;
+    }
     if (mt->frame_kind != 0) { /* compress.go:973-981 */
         if (threadIdx.x == 0) {
             o[0] = 'P'; o[1] = 'R'; o[2] = 'Z'; o[3] = 'M'; o[4] = 0x04;

@@ -214,6 +214,7 @@ struct przm_gpu {
     uint32_t rans_nt = 64;     /* coder threads per workgroup: 64 or 128 (which coder kernel) */
     bool s3_prefix = false;    /* Section 3 as PackageOffsetsSection chooses it (else flat LEB128) */
     bool s4_sheaf = false;     /* Section 4 as PackageLiteralsSection chooses it (else V-ANS or raw) */
+    bool przs = false;         /* a buffer of more than one block as one PRZS stream (else bare frames) */
     bool use_graph = true;
     Ctx c;
     std::mutex mu;
@@ -254,8 +255,12 @@ int32_t mock_check() {
     return 0;
 }
 
+/* The block table of the buffers.  stream: every buffer of more than one block becomes one PRZS
+ * stream (its blocks are length-prefixed chunks, its first block carries the header and the chunk
+ * count); a buffer of one block (or none) stays a bare frame, as the reference writes a single
+ * frame for input that fits one chunk. */
 void build_blocks(const przm_span *bufs, uint32_t nbufs, std::vector<przm_block_t> &blocks,
-                  std::vector<uint64_t> &first) {
+                  std::vector<uint64_t> &first, bool stream = false) {
     blocks.clear();
     first.assign(nbufs + 1, 0);
     for (uint32_t i = 0; i < nbufs; i++) {
@@ -265,12 +270,32 @@ void build_blocks(const przm_span *bufs, uint32_t nbufs, std::vector<przm_block_
             blocks.push_back({bufs[i].offset, 0, PRZM_BF_RAW});
             continue;
         }
+        const uint64_t cnt = (n + PRZM_BLOCK - 1) / PRZM_BLOCK;
+        const uint32_t sf = stream && cnt > 1 ? PRZM_BF_CHUNK : 0u;
         for (uint64_t o = 0; o < n; o += PRZM_BLOCK) {
             const uint32_t l = (uint32_t)std::min<uint64_t>(PRZM_BLOCK, n - o);
-            blocks.push_back({bufs[i].offset + o, l, 0});
+            const uint32_t hf = sf && o == 0 ? (PRZM_BF_HEAD | (uint32_t)(cnt << PRZM_BF_COUNT_SHIFT)) : 0u;
+            blocks.push_back({bufs[i].offset + o, l, sf | hf});
         }
     }
     first[nbufs] = blocks.size();
+}
+
+/* output bytes of a block: its frame's bound, plus the stream prefix and header it carries */
+uint64_t block_bound(const przm_block_t &b) {
+    return 8ull + b.len + ((b.flags & PRZM_BF_CHUNK) ? PRZM_PRZS_PREFIX : 0u) +
+           ((b.flags & PRZM_BF_HEAD) ? PRZM_PRZS_HDR : 0u);
+}
+
+/* the stream header's chunk count is 24 bits of the flag word: 2^24 - 1 blocks (just under 1 TiB) */
+int32_t check_stream_sizes(const przm_span *bufs, uint32_t nbufs) {
+    for (uint32_t i = 0; i < nbufs; i++)
+        if (bufs[i].len > (uint64_t)((1u << (32 - PRZM_BF_COUNT_SHIFT)) - 1u) * PRZM_BLOCK) {
+            snprintf(g_err, sizeof g_err, "buffer %u: %llu bytes exceed a PRZS stream's chunk count", i,
+                     (unsigned long long)bufs[i].len);
+            return PRZM_GPU_ERR_CAPACITY;
+        }
+    return 0;
 }
 
 /* device buffers for nblocks blocks processed in chunks of `chunk`; offs / sheaf: the per-block
@@ -418,10 +443,11 @@ int32_t run_batch(przm_gpu *g, Ctx &c, const uint8_t *d_src, const przm_span *bu
                   uint8_t *d_out, uint64_t out_cap, bool use_graph, std::vector<przm_block_t> &blocks,
                   std::vector<uint64_t> &first, BatchOut &bo, przm_gpu_profile *prof) {
     g->epoch++;
-    build_blocks(bufs, nbufs, blocks, first);
+    if (g->przs) CHK(check_stream_sizes(bufs, nbufs));
+    build_blocks(bufs, nbufs, blocks, first, g->przs);
     const uint32_t nblocks = (uint32_t)blocks.size();
     uint64_t bound = 0;
-    for (auto &b : blocks) bound += 8ull + b.len;
+    for (auto &b : blocks) bound += block_bound(b);
     if (bound > out_cap) {
         snprintf(g_err, sizeof g_err, "output capacity %llu < bound %llu", (unsigned long long)out_cap,
                  (unsigned long long)bound);
@@ -642,6 +668,18 @@ uint64_t przm_gpu_frames_bound(uint64_t n) {
     return n + 8 * ((n + PRZM_BLOCK - 1) / PRZM_BLOCK);
 }
 
+uint64_t przm_gpu_stream_bound(uint64_t n) {
+    if (n <= PRZM_BLOCK) return przm_gpu_frames_bound(n);
+    return przm_gpu_frames_bound(n) + PRZM_PRZS_HDR + PRZM_PRZS_PREFIX * ((n + PRZM_BLOCK - 1) / PRZM_BLOCK);
+}
+
+int32_t przm_gpu_set_container(przm_gpu *g, int32_t mode) {
+    if (!g || (mode != PRZM_CONTAINER_FRAMES && mode != PRZM_CONTAINER_STREAM)) return PRZM_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->przs = mode == PRZM_CONTAINER_STREAM;
+    return 0;
+}
+
 int32_t przm_gpu_set_tuning(przm_gpu *g, int32_t lz_grid, int32_t chunk_blocks) {
     if (!g) return PRZM_GPU_ERR_ARG;
     std::lock_guard<std::mutex> lk(g->mu);
@@ -803,16 +841,21 @@ int32_t przm_gpu_encode_device_noalloc(przm_gpu *g, const void *d_src, uint64_t 
         return PRZM_GPU_ERR_CAPACITY;
     }
     const uint32_t nb = n ? (uint32_t)((n + PRZM_BLOCK - 1) / PRZM_BLOCK) : 1u;
-    if (nb > frames_cap || przm_gpu_frames_bound(n) > out_cap) return PRZM_GPU_ERR_CAPACITY;
     std::lock_guard<std::mutex> lk(g->mu);
+    const bool stream = g->przs && nb > 1;
+    if (nb > frames_cap || (stream ? przm_gpu_stream_bound(n) : przm_gpu_frames_bound(n)) > out_cap)
+        return PRZM_GPU_ERR_CAPACITY;
     HIPCHK(hipSetDevice(g->dev));
     uint8_t *base = (uint8_t *)g_na_ptr[g->dev];
     if (!base) return PRZM_GPU_ERR_UNAVAILABLE;
     /* the block table on the stack, uploaded into the array (no device allocation) */
     przm_block_t blocks[NA_BLOCKS];
     if (n == 0) blocks[0] = {0, 0, PRZM_BF_RAW};
-    for (uint32_t b = 0; b < nb && n; b++)
-        blocks[b] = {(uint64_t)b * PRZM_BLOCK, (uint32_t)std::min<uint64_t>(PRZM_BLOCK, n - (uint64_t)b * PRZM_BLOCK), 0};
+    for (uint32_t b = 0; b < nb && n; b++) {
+        const uint32_t sf = stream ? (PRZM_BF_CHUNK | (b == 0 ? PRZM_BF_HEAD | (nb << PRZM_BF_COUNT_SHIFT) : 0u)) : 0u;
+        blocks[b] = {(uint64_t)b * PRZM_BLOCK, (uint32_t)std::min<uint64_t>(PRZM_BLOCK, n - (uint64_t)b * PRZM_BLOCK),
+                     sf};
+    }
     /* a context whose buffers are views into the array: never ensure()d or release()d */
     Ctx v;
     v.st = g->c.st;
@@ -864,7 +907,11 @@ int32_t przm_gpu_encode_batch(przm_gpu *g, const uint8_t *src, const przm_span *
         nb += bufs[i].len ? (bufs[i].len + PRZM_BLOCK - 1) / PRZM_BLOCK : 1;
     }
     if (frame_sizes && nb > frames_cap) return PRZM_GPU_ERR_CAPACITY;
-    const uint64_t bound = in_bytes + 8 * nb;
+    uint64_t bound = in_bytes + 8 * nb;
+    if (g->przs) /* chunk prefixes and stream headers */
+        for (uint32_t i = 0; i < nbufs; i++)
+            if (bufs[i].len > PRZM_BLOCK)
+                bound += PRZM_PRZS_HDR + PRZM_PRZS_PREFIX * ((bufs[i].len + PRZM_BLOCK - 1) / PRZM_BLOCK);
     if (bound > out_cap && out) {
         /* the exact size is known only afterwards; require the bound like the device form */
     }

@@ -326,7 +326,11 @@ int32_t przm_gpu_vans_encode_device(przm_gpu *g, const void *d_in, void *d_out, 
 
 int32_t przm_gpu_vans_encode(przm_gpu *g, const uint8_t *src, uint64_t n, uint8_t *out, uint64_t cap,
                              uint64_t *out_len, przm_gpu_profile *prof) {
-    if (!g || !out || !out_len || (!src && n) || n > (16ull << 20)) return PRZM_GPU_ERR_ARG;
+    if (!g || !out || !out_len || (!src && n)) return PRZM_GPU_ERR_ARG;
+    if (n > (16ull << 20)) { /* MaxBlockSize, api.go:22 */
+        snprintf(g_err, sizeof g_err, "przm_gpu_vans_encode: %llu bytes above 16 MiB", (unsigned long long)n);
+        return PRZM_GPU_ERR_ARG;
+    }
     CHK(mock_check());
     if (n == 0) { *out_len = 0; return 0; }
     std::lock_guard<std::mutex> lk(g->mu);

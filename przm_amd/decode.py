@@ -38,6 +38,10 @@ def lib():
         l.przm_decode_frame.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t]
         l.przm_decode_frames.restype = ctypes.c_int64
         l.przm_decode_frames.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, P, ctypes.c_size_t]
+        l.przm_decode_stream_size.restype = ctypes.c_int64
+        l.przm_decode_stream_size.argtypes = [P, ctypes.c_size_t]
+        l.przm_decode_stream.restype = ctypes.c_int64
+        l.przm_decode_stream.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t]
         _lib = l
     return _lib
 
@@ -81,6 +85,30 @@ def DecompressFrames(frames, sizes, total=None) -> bytes:
             pos += int(n)
     out = np.zeros(max(total, 1), np.uint8)
     r = lib().przm_decode_frames(_p(f), f.size, _p(s), s.size, _p(out), out.size)
+    if r < 0:
+        raise DecodeError(r)
+    return out[:r].tobytes()
+
+
+def ParseStreamHeader(stream):
+    """(chunk count, total uncompressed size, flags) of a PRZS stream's 17-byte header
+    (decompress.go:849-866 ParseStreamHeader)."""
+    f = _u8(stream)
+    if f.size < 17:
+        raise DecodeError(ERR_CORRUPT)
+    if int.from_bytes(f[:4].tobytes(), "little") != 0x535A5250:
+        raise DecodeError(ERR_MAGIC)
+    return (int.from_bytes(f[5:9].tobytes(), "little"), int.from_bytes(f[9:17].tobytes(), "little"), int(f[4]))
+
+
+def Decompress(data) -> bytes:
+    """A PRZS stream or a single frame -> the input (Decompress of decompress.go for these two forms)."""
+    f = _u8(data)
+    n = lib().przm_decode_stream_size(_p(f), f.size)
+    if n < 0:
+        raise DecodeError(n)
+    out = np.zeros(max(int(n), 1), np.uint8)
+    r = lib().przm_decode_stream(_p(f), f.size, _p(out), out.size)
     if r < 0:
         raise DecodeError(r)
     return out[:r].tobytes()

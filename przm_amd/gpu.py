@@ -33,6 +33,11 @@ OffsetsPrefix = 1  # the distance-prefix container when shorter
 # Section 4 of the frames (GPUCompressor.SetLiteralsMode)
 LiteralsVANS = 0   # adaptive V-ANS when shorter, else raw
 LiteralsSheaf = 1  # also the four sheaf containers
+# Layout of an encoded buffer (GPUCompressor.SetContainer)
+ContainerFrames = 0  # frames back to back, sizes beside them
+ContainerStream = 1  # a buffer of more than one block as one PRZS stream
+MagicPRZS = 0x535A5250  # decompress.go:25
+StreamHeaderSize = 17   # decompress.go:849-866
 # bridge.go:148-153, the reference tokenizer's tile and per-tile stream slots.  Here the tile is
 # the 64 KiB block (window and extension bounded by it), so the reference's GPU tests that
 # expect a 1 MiB window and a 4095 length cap (gpu_tier2_boundaries_test.go) see 64 KiB and 1024.
@@ -311,6 +316,19 @@ class GPUCompressor:
         containers, as the reference CPU path writes it)."""
         self._check(self._lib.przm_gpu_set_literals_mode(self._h, int(mode)))
 
+    def SetContainer(self, mode: int) -> None:
+        """Layout of the buffers encoded from now on: ContainerFrames (frames back to back, the
+        default) or ContainerStream (a buffer of more than one 64 KiB block as one PRZS stream:
+        the header of decompress.go:849-866, then each frame behind its u32 length, as the
+        reference writes multi-chunk input)."""
+        self._check(self._lib.przm_gpu_set_container(self._h, int(mode)))
+        self._container = int(mode)
+
+    def _bound(self, n: int) -> int:
+        if getattr(self, "_container", ContainerFrames) == ContainerStream:
+            return int(self._lib.przm_gpu_stream_bound(n))
+        return int(self._lib.przm_gpu_frames_bound(n))
+
     def _check(self, rc):
         if rc:
             raise GPUError(rc)
@@ -547,10 +565,11 @@ class GPUCompressor:
 
     # -- encode --
     def CompressFrames(self, src, level: int = 6) -> Tuple[bytes, np.ndarray, GPUProfile]:
-        """Frames of src (one per 64 KiB block) and their sizes."""
+        """Frames of src (one per 64 KiB block) and their sizes; with ContainerStream and more than
+        one block, the PRZS stream of them and the bytes each block occupies in it."""
         self._mock()
         a = _u8(src)
-        bound = int(self._lib.przm_gpu_frames_bound(a.size))
+        bound = self._bound(a.size)
         out = np.zeros(bound + 64, np.uint8)
         nb = max((a.size + BlockSize - 1) // BlockSize, 1)
         sizes = np.zeros(nb, np.uint32)
@@ -578,7 +597,7 @@ class GPUCompressor:
         for i, a in enumerate(arrs):
             spans[i].offset, spans[i].len = int(offs[i]), a.size
             nb += max((a.size + BlockSize - 1) // BlockSize, 1)
-            bound += int(self._lib.przm_gpu_frames_bound(a.size))
+            bound += self._bound(a.size)
         out = np.zeros(bound + 64, np.uint8)
         out_off = np.zeros(len(arrs) + 1, np.uint64)
         sizes = np.zeros(max(nb, 1), np.uint32)

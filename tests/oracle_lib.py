@@ -284,7 +284,15 @@ class RefDecoder:
         l.przm_split_commands.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
                                           P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P, ctypes.c_size_t,
                                           ctypes.POINTER(ctypes.c_size_t)]
+        l.przm_get_uncompressed_size.restype = ctypes.c_int32
+        l.przm_get_uncompressed_size.argtypes = [P, ctypes.c_size_t]
         self.l = l
+
+    def uncompressed_size(self, data):
+        """przm_get_uncompressed_size (src/przm_dec.c:1554-1600): for a PRZS stream it reads the
+        u32 at byte 5 (:1564-1567)."""
+        f = u8(data)
+        return self.l.przm_get_uncompressed_size(ptr(f), f.size)
 
     def decompress(self, frame, expect_len=None):
         f = u8(frame)
@@ -356,6 +364,25 @@ class RefDecoder:
         if r != 0:
             raise ValueError(f"split error {r}")
         return tuple(b[: l.value] for b, l in zip(bufs, lens))
+
+
+def stream_of(frames, sizes, total):
+    """The PRZS stream of a buffer's frames: the 17-byte header of ParseStreamHeader
+    (decompress.go:849-866: 'P','R','Z','S', flags, u32 chunk count, u64 total uncompressed size),
+    then every frame behind its u32 length (the chunk walk of cmd/przm-viz/analyzer.go:242-256).
+    A buffer of one frame stays that frame, as the reference writes a single frame for input that
+    fits one chunk (tests/stress/stress_test.go:1934-1940, tests/e2e/gpu_tier3_interactions_test.go:62)."""
+    sizes = [int(x) for x in sizes]
+    if len(sizes) <= 1:
+        return bytes(frames)
+    out = [b"PRZS", b"\x00", len(sizes).to_bytes(4, "little"), int(total).to_bytes(8, "little")]
+    pos = 0
+    for n in sizes:
+        out.append(n.to_bytes(4, "little"))
+        out.append(bytes(frames[pos: pos + n]))
+        pos += n
+    assert pos == len(frames)
+    return b"".join(out)
 
 
 _oracle = None

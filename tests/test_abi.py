@@ -81,6 +81,7 @@ DEC_LIB = os.path.join(ROOT, "przm_amd", "libprzm_decode.so")
 def test_decoder_library_exports_its_header():
     text = re.sub(r"/\*.*?\*/", "", open(DEC_HEADER).read(), flags=re.S)
     names = sorted(set(re.findall(r"\b(przm_decode_\w+)\s*\(", text)))
-    assert names == ["przm_decode_frame", "przm_decode_frames", "przm_decode_size"], names
+    assert names == ["przm_decode_frame", "przm_decode_frames", "przm_decode_size", "przm_decode_stream",
+                     "przm_decode_stream_size"], names
     lib = ctypes.CDLL(DEC_LIB)
     assert all(hasattr(lib, n) for n in names)

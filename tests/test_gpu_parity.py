@@ -154,6 +154,19 @@ def test_histogram_and_vans_match(g):
     assert f == b"\x00\x05\x00\x00\x00hello"
 
 
+def test_vans_largest_input(g):
+    """The stage coder takes up to 16 MiB (MaxBlockSize, api.go:22), where a one-symbol table
+    reaches a count of 2^24; one byte more is refused."""
+    from przm_amd import _lib as L
+    from przm_amd.gpu import GPUError
+    a = np.full(16 << 20, 7, np.uint8)
+    v, _ = g.VANSEncode4xHost(a)
+    assert v == oracle().vans(a)
+    with pytest.raises(GPUError) as e:
+        g.VANSEncode4xHost(np.zeros((16 << 20) + 1, np.uint8))
+    assert e.value.code == L.ERR_ARG
+
+
 def test_batched_equals_alone(g):
     """Each buffer's frames are the same alone and in a mixed batch."""
     arena, offs, sizes = D.batch(60, 1024, 1 << 20, 9)
