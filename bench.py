@@ -32,7 +32,10 @@ generator (JSON-like records, int32 tables, motif-random, uniform random),
             batched pass over the first --profile-buffers buffers, nothing
             else: the program the rocprofv3 --pmc passes run.
   cpu_baseline  the CPU oracle of the same mode on the host's cores over a
-            time-bounded sample of the same buffers.
+            time-bounded sample of the same buffers; beside it, config 0 (16 MiB
+            of Markov text as 256 frames) through the C restatement of the
+            reference CPU compressor (the trellis path at level 6) and through
+            the plain GPU path.
 """
 import argparse
 import ctypes
@@ -99,6 +102,30 @@ def cpu_baseline(bufs, seconds, threads=None, level=6):
                 done_bytes += int(bufs[j].size)
     wall = time.perf_counter() - t0
     return done_bytes / wall / 1e9, i, done_bytes, wall
+
+
+def refcpu_leg(src, threads, level=6):
+    """The C restatement of the reference Go CPU compressor (oracle/przm_refcpu.c: the trellis
+    path of compressInternal at `level`, with the static dictionary when its tables are built)
+    on src as independent 64 KiB frames, on `threads` threads and on one thread over the first
+    16 frames; every frame decoded by the reference C99 decoder when it is built."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import refcpu, refdec, have_refdec
+    R = refcpu()
+    t0 = time.perf_counter()
+    frames, sizes = R.compress_frames(src, level, threads)
+    w = time.perf_counter() - t0
+    one = src[: 16 * 65536]
+    t0 = time.perf_counter()
+    R.compress_frames(one, level, 1)
+    w1 = time.perf_counter() - t0
+    ok = None
+    if have_refdec():
+        ok = refdec().decompress_frames(frames, sizes) == src.tobytes()
+    return {"value": round(src.size / w / 1e9, 5), "unit": "GB/s", "threads": threads,
+            "one_thread": round(one.size / w1 / 1e9, 5), "compression_ratio": round(src.size / len(frames), 4),
+            "frames": int(sizes.size), "dictionary": bool(R.dict), "decoded_by_reference_decoder": ok,
+            "wall_s": round(w, 3)}
 
 
 def pmc_traffic():
@@ -429,6 +456,28 @@ def main():
                                 "sample": f"16 MiB order-1 Markov text (seed {args.seed}) as 256 frames of "
                                           f"64 KiB, level {args.level}: {nt} threads {c0w:.2f} s, 1 thread "
                                           f"{c01w:.2f} s"}}
+        # config 0 through the reference CPU path itself (restated in C), and through the GPU
+        ref0 = refcpu_leg(m0, nt, args.level)
+        ref0["label"] = "reference CPU path (C restatement of compressInternal, level 6)"
+        ref0["sample"] = (f"16 MiB order-1 Markov text (seed {args.seed}) as 256 frames of 64 KiB, "
+                          f"{nt} threads; one thread on the first 16 frames")
+        d0 = dmalloc(m0.size)
+        lib.przm_gpu_h2d(g.handle, d0, ctypes.c_void_p(m0.ctypes.data), m0.size)
+        o0 = dmalloc(lib.przm_gpu_frames_bound(m0.size) + 64)
+        olen0, nf0 = ctypes.c_uint64(), ctypes.c_uint64()
+        s0 = np.zeros(256, np.uint32)
+        lib.przm_gpu_encode_device(g.handle, d0, m0.size, args.level, o0, lib.przm_gpu_frames_bound(m0.size) + 64,
+                                   ctypes.byref(olen0), ctypes.c_void_p(s0.ctypes.data), 256, ctypes.byref(nf0), None)
+        p0 = L.GpuProfile()
+        lib.przm_gpu_encode_device(g.handle, d0, m0.size, args.level, o0, lib.przm_gpu_frames_bound(m0.size) + 64,
+                                   ctypes.byref(olen0), ctypes.c_void_p(s0.ctypes.data), 256, ctypes.byref(nf0),
+                                   ctypes.byref(p0))
+        lib.przm_gpu_free(g.handle, d0)
+        lib.przm_gpu_free(g.handle, o0)
+        ref0["gpu_same_input"] = {"value": round(m0.size / (p0.kernel_ms * 1e6), 4), "unit": "GB/s",
+                                  "compression_ratio": round(m0.size / olen0.value, 4),
+                                  "path": "przm_gpu_encode_device, one launch per stage (mode G)"}
+        cpu["reference_cpu_path_markov_16MiB"] = ref0
         log(f"cpu baseline: {cpu}")
 
     # measured copy bandwidth (the STREAM-copy figure beside the 8 TB/s peak)

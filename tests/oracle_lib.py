@@ -9,6 +9,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
 REFDEC = os.path.join(ROOT, "oracle", "_ref", "libprzm_ref_dec.so")
+REFTABLES = os.path.join(ROOT, "oracle", "_ref", "libprzm_ref_tables.so")
 P = ctypes.c_void_p
 U32 = ctypes.c_uint32
 U64 = ctypes.c_uint64
@@ -385,8 +386,121 @@ def stream_of(frames, sizes, total):
     return b"".join(out)
 
 
+class RefInfo(ctypes.Structure):  # orc_ref_info (oracle/przm_refcpu.c)
+    _fields_ = [(k, U32) for k in ("stride", "bmin", "wlog", "uncompressed", "kind", "ncmd", "noff", "nlit",
+                                   "s2", "s3", "s4")] + [("edges", U32 * 7), ("dict", U32)]
+
+
+class RefCPU:
+    """The C restatement of the reference Go CPU compressor (compressInternal at a level,
+    oracle/przm_refcpu.c), with the static dictionary when its tables were built into
+    oracle/_ref/ (from the reference decoder's own table header)."""
+
+    def __init__(self, lib):
+        l = lib
+        I, Pi = ctypes.c_int, ctypes.POINTER(ctypes.c_int)
+        l.orc_ref_compress_ex.argtypes = [P, U64, I, P, ctypes.POINTER(RefInfo), P, U64]
+        l.orc_ref_compress_ex.restype = U64
+        l.orc_ref_load_dict.argtypes = [ctypes.c_char_p]
+        l.orc_ref_load_dict.restype = I
+        l.orc_ref_dict_entries.restype = I
+        l.orc_ref_sublengths.argtypes = [I, I, Pi]
+        l.orc_ref_sublengths.restype = I
+        l.orc_ref_lit_cost.argtypes = [I, I]
+        l.orc_ref_lit_cost.restype = U32
+        l.orc_ref_dist_cost.argtypes = [I]
+        l.orc_ref_dist_cost.restype = U32
+        l.orc_ref_cost_explicit.argtypes = [I, U32, I, I]
+        l.orc_ref_cost_explicit.restype = U32
+        l.orc_ref_cost_repeat.argtypes = [I, I, I, I]
+        l.orc_ref_cost_repeat.restype = U32
+        l.orc_ref_detect_stride.argtypes = [P, I]
+        l.orc_ref_detect_stride.restype = U32
+        l.orc_ref_hash.argtypes = [I, P]
+        l.orc_ref_hash.restype = U32
+        l.orc_ref_find_dict.argtypes = [P, I, P, P, P, I]
+        l.orc_ref_find_dict.restype = I
+        self.l = l
+        self.dict = os.path.exists(REFTABLES) and l.orc_ref_load_dict(REFTABLES.encode()) == 0
+
+    def compress(self, src, level=6, with_cmds=False):
+        """one frame of src (one compressInternal call); with_cmds: also the command stream
+        before Section 2 packs it"""
+        a = u8(src)
+        out = np.zeros(a.size + 64, np.uint8)
+        info = RefInfo()
+        cmds = np.zeros(4 * a.size + 64, np.uint8) if with_cmds else None
+        n = self.l.orc_ref_compress_ex(ptr(a), a.size, level, ptr(out), ctypes.byref(info),
+                                       ptr(cmds) if with_cmds else None, cmds.size if with_cmds else 0)
+        if with_cmds:
+            return out[:n].tobytes(), info, cmds[: info.ncmd].tobytes()
+        return out[:n].tobytes(), info
+
+    def compress_frames(self, src, level=6, threads=1, block=65536):
+        """src as independent frames of `block` bytes (frames, sizes), on `threads` threads"""
+        a = u8(src)
+        parts = [a[o: o + block] for o in range(0, max(a.size, 1), block)] if a.size else [a]
+        if threads > 1:
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(threads) as ex:
+                res = list(ex.map(lambda b: self.compress(b, level)[0], parts))
+        else:
+            res = [self.compress(b, level)[0] for b in parts]
+        return b"".join(res), np.array([len(f) for f in res], np.uint32)
+
+    def sublengths(self, L, bmin):
+        out = (ctypes.c_int * 32)()
+        n = self.l.orc_ref_sublengths(L, bmin, out)
+        return list(out[:n])
+
+    def find_dict(self, s):
+        a = u8(s)
+        cap = 256
+        wid, tau, ln = np.zeros(cap, np.uint32), np.zeros(cap, np.uint8), np.zeros(cap, np.int32)
+        n = self.l.orc_ref_find_dict(ptr(a), a.size, ptr(wid), ptr(tau), ptr(ln), cap)
+        if n < 0:
+            return None
+        return [(int(wid[i]), int(tau[i]), int(ln[i])) for i in range(min(n, cap))]
+
+
+class RefTables:
+    """The static dictionary tables of the reference C99 decoder (src/brotli_tables.h, built
+    into oracle/_ref/libprzm_ref_tables.so), read as data for independent checks."""
+
+    def __init__(self):
+        l = ctypes.CDLL(REFTABLES)
+        l.przm_ref_table.restype = P
+        l.przm_ref_table.argtypes = [ctypes.c_int]
+        at = lambda k, t, n: (t * n).from_address(l.przm_ref_table(k))
+        self.word_base = list(at(0, ctypes.c_uint16, 22))
+        self.dict_offsets = list(at(1, ctypes.c_uint32, 21))
+        tr = at(2, ctypes.c_uint8, 121 * 3)
+        self.transforms = [tuple(tr[3 * i: 3 * i + 3]) for i in range(121)]
+        cp = at(3, ctypes.c_uint16, 2048 * 2)
+        self.compact = [(cp[2 * i], cp[2 * i + 1]) for i in range(2048)]
+
+        class Str(ctypes.Structure):
+            _fields_ = [("s", ctypes.c_void_p), ("n", ctypes.c_uint8)]
+        self.ps = [ctypes.string_at(e.s, e.n) if e.n else b"" for e in (Str * 50).from_address(l.przm_ref_table(4))]
+        self.top = [ctypes.string_at(e.s, e.n) if e.n else b"" for e in (Str * 512).from_address(l.przm_ref_table(5))]
+        self.data = bytes(at(6, ctypes.c_uint8, 122784))
+        self.l = l
+
+
 _oracle = None
 _ref = None
+_refcpu = None
+
+
+def refcpu():
+    global _refcpu
+    if _refcpu is None:
+        _refcpu = RefCPU(oracle().l)
+    return _refcpu
+
+
+def have_reftables():
+    return os.path.exists(REFTABLES)
 
 
 def oracle():
