@@ -1477,18 +1477,32 @@ __device__ __forceinline__ void lz_body(LzShared &S, const uint8_t *__restrict__
             if (tid < PRZM_S_N) mt->len[tid] = 0;
             continue;
         }
-        /* load the block into LDS (16-byte loads when the block is aligned) */
+        /* load the block into LDS.  A full aligned block (the common case): the eight rounds'
+         * 16-byte loads are all issued before the first store, so the load waits for one memory
+         * latency; the 64 bytes past the block are zeroed.  Any other block: 16 bytes a round,
+         * byte by byte past its end. */
         const uint8_t *g = src + bd.src_off;
-        const bool al16 = (bd.src_off & 15u) == 0;
-        for (uint32_t i = tid * 16; i < PRZM_BLOCK + 64; i += LZ_NT * 16) {
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (al16 && i + 16 <= n) v = *(const uint4 *)(g + i);
-            else if (i < n) {
-                uint8_t tmp[16];
-                for (int k = 0; k < 16; k++) tmp[k] = i + k < n ? g[i + k] : 0;
-                v = *(uint4 *)tmp;
+        if ((bd.src_off & 15u) == 0 && n == PRZM_BLOCK) {
+            constexpr uint32_t LR = PRZM_BLOCK / (LZ_NT * 16);
+            const uint4 *g4 = (const uint4 *)g + tid;
+            uint4 lv[LR];
+#pragma unroll
+            for (uint32_t r = 0; r < LR; r++) lv[r] = g4[r * LZ_NT];
+#pragma unroll
+            for (uint32_t r = 0; r < LR; r++) *(uint4 *)(S.in + 16u * (tid + r * LZ_NT)) = lv[r];
+            if (tid < 4) *(uint4 *)(S.in + PRZM_BLOCK + 16u * tid) = make_uint4(0, 0, 0, 0);
+        } else {
+            const bool al16 = (bd.src_off & 15u) == 0;
+            for (uint32_t i = tid * 16; i < PRZM_BLOCK + 64; i += LZ_NT * 16) {
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (al16 && i + 16 <= n) v = *(const uint4 *)(g + i);
+                else if (i < n) {
+                    uint8_t tmp[16];
+                    for (int k = 0; k < 16; k++) tmp[k] = i + k < n ? g[i + k] : 0;
+                    v = *(uint4 *)tmp;
+                }
+                if (i + 16 <= PRZM_BLOCK + 64) *(uint4 *)(S.in + i) = v;
             }
-            if (i + 16 <= PRZM_BLOCK + 64) *(uint4 *)(S.in + i) = v;
         }
         if (tid < ST_N) S.st[tid] = 0;
         if (tid < LZ_NW) S.bigq[tid] = 0;
@@ -1730,6 +1744,6 @@ przm_lz_diag_kernel(const uint8_t *__restrict__ src, const przm_block_t *__restr
                     uint8_t *__restrict__ scratch, uint64_t scratch_stride, uint64_t *__restrict__ phase_ts,
                     uint32_t *__restrict__ next_block) {
     __shared__ LzShared S;
-    lz_body<false, 4>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, nullptr, phase_ts, next_block);
+    lz_body<false, 8>(S, src, blocks, nblocks, depth, ws, meta, scratch, scratch_stride, nullptr, phase_ts, next_block);
 }
 #endif
