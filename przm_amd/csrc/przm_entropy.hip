@@ -96,11 +96,9 @@ __device__ __forceinline__ void stream_of(uint32_t k, uint8_t *wsb, const przm_m
 
 /*
  * NormalizeFrequencies (literals.go:78-140) + CDF + joint-nibble matrix
- * (literals.go:143-184) for the 256 counts in cnt[], by one wave.  frq[] is LDS
- * scratch of 256 words.
+ * (literals.go:143-184) for the 256 counts in cnt[], by one wave.
  */
-__device__ void normalize_wave(const uint32_t *cnt, uint32_t *frq, przm_ftab_t *ft,
-                               uint32_t *mat_len) {
+__device__ void normalize_wave(const uint32_t *cnt, przm_ftab_t *ft, uint32_t *mat_len) {
     const uint32_t lane = threadIdx.x & 63;
     uint32_t c4[4], f4[4], k4[4];
     uint32_t tot = 0, uni = 0;
@@ -133,8 +131,6 @@ __device__ void normalize_wave(const uint32_t *cnt, uint32_t *frq, przm_ftab_t *
         }
         for (int i = 0; i < 4; i++) f4[i] += k4[i] >= T ? 1u : 0u;
     }
-    for (int i = 0; i < 4; i++) frq[lane * 4 + i] = f4[i];
-    __builtin_amdgcn_wave_barrier();
     uint32_t loc[4], acc = 0;
     for (int i = 0; i < 4; i++) { loc[i] = f4[i]; acc += loc[i]; }
     uint32_t cum = wave_scan_incl(acc) - acc;
@@ -142,27 +138,38 @@ __device__ void normalize_wave(const uint32_t *cnt, uint32_t *frq, przm_ftab_t *
         ft->fc[lane * 4 + i] = loc[i] | (cum << 16);
         cum += loc[i];
     }
-    if (lane == 0) {
-        uint8_t *o = ft->mat;
-        uint32_t len = 0, rows = 0;
-        for (int h = 0; h < 16; h++)
-            for (int l = 0; l < 16; l++)
-                if (frq[(h << 4) | l]) { rows |= 1u << h; break; }
-        o[len++] = (uint8_t)rows; o[len++] = (uint8_t)(rows >> 8);
-        for (int h = 0; h < 16; h++) {
-            if (!(rows & (1u << h))) continue;
-            uint32_t cols = 0;
-            for (int l = 0; l < 16; l++) if (frq[(h << 4) | l]) cols |= 1u << l;
-            o[len++] = (uint8_t)cols; o[len++] = (uint8_t)(cols >> 8);
-            for (int l = 0; l < 16; l++) {
-                if (!(cols & (1u << l))) continue;
-                const uint32_t wgt = frq[(h << 4) | l];
-                if (wgt < 128) o[len++] = (uint8_t)wgt;
-                else { o[len++] = (uint8_t)((wgt >> 8) | 0x80); o[len++] = (uint8_t)wgt; }
-            }
+    /* the joint-nibble matrix (literals.go:143-184) by the whole wave: lane l holds symbols
+     * 4l..4l+3, a quarter of row l >> 2 (high nibble) with low nibbles 4q..4q+3, q = l & 3.  The
+     * matrix is the row mask, then per present row its column mask and the weights of its
+     * present symbols in order (one byte below 128, else two); each lane's bytes go at an
+     * exclusive scan of the lane byte counts, in row-major order. */
+    const uint32_t q = lane & 3u;
+    uint32_t colbits = 0, nb = 0;
+    for (int i = 0; i < 4; i++)
+        if (f4[i]) {
+            colbits |= 1u << (4u * q + (uint32_t)i);
+            nb += f4[i] < 128u ? 1u : 2u;
         }
-        *mat_len = len;
+    uint32_t cols = colbits;
+    cols |= (uint32_t)__shfl_xor((int)cols, 1);
+    cols |= (uint32_t)__shfl_xor((int)cols, 2);
+    const bool head = q == 0 && cols != 0; /* the row's first lane writes its column mask */
+    const uint64_t rb = __ballot(head);
+    uint32_t rows = 0;
+    for (int h = 0; h < 16; h++) rows |= (uint32_t)((rb >> (4 * h)) & 1ull) << h;
+    const uint32_t c = nb + (head ? 2u : 0u);
+    const uint32_t incl = wave_scan_incl(c);
+    uint32_t off = 2u + incl - c;
+    uint8_t *o = ft->mat;
+    if (head) { o[off] = (uint8_t)cols; o[off + 1] = (uint8_t)(cols >> 8); off += 2; }
+    for (int i = 0; i < 4; i++) {
+        const uint32_t wgt = f4[i];
+        if (!wgt) continue;
+        if (wgt < 128u) o[off++] = (uint8_t)wgt;
+        else { o[off] = (uint8_t)((wgt >> 8) | 0x80u); o[off + 1] = (uint8_t)wgt; off += 2; }
     }
+    if (lane == 0) { o[0] = (uint8_t)rows; o[1] = (uint8_t)(rows >> 8); }
+    if (lane == 63) *mat_len = 2u + incl;
 }
 
 /* one wave per (block, stream) job: lit, op, meta, var */
@@ -170,7 +177,6 @@ extern "C" __global__ void __launch_bounds__(256)
 przm_hist_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, przm_ftab_t *__restrict__ ftab,
                  uint32_t nblocks) {
     __shared__ uint32_t cnt[4][256];
-    __shared__ uint32_t frq[4][256];
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t job = blockIdx.x * 4 + w;
     const bool alive = job < nblocks * 4;
@@ -213,7 +219,7 @@ przm_hist_kernel(uint8_t *__restrict__ ws, przm_meta_t *__restrict__ meta, przm_
     }
     __syncthreads();
     if (!run) return;
-    normalize_wave(cnt[w], frq[w], ftab + (uint64_t)blk * 4 + k, &meta[blk].mat_len[k]);
+    normalize_wave(cnt[w], ftab + (uint64_t)blk * 4 + k, &meta[blk].mat_len[k]);
 }
 
 /* whole-buffer histogram (one workgroup): counts_out, and the normalised table */
@@ -221,7 +227,6 @@ extern "C" __global__ void __launch_bounds__(256)
 przm_hist1_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__restrict__ counts_out,
                   przm_ftab_t *__restrict__ ft, uint32_t *__restrict__ mat_len) {
     __shared__ uint32_t cnt[256];
-    __shared__ uint32_t frq[256];
     const uint32_t tid = threadIdx.x;
     cnt[tid] = 0;
     __syncthreads();
@@ -233,7 +238,7 @@ przm_hist1_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__restr
     for (; i < n; i += 256) atomicAdd(&cnt[src[i]], 1u);
     __syncthreads();
     counts_out[tid] = cnt[tid];
-    if (tid < 64 && ft) normalize_wave(cnt, frq, ft, mat_len);
+    if (tid < 64 && ft) normalize_wave(cnt, ft, mat_len);
 }
 
 /* ---------------- rANS ---------------- */
@@ -1059,10 +1064,9 @@ przm_hist_grid_kernel(const uint8_t *__restrict__ src, uint64_t n, uint32_t *__r
 extern "C" __global__ void __launch_bounds__(64)
 przm_normalize_kernel(const uint32_t *__restrict__ counts, przm_ftab_t *__restrict__ ft, uint32_t *__restrict__ mat_len) {
     __shared__ uint32_t cnt[256];
-    __shared__ uint32_t frq[256];
     for (uint32_t s = threadIdx.x; s < 256; s += 64) cnt[s] = counts[s];
     __syncthreads();
-    normalize_wave(cnt, frq, ft, mat_len);
+    normalize_wave(cnt, ft, mat_len);
 }
 
 /* ModeFlat section [0x00][u32 n][payload] (PackStreamFlat, gpu/kernel/przm_ans.cu:202-234) */

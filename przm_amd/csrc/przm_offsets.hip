@@ -88,7 +88,6 @@ przm_offs_prep_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__restr
                       uint32_t nblocks) {
     __shared__ __align__(16) uint32_t ext[OFFS_EXT_WORDS];
     __shared__ uint32_t cnt[256];
-    __shared__ uint32_t frq[256];
     __shared__ uint32_t part[OFFS_NT / 64u];
     const uint32_t blk = blockIdx.x, tid = threadIdx.x;
     if (blk >= nblocks) return;
@@ -187,7 +186,7 @@ przm_offs_prep_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__restr
         H->mat_len = 0;
     }
     if (nd >= 64u && nd <= PRZM_OFFS_MAX && tid < 64u)
-        normalize_wave(cnt, frq, (przm_ftab_t *)(ob + PRZM_OWS_FTAB), &H->mat_len);
+        normalize_wave(cnt, (przm_ftab_t *)(ob + PRZM_OWS_FTAB), &H->mat_len);
 }
 
 /*

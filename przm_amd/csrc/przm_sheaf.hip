@@ -110,7 +110,6 @@ przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__res
     __shared__ uint8_t kind[256];
     __shared__ uint32_t hs[8][256];
     __shared__ uint32_t cnt[16][256];
-    __shared__ uint32_t frq[SHEAF_NT / 64u][256];
     __shared__ uint32_t part[SHEAF_NT / 64u][2];
     __shared__ uint32_t flen[16];
     const uint32_t blk = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63u;
@@ -162,7 +161,7 @@ przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__res
     }
     for (uint32_t r = w; r < 16; r += SHEAF_NT / 64u)
         if (flen[r] >= SHEAF_MIN_CODED)
-            normalize_wave(cnt[r], frq[w], (przm_ftab_t *)(sb + PRZM_SWS_FTAB) + r, &H->mat[r >> 2][r & 3u]);
+            normalize_wave(cnt[r], (przm_ftab_t *)(sb + PRZM_SWS_FTAB) + r, &H->mat[r >> 2][r & 3u]);
     /* scatter pass, one mode at a time: a literal's place in its fiber is the number of earlier
      * literals of that fiber; one workgroup scan per round of the four per-thread counts (16 bits
      * each: a round holds at most 4096 literals) */
