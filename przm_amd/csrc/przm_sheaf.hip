@@ -235,46 +235,72 @@ przm_sheaf_order_kernel(const uint8_t *__restrict__ sws, uint32_t nblocks, uint3
     for (uint32_t j = tid; j < njobs; j += blockDim.x) perm[atomicAdd(&cnt[cls(j)], 1u)] = j;
 }
 
+/* one counting step of quad lane j for step g with table entry e (freq | cdf << 16), live for
+ * symbols below nl: the coder step of literals.go:305-325 without its stores, counting the bytes
+ * its renormalisation emits in cnt */
+__device__ __forceinline__ void count_step(uint32_t g, uint32_t e, uint32_t j, uint32_t nl, uint32_t &x,
+                                           uint32_t &cnt) {
+    const bool v = 4u * g + j < nl;
+    const uint32_t f = e & 0xFFFFu, t = 4096u * f - 1u;
+    const bool r1 = v && x > t;
+    const uint32_t x1 = r1 ? x >> 8 : x;
+    const bool r2 = r1 && x1 > t;
+    const uint32_t xn = r2 ? x1 >> 8 : x1;
+    cnt += (uint32_t)r1 + (uint32_t)r2;
+    uint32_t qq, rr;
+    divmod(xn, f, &qq, &rr);
+    x = v ? qq * 4096u + (e >> 16) + rr : x;
+}
+
 /*
  * Stream bytes of the V-ANS coding of fiber src[0, n) with table tab, by one lane quad: lane
  * j runs state j over the symbols i = j (mod 4) from the top (literals.go:302-325), counting
  * the bytes its renormalisation emits.  Once the quad has emitted `limit` bytes the fiber
  * cannot be shorter coded than raw: PRZM_ENC_OVERFLOW.
+ *
+ * The chunks are read as rans_quad reads them: RANS_PF 16-byte chunks in flight in a register
+ * ring, every load issued (outside the fiber from przm_zero16), every quad of the wave stepping
+ * as often as the wave's longest fiber (gmax, in 4-symbol groups) in uniform control flow.  The
+ * limit is checked once per turn of the ring (the count only grows), and the wave stops when
+ * all its quads are past it.
  */
 __device__ uint32_t vans_count_quad(const uint8_t *__restrict__ src, uint32_t n, const uint32_t *__restrict__ tab,
-                                    uint32_t limit) {
+                                    uint32_t limit, uint32_t gmax) {
     const uint32_t j = threadIdx.x & 3u;
-    const uint4 *s4 = (const uint4 *)src;
-    uint32_t x = 65536u, cnt = 0;
-    const int32_t nc = (int32_t)((n + 15u) >> 4);
-    const uint4 z = make_uint4(0, 0, 0, 0);
-    uint4 cur = nc > 0 ? s4[nc - 1] : z;
-    for (int32_t c = nc - 1; c >= 0; c--) {
-        const uint4 nxt = c > 0 ? s4[c - 1] : z;
-        const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+    const int32_t own = (int32_t)((n + 15u) >> 4);
+    const int32_t NC = (int32_t)((((gmax + 3u) >> 2) + RANS_PF - 1u) / RANS_PF * RANS_PF);
+    const uint4 *src4 = (const uint4 *)src;
+    auto chunk = [&](int32_t c) -> const uint4 * { return c >= 0 && c < own ? src4 + c : &przm_zero16; };
+    auto ent = [&](uint32_t dw) -> uint32_t { return tab[(dw >> (8u * j)) & 0xFFu]; };
+    uint32_t x = 65536u, cnt = 0, nl = n;
+    bool over = false;
+    uint4 buf[RANS_PF];
 #pragma unroll
-        for (int32_t t = 3; t >= 0; t--) {
-            const uint32_t i = 16u * (uint32_t)c + 4u * (uint32_t)t + j;
-            if (i >= n) continue;
-            const uint32_t e = tab[(wd[t] >> (8u * j)) & 0xFFu];
-            const uint32_t f = e & 0xFFFFu, lim = 4096u * f - 1u;
-            if (x > lim) {
-                cnt++;
-                x >>= 8;
-                if (x > lim) { cnt++; x >>= 8; }
-            }
-            uint32_t q, r;
-            divmod(x, f, &q, &r);
-            x = (q << 12) + (e >> 16) + r;
+    for (uint32_t d = 0; d < RANS_PF; d++) buf[d] = *chunk(NC - 1 - (int32_t)d);
+    uint32_t e[4] = {ent(buf[0].w), ent(buf[0].z), ent(buf[0].y), ent(buf[0].x)};
+    for (int32_t cb = NC - 1; cb >= 0; cb -= (int32_t)RANS_PF) {
+#pragma unroll
+        for (uint32_t d = 0; d < RANS_PF; d++) {
+            const int32_t c = cb - (int32_t)d;
+            const uint4 nb = buf[(d + 1) % RANS_PF];
+            const uint32_t en[4] = {ent(nb.w), ent(nb.z), ent(nb.y), ent(nb.x)};
+            const uint32_t g0 = 4u * (uint32_t)c;
+            count_step(g0 + 3, e[0], j, nl, x, cnt);
+            count_step(g0 + 2, e[1], j, nl, x, cnt);
+            count_step(g0 + 1, e[2], j, nl, x, cnt);
+            count_step(g0, e[3], j, nl, x, cnt);
+#pragma unroll
+            for (int u = 0; u < 4; u++) e[u] = en[u];
+            buf[d] = *chunk(c - (int32_t)RANS_PF);
         }
-        cur = nxt;
         uint32_t tot = cnt + QPERM(cnt, 0xB1);
         tot += QPERM(tot, 0x4E);
-        if (tot >= limit) return PRZM_ENC_OVERFLOW;
+        if (tot >= limit) { over = true; nl = 0; }
+        if (__all(over)) break;
     }
     uint32_t tot = cnt + QPERM(cnt, 0xB1);
     tot += QPERM(tot, 0x4E);
-    return tot;
+    return over || tot >= limit ? PRZM_ENC_OVERFLOW : tot;
 }
 
 /* one quad per (block, mode, fiber) job, 16 jobs a workgroup in the order perm gives
@@ -298,13 +324,19 @@ przm_sheaf_count_kernel(uint8_t *__restrict__ sws, uint32_t nblocks, const uint3
             for (uint32_t s = j; s < 256; s += 4) vt[q][s] = ft->fc[s];
         }
     }
-    __syncthreads();
-    if (!has) return;
     /* coded = matrix + 4 states + stream, taken only when shorter than the fiber */
     const uint32_t head = mat + 16u;
-    const uint32_t K = n >= SHEAF_MIN_CODED && n > head ? vans_count_quad(sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP + fs, n, vt[q],
-                                                  n - head)
-                                : PRZM_ENC_OVERFLOW;
+    const bool coded = has && n >= SHEAF_MIN_CODED && n > head;
+    uint32_t gmax = coded ? (n + 3u) >> 2 : 0u;
+    for (uint32_t o = 32; o >= 4; o >>= 1) {
+        const uint32_t v = (uint32_t)__shfl_xor((int)gmax, (int)o, 64);
+        gmax = v > gmax ? v : gmax;
+    }
+    __syncthreads();
+    if (!has) return;
+    const uint32_t K = coded ? vans_count_quad(sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP + fs, n, vt[q],
+                                               n - head, gmax)
+                             : PRZM_ENC_OVERFLOW;
     if (j == 0) H->k[m][k] = K;
 }
 
