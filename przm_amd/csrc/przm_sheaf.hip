@@ -31,8 +31,12 @@
 
 #define SHEAF_NT 256u
 
-/* each byte's kind for the context signature below: 0 white space, 1 '"' or ':', 2 other syntax,
- * 3 identifier, 4 digit, 5 anything else (the byte classes of sheaf.go:36-102) */
+#define SHEAF_ROUND (16u * SHEAF_NT) /* literals per round of the passes: 16 a thread */
+#define SHEAF_STG (SHEAF_ROUND + 128u) /* LDS staging of one mode's fibers in a round */
+
+/* each byte's kind for the context signature below: 0 white space, 1 '"' or ':', 2 other syntax
+ * except '.' and '-', 3 identifier, 4 digit, 5 anything else, 6 '.' or '-' (the byte classes of
+ * sheaf.go:36-102) */
 struct SheafTable {
     uint8_t kind[256];
     static constexpr bool ws(int b) { return b == ' ' || b == '\t' || b == '\r' || b == '\n'; }
@@ -46,7 +50,8 @@ struct SheafTable {
     static constexpr bool digit(int b) { return b >= '0' && b <= '9'; }
     constexpr SheafTable() : kind() {
         for (int b = 0; b < 256; b++)
-            kind[b] = ws(b) ? 0 : (b == '"' || b == ':') ? 1 : syntax(b) ? 2 : ident(b) ? 3 : digit(b) ? 4 : 5;
+            kind[b] = ws(b) ? 0 : (b == '"' || b == ':') ? 1 : (b == '.' || b == '-') ? 6 : syntax(b) ? 2
+                    : ident(b) ? 3 : digit(b) ? 4 : 5;
     }
 };
 __constant__ SheafTable przm_sheaf_lut = SheafTable();
@@ -73,27 +78,175 @@ __device__ __forceinline__ uint32_t lit_at(const uint8_t *L, int64_t i) { return
 #define SHEAF_T1 0xE973u /* 3 2 2 1 1 3 0 3 */
 #define SHEAF_T2 0x5540u /* 1 1 1 1 1 0 0 0 */
 #define SHEAF_T3 0x5394u /* 1 1 0 3 2 1 1 0 */
-__device__ __forceinline__ uint32_t sheaf_fiber(uint32_t m, uint32_t sig) {
-    const uint32_t t = m == 0 ? SHEAF_T0 : m == 1 ? SHEAF_T1 : m == 2 ? SHEAF_T2 : SHEAF_T3;
-    return (t >> (2u * sig)) & 3u;
+__host__ __device__ constexpr uint32_t sheaf_fiber(uint32_t m, uint32_t sig) {
+    return ((m == 0 ? SHEAF_T0 : m == 1 ? SHEAF_T1 : m == 2 ? SHEAF_T2 : SHEAF_T3) >> (2u * sig)) & 3u;
 }
-__device__ __forceinline__ uint32_t sheaf_sig(const uint8_t *kind, uint32_t p2, uint32_t p1) {
-    const uint32_t k1 = kind[p1], k2 = kind[p2];
-    if (k1 < 3u) return k1;
-    if (k1 == 3u) return k2 == 3u ? 3u : 4u;
-    if (k1 == 4u) return (k2 == 4u || p2 == '.' || p2 == '-') ? 5u : 6u;
-    return 7u;
+/* the signature after a literal of kind k1 preceded by one of kind k2 */
+__host__ __device__ constexpr uint32_t sheaf_sig(uint32_t k1, uint32_t k2) {
+    return k1 == 6u ? 2u : k1 < 3u ? k1 : k1 == 3u ? (k2 == 3u ? 3u : 4u)
+         : k1 == 4u ? ((k2 == 4u || k2 == 6u) ? 5u : 6u) : 7u;
+}
+/* the word of kinds (k1, k2), at sigw[8 k1 + k2]: bits 6m .. 6m + 5 hold 16 * the fiber in mode
+ * m, bits 24 .. 29 8 * the signature (the shifts of packed 16-bit and 8-bit counters) */
+__host__ __device__ constexpr uint32_t sheaf_word(uint32_t k1, uint32_t k2) {
+    const uint32_t s = sheaf_sig(k1, k2);
+    return (8u * s) << 24 | (16u * sheaf_fiber(0, s)) | (16u * sheaf_fiber(1, s)) << 6 |
+           (16u * sheaf_fiber(2, s)) << 12 | (16u * sheaf_fiber(3, s)) << 18;
 }
 
-/* the context signatures of the 16 literals at i0 (w: those literals, byte t of word t >> 2) */
-__device__ __forceinline__ void sheaf_sigs16(const uint8_t *kind, uint32_t p2, uint32_t p1, const uint32_t *w,
-                                             uint32_t sig[16]) {
+/* the 16 literals of a thread in a round (FULL: all of them live) and the table words of their
+ * contexts; returns the number live */
+template <bool FULL>
+__device__ __forceinline__ uint32_t sheaf_load16(const uint8_t *__restrict__ L, uint32_t n, uint32_t i0,
+                                                 const uint8_t *kind, const uint32_t *sigw, uint32_t wd[4],
+                                                 uint32_t sw[16]) {
+    const bool any = FULL || i0 < n;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (any) v = *(const uint4 *)(L + i0);
+    wd[0] = v.x; wd[1] = v.y; wd[2] = v.z; wd[3] = v.w;
+    uint32_t k2 = kind[any ? lit_at(L, (int64_t)i0 - 2) : 0u], k1 = kind[any ? lit_at(L, (int64_t)i0 - 1) : 0u];
 #pragma unroll
     for (uint32_t t = 0; t < 16u; t++) {
-        const uint32_t b = (w[t >> 2] >> (8u * (t & 3u))) & 0xFFu;
-        sig[t] = sheaf_sig(kind, p2, p1);
-        p2 = p1;
-        p1 = b;
+        sw[t] = sigw[8u * k1 + k2];
+        k2 = k1;
+        k1 = kind[(wd[t >> 2] >> (8u * (t & 3u))) & 0xFFu];
+    }
+    return FULL ? 16u : any ? (n - i0 < 16u ? n - i0 : 16u) : 0u;
+}
+
+/* the counting pass's round: each live literal into the histogram of its signature */
+template <bool FULL>
+__device__ __forceinline__ void sheaf_hist_round(const uint8_t *__restrict__ L, uint32_t n, uint32_t i0,
+                                                 const uint8_t *kind, const uint32_t *sigw, uint32_t *hs) {
+    uint32_t wd[4], sw[16];
+    const uint32_t live = sheaf_load16<FULL>(L, n, i0, kind, sigw, wd, sw);
+#pragma unroll
+    for (uint32_t t = 0; t < 16u; t++)
+        if (FULL || t < live) atomicAdd(&hs[((sw[t] >> 24) << 5) + ((wd[t >> 2] >> (8u * (t & 3u))) & 0xFFu)], 1u);
+}
+
+/*
+ * The per-fiber state of the scatter pass, in LDS: fdst[f] is the offset in its mode's fiber area
+ * where fiber f's next 16-byte word goes, fcy[f] the number of its bytes carried over (below 16,
+ * held by thread f), freg[f] / fnw[f] the round's staging region of the fiber (16-byte aligned in
+ * its mode's staging area: the carried bytes, then the round's literals) and its whole words.
+ */
+struct SheafFib {
+    uint32_t dst[16], cy[16], reg[16], nw[16];
+};
+
+/*
+ * A round of the scatter pass, all four modes at once: each live literal goes to its fiber in
+ * every mode in order.  A literal's place among the round's literals of its fiber is the number of
+ * earlier ones of that fiber: the per-signature counts of the thread (8 bits each) are scanned
+ * over the workgroup as 16-bit fields, and the fiber's count is the sum over the signatures that
+ * map to it.  The literals are written to LDS staging, then the round's whole 16-byte words of
+ * every fiber are stored; the last partial word of a fiber is carried into the next round.
+ */
+template <bool FULL>
+__device__ __forceinline__ void sheaf_scatter_round(const uint8_t *__restrict__ L, uint32_t n, uint32_t base,
+                                                    const uint8_t *kind, const uint32_t *sigw, uint8_t *stg,
+                                                    uint32_t (*part)[4], SheafFib &F, uint8_t *__restrict__ fib,
+                                                    uint4 &cw, uint32_t &my_dst, uint32_t &my_cy) {
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63u;
+    uint32_t wd[4], sw[16];
+    const uint32_t live = sheaf_load16<FULL>(L, n, base + 16u * tid, kind, sigw, wd, sw);
+    uint64_t sc = 0;
+#pragma unroll
+    for (uint32_t t = 0; t < 16u; t++)
+        if (FULL || t < live) sc += 1ull << (sw[t] >> 24);
+    const uint32_t lo = (uint32_t)sc, hi = (uint32_t)(sc >> 32);
+    const uint32_t c4[4] = {(lo & 0xFFu) | ((lo & 0xFF00u) << 8), ((lo >> 16) & 0xFFu) | ((lo >> 8) & 0xFF0000u),
+                            (hi & 0xFFu) | ((hi & 0xFF00u) << 8), ((hi >> 16) & 0xFFu) | ((hi >> 8) & 0xFF0000u)};
+    uint32_t in4[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) in4[i] = wave_scan_incl(c4[i]);
+    if (lane == 63u) *(uint4 *)part[w] = make_uint4(in4[0], in4[1], in4[2], in4[3]);
+    __syncthreads();
+    uint32_t E[8], T[8];
+    {
+        uint32_t pre[4] = {0, 0, 0, 0}, tot[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t k = 0; k < SHEAF_NT / 64u; k++) {
+            const uint4 pk = *(const uint4 *)part[k];
+            const uint32_t a[4] = {pk.x, pk.y, pk.z, pk.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                pre[i] += k < w ? a[i] : 0u;
+                tot[i] += a[i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t e = pre[i] + in4[i] - c4[i];
+            E[2 * i] = e & 0xFFFFu;
+            E[2 * i + 1] = e >> 16;
+            T[2 * i] = tot[i] & 0xFFFFu;
+            T[2 * i + 1] = tot[i] >> 16;
+        }
+    }
+    /* per mode: the fibers' round totals, staging regions and this thread's first places */
+    uint64_t P[4];
+#pragma unroll
+    for (uint32_t m = 0; m < 4; m++) {
+        uint32_t tf[4] = {0, 0, 0, 0}, ps[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t s = 0; s < 8; s++) {
+            tf[sheaf_fiber(m, s)] += T[s];
+            ps[sheaf_fiber(m, s)] += E[s];
+        }
+        uint32_t off = 0;
+        P[m] = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t cy = F.cy[4u * m + k], r = off;
+            P[m] |= (uint64_t)(r + cy + ps[k]) << (16u * k);
+            off += al16(cy + tf[k]);
+            if (tid == 4u * m + k) { /* the fiber's carry thread: its region, words and carried bytes */
+                F.reg[tid] = r;
+                F.nw[tid] = (cy + tf[k]) >> 4;
+                const uint32_t cwd[4] = {cw.x, cw.y, cw.z, cw.w};
+                uint8_t *d = stg + m * SHEAF_STG + r;
+#pragma unroll
+                for (uint32_t b = 0; b < 15u; b++)
+                    if (b < cy) d[b] = (uint8_t)(cwd[b >> 2] >> (8u * (b & 3u)));
+                my_cy = (cy + tf[k]) & 15u;
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t t = 0; t < 16u; t++) {
+        if (!(FULL || t < live)) continue;
+        const uint8_t b = (uint8_t)(wd[t >> 2] >> (8u * (t & 3u)));
+#pragma unroll
+        for (uint32_t m = 0; m < 4; m++) {
+            const uint32_t sh = (sw[t] >> (6u * m)) & 63u;
+            stg[m * SHEAF_STG + ((uint32_t)(P[m] >> sh) & 0xFFFFu)] = b;
+            P[m] += 1ull << sh;
+        }
+    }
+    __syncthreads();
+    /* the whole words of every fiber; fiber f's carry thread keeps its partial last word */
+#pragma unroll
+    for (uint32_t m = 0; m < 4; m++) {
+        const uint32_t n0 = F.nw[4u * m], n1 = n0 + F.nw[4u * m + 1u], n2 = n1 + F.nw[4u * m + 2u],
+                       W = n2 + F.nw[4u * m + 3u];
+        for (uint32_t g = tid; g < W; g += SHEAF_NT) {
+            const uint32_t k = (g >= n0 ? 1u : 0u) + (g >= n1 ? 1u : 0u) + (g >= n2 ? 1u : 0u);
+            const uint32_t gb = k == 0 ? 0u : k == 1 ? n0 : k == 2 ? n1 : n2, f = 4u * m + k;
+            const uint4 x = *(const uint4 *)(stg + m * SHEAF_STG + F.reg[f] + 16u * (g - gb));
+            *(uint4 *)(fib + (uint64_t)m * PRZM_SWS_FIB_CAP + F.dst[f] + 16u * (g - gb)) = x;
+        }
+    }
+    if (tid < 16u) {
+        const uint32_t m = tid >> 2;
+        if (my_cy) cw = *(const uint4 *)(stg + m * SHEAF_STG + F.reg[tid] + 16u * F.nw[tid]);
+        my_dst = F.dst[tid] + 16u * F.nw[tid];
+    }
+    __syncthreads();
+    if (tid < 16u) {
+        F.dst[tid] = my_dst;
+        F.cy[tid] = my_cy;
     }
 }
 
@@ -102,16 +255,22 @@ __device__ __forceinline__ void sheaf_sigs16(const uint8_t *kind, uint32_t p2, u
  * round, loaded as one 16-byte word) fills one histogram per context signature; the 16
  * histograms (mode, fiber) are sums of those rows, the fiber lengths come from them, and each
  * wave normalises the tables of four of them (the fibers long enough to be coded); a scatter
- * pass per mode then writes the fibers.
+ * pass then writes the fibers of all four modes, each 16-byte aligned in its mode's area, the
+ * bytes after its end up to the next 16-byte boundary zero.
  */
 extern "C" __global__ void __launch_bounds__(SHEAF_NT)
 przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__restrict__ meta,
                         uint8_t *__restrict__ sws, uint32_t nblocks) {
     __shared__ uint8_t kind[256];
-    __shared__ uint32_t hs[8][256];
-    __shared__ uint32_t cnt[16][256];
-    __shared__ uint32_t part[SHEAF_NT / 64u][2];
+    __shared__ uint32_t sigw[64];
+    /* hs[8][256] (per signature) and cnt[16][256] (per mode and fiber); the scatter pass's staging */
+    __shared__ __align__(16) uint32_t pool[24u * 256u];
+    __shared__ __align__(16) uint32_t part[SHEAF_NT / 64u][4];
     __shared__ uint32_t flen[16];
+    __shared__ SheafFib F;
+    static_assert(4u * SHEAF_STG <= sizeof(pool), "staging fits the histogram space");
+    uint32_t *hs = pool;
+    uint32_t (*cnt)[256] = (uint32_t (*)[256])(pool + 8u * 256u);
     const uint32_t blk = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63u;
     if (blk >= nblocks) return;
     const uint8_t *L = ws + (uint64_t)blk * PRZM_WS_BYTES + PRZM_WS_LIT; /* 16-byte aligned */
@@ -119,24 +278,20 @@ przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__res
     uint8_t *sb = sws + (uint64_t)blk * PRZM_SWS_BYTES;
     przm_sheaf_t *H = (przm_sheaf_t *)(sb + PRZM_SWS_HDR);
     for (uint32_t i = tid; i < 256; i += SHEAF_NT) kind[i] = przm_sheaf_lut.kind[i];
-    for (uint32_t i = tid; i < 8 * 256 / 4; i += SHEAF_NT) ((uint4 *)&hs[0][0])[i] = make_uint4(0, 0, 0, 0);
+    if (tid < 64u) sigw[tid] = sheaf_word(tid >> 3, tid & 7u);
+    for (uint32_t i = tid; i < 8 * 256 / 4; i += SHEAF_NT) ((uint4 *)hs)[i] = make_uint4(0, 0, 0, 0);
     __syncthreads();
     /* counting pass: one histogram per context signature */
-    for (uint32_t base = 0; base < n; base += 16u * SHEAF_NT) {
+    for (uint32_t base = 0; base < n; base += SHEAF_ROUND) {
         const uint32_t i0 = base + 16u * tid;
-        if (i0 >= n) continue;
-        const uint4 v = *(const uint4 *)(L + i0);
-        const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
-        uint32_t sig[16];
-        sheaf_sigs16(kind, lit_at(L, (int64_t)i0 - 2), lit_at(L, (int64_t)i0 - 1), wd, sig);
-        for (uint32_t t = 0; t < 16u && i0 + t < n; t++)
-            atomicAdd(&hs[sig[t]][(wd[t >> 2] >> (8u * (t & 3u))) & 0xFFu], 1u);
+        if (i0 + 16u <= n) sheaf_hist_round<true>(L, n, i0, kind, sigw, hs);
+        else if (i0 < n) sheaf_hist_round<false>(L, n, i0, kind, sigw, hs);
     }
     __syncthreads();
     /* the (mode, fiber) histograms: thread s sums bin s over the signatures of each fiber */
     for (uint32_t sb_ = tid; sb_ < 256; sb_ += SHEAF_NT) {
         uint32_t h[8], acc[16];
-        for (uint32_t g = 0; g < 8; g++) h[g] = hs[g][sb_];
+        for (uint32_t g = 0; g < 8; g++) h[g] = hs[g * 256u + sb_];
 #pragma unroll
         for (uint32_t r = 0; r < 16; r++) acc[r] = 0;
 #pragma unroll
@@ -158,50 +313,33 @@ przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__res
         if (tid == 0) H->n = n;
         H->l[tid >> 2][tid & 3u] = flen[tid];
         if (flen[tid] < SHEAF_MIN_CODED) H->mat[tid >> 2][tid & 3u] = 0; /* the fiber's wave writes it otherwise */
+        uint32_t fs = 0; /* the fiber's place in its mode's area: the 16-byte rounded lengths before it */
+        for (uint32_t k = 0; k < (tid & 3u); k++) fs += al16(flen[(tid & ~3u) + k]);
+        F.dst[tid] = fs;
+        F.cy[tid] = 0;
     }
     for (uint32_t r = w; r < 16; r += SHEAF_NT / 64u)
         if (flen[r] >= SHEAF_MIN_CODED)
             normalize_wave(cnt[r], (przm_ftab_t *)(sb + PRZM_SWS_FTAB) + r, &H->mat[r >> 2][r & 3u]);
-    /* scatter pass, one mode at a time: a literal's place in its fiber is the number of earlier
-     * literals of that fiber; one workgroup scan per round of the four per-thread counts (16 bits
-     * each: a round holds at most 4096 literals) */
-    for (uint32_t m = 0; m < 4; m++) {
-        uint8_t *fib = sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP;
-        uint32_t fs[4];
-        fs[0] = 0;
-        for (uint32_t k = 1; k < 4; k++) fs[k] = fs[k - 1] + al16(flen[4u * m + k - 1]);
-        uint32_t run[4] = {0, 0, 0, 0};
-        for (uint32_t base = 0; base < n; base += 16u * SHEAF_NT) {
-            const uint32_t i0 = base + 16u * tid;
-            uint32_t wd[4] = {0, 0, 0, 0}, cls[16], mine[4] = {0, 0, 0, 0};
-            if (i0 < n) {
-                const uint4 v = *(const uint4 *)(L + i0);
-                wd[0] = v.x; wd[1] = v.y; wd[2] = v.z; wd[3] = v.w;
-                sheaf_sigs16(kind, lit_at(L, (int64_t)i0 - 2), lit_at(L, (int64_t)i0 - 1), wd, cls);
-                for (uint32_t t = 0; t < 16u; t++) cls[t] = sheaf_fiber(m, cls[t]);
-                for (uint32_t t = 0; t < 16u && i0 + t < n; t++) mine[cls[t]]++;
-            }
-            const uint32_t lo = mine[0] | (mine[1] << 16), hi = mine[2] | (mine[3] << 16);
-            const uint32_t lin = wave_scan_incl(lo), hin = wave_scan_incl(hi);
-            if (lane == 63u) { part[w][0] = lin; part[w][1] = hin; }
-            __syncthreads();
-            uint32_t blo = 0, bhi = 0, tlo = 0, thi = 0;
-            for (uint32_t k = 0; k < SHEAF_NT / 64u; k++) {
-                if (k < w) { blo += part[k][0]; bhi += part[k][1]; }
-                tlo += part[k][0];
-                thi += part[k][1];
-            }
-            __syncthreads();
-            const uint32_t elo = blo + lin - lo, ehi = bhi + hin - hi;
-            uint32_t pos[4] = {fs[0] + run[0] + (elo & 0xFFFFu), fs[1] + run[1] + (elo >> 16),
-                               fs[2] + run[2] + (ehi & 0xFFFFu), fs[3] + run[3] + (ehi >> 16)};
-            for (uint32_t t = 0; t < 16u && i0 + t < n; t++)
-                fib[pos[cls[t]]++] = (uint8_t)(wd[t >> 2] >> (8u * (t & 3u)));
-            run[0] += tlo & 0xFFFFu;
-            run[1] += tlo >> 16;
-            run[2] += thi & 0xFFFFu;
-            run[3] += thi >> 16;
+    __syncthreads(); /* the staging below reuses the histograms' space */
+    /* scatter pass */
+    uint8_t *stg = (uint8_t *)pool, *fib = sb + PRZM_SWS_FIB;
+    uint4 cw = make_uint4(0, 0, 0, 0);
+    uint32_t my_dst = 0, my_cy = 0;
+    for (uint32_t base = 0; base < n; base += SHEAF_ROUND) {
+        if (base + SHEAF_ROUND <= n)
+            sheaf_scatter_round<true>(L, n, base, kind, sigw, stg, part, F, fib, cw, my_dst, my_cy);
+        else
+            sheaf_scatter_round<false>(L, n, base, kind, sigw, stg, part, F, fib, cw, my_dst, my_cy);
+    }
+    if (tid < 16u && my_cy) { /* the fiber's last partial word, zero after its end */
+        uint32_t cwd[4] = {cw.x, cw.y, cw.z, cw.w};
+#pragma unroll
+        for (uint32_t q = 0; q < 4; q++) {
+            const uint32_t keep = my_cy >= 4u * q + 4u ? 4u : my_cy > 4u * q ? my_cy - 4u * q : 0u;
+            cwd[q] &= keep == 4u ? 0xFFFFFFFFu : (1u << (8u * keep)) - 1u;
         }
+        *(uint4 *)(fib + (uint64_t)(tid >> 2) * PRZM_SWS_FIB_CAP + my_dst) = make_uint4(cwd[0], cwd[1], cwd[2], cwd[3]);
     }
 }
 
