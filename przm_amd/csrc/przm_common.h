@@ -149,5 +149,8 @@ typedef struct {
 #define PRZM_SWS_OUT (PRZM_SWS_FIB + 4u * PRZM_SWS_FIB_CAP)
 #define PRZM_SWS_OUT_CAP (65536u + 4u * 80u + 64u)
 #define PRZM_SWS_BYTES (PRZM_SWS_OUT + PRZM_SWS_OUT_CAP)
+/* u32 class counts and reservations of the sheaf counting-job order, after the job orders in the
+ * coder-order buffer (8-byte aligned: one word of room) */
+#define PRZM_SHEAF_BINS 512u
 
 #endif

@@ -155,7 +155,7 @@ constexpr uint64_t NA_OFF_BLOCKS = 0;
 constexpr uint64_t NA_OFF_META = NA_OFF_BLOCKS + na_al(NA_BLOCKS * sizeof(przm_block_t));
 constexpr uint64_t NA_OFF_FTAB = NA_OFF_META + na_al(NA_BLOCKS * sizeof(przm_meta_t));
 constexpr uint64_t NA_OFF_PERM = NA_OFF_FTAB + na_al(NA_BLOCKS * 4ull * sizeof(przm_ftab_t));
-constexpr uint64_t NA_OFF_FLEN = NA_OFF_PERM + na_al(NA_BLOCKS * (PRZM_J_N + 16) * 4ull);
+constexpr uint64_t NA_OFF_FLEN = NA_OFF_PERM + na_al(NA_BLOCKS * (PRZM_J_N + 16) * 4ull + PRZM_SHEAF_BINS * 4ull + 4ull);
 constexpr uint64_t NA_OFF_FOFF = NA_OFF_FLEN + na_al(NA_BLOCKS * 4ull);
 constexpr uint64_t NA_OFF_RUN = NA_OFF_FOFF + na_al(NA_BLOCKS * 8ull);
 constexpr uint64_t NA_OFF_WS = NA_OFF_RUN + 256;
@@ -307,7 +307,7 @@ int32_t ensure_ctx(Ctx &c, uint32_t nblocks, uint32_t chunk, int grid, bool dbg,
     CHK(c.ws.ensure((size_t)ch * PRZM_WS_BYTES, &c.gen));
     CHK(c.meta.ensure((size_t)std::max<uint32_t>(nblocks, 1) * sizeof(przm_meta_t), &c.gen));
     CHK(c.ftab.ensure((size_t)ch * 4 * sizeof(przm_ftab_t), &c.gen));
-    CHK(c.perm.ensure((size_t)ch * (PRZM_J_N + (sheaf ? 16u : 0u)) * 4, &c.gen));
+    CHK(c.perm.ensure((size_t)ch * (PRZM_J_N + (sheaf ? 16u : 0u)) * 4 + (sheaf ? PRZM_SHEAF_BINS * 4u + 4u : 0u), &c.gen));
     CHK(c.flen.ensure((size_t)std::max<uint32_t>(nblocks, 1) * 4, &c.gen));
     CHK(c.foff.ensure((size_t)std::max<uint32_t>(nblocks, 1) * 8, &c.gen));
     /* run: the running frame offset (u64) at 0, then one block counter (u32) per chunk at 64 */
@@ -351,7 +351,7 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
     uint8_t *sws = sheaf ? (uint8_t *)c.sws.p : nullptr;
     const size_t chb = (size_t)std::min(chunk, std::max<uint32_t>(nblocks, 1));
     if ((offs && (!ows || c.ows.cap < chb * PRZM_OWS_BYTES)) ||
-        (sheaf && (!sws || c.sws.cap < chb * PRZM_SWS_BYTES || c.perm.cap < chb * (PRZM_J_N + 16) * 4))) {
+        (sheaf && (!sws || c.sws.cap < chb * PRZM_SWS_BYTES || c.perm.cap < chb * (PRZM_J_N + 16) * 4 + PRZM_SHEAF_BINS * 4u + 4u))) {
         snprintf(g_err, sizeof g_err, "a section option needs its workspace (set the option before the call)");
         return PRZM_GPU_ERR_ARG;
     }
@@ -405,7 +405,12 @@ int32_t enqueue(przm_gpu *g, Ctx &c, const uint8_t *d_src, uint32_t nblocks, uin
             uint32_t *sperm = (uint32_t *)c.perm.p + (size_t)PRZM_J_N * nb;
             hipLaunchKernelGGL(przm_sheaf_split_kernel, dim3(nb), dim3(SHEAF_NT), 0, st, (const uint8_t *)c.ws.p,
                                (const przm_meta_t *)(meta + c0), sws, nb);
-            hipLaunchKernelGGL(przm_sheaf_order_kernel, dim3(1), dim3(1024), 0, st, (const uint8_t *)sws, nb, sperm);
+            uint32_t *sbins = (uint32_t *)c.perm.p + (((size_t)(PRZM_J_N + 16) * nb + 1u) & ~(size_t)1);
+            hipLaunchKernelGGL(przm_zero_kernel, dim3(1), dim3(256), 0, st, (uint64_t *)sbins, PRZM_SHEAF_BINS / 2u);
+            hipLaunchKernelGGL(przm_sheaf_class_kernel, dim3((nb + 15) / 16), dim3(256), 0, st, (const uint8_t *)sws,
+                               nb, sbins);
+            hipLaunchKernelGGL(przm_sheaf_place_kernel, dim3((nb + 15) / 16), dim3(256), 0, st, (const uint8_t *)sws,
+                               nb, sbins, sperm);
             hipLaunchKernelGGL(przm_sheaf_count_kernel, dim3(nb), dim3(64), 0, st, sws, nb, (const uint32_t *)sperm);
             hipLaunchKernelGGL(przm_sheaf_encode_kernel, dim3((nb + 3) / 4), dim3(64), 0, st,
                                (const przm_meta_t *)(meta + c0), sws, nb);
@@ -863,7 +868,7 @@ int32_t przm_gpu_encode_device_noalloc(przm_gpu *g, const void *d_src, uint64_t 
     v.meta.p = base + NA_OFF_META;
     v.ftab.p = base + NA_OFF_FTAB;
     v.perm.p = base + NA_OFF_PERM;
-    v.perm.cap = (size_t)NA_BLOCKS * (PRZM_J_N + 16) * 4;
+    v.perm.cap = (size_t)NA_BLOCKS * (PRZM_J_N + 16) * 4 + PRZM_SHEAF_BINS * 4u + 4u;
     v.flen.p = base + NA_OFF_FLEN;
     v.foff.p = base + NA_OFF_FOFF;
     v.run.p = base + NA_OFF_RUN;

@@ -14,7 +14,8 @@
  *                            literals.go:78-184), and a scatter pass per mode
  *                            writes the fibers (DecomposeFibers,
  *                            sheaf.go:125-152), each 16-byte aligned.
- *  przm_sheaf_order_kernel   the counting jobs by fiber length.
+ *  przm_sheaf_class_kernel,
+ *  przm_sheaf_place_kernel   the counting jobs by fiber length.
  *  przm_sheaf_count_kernel   one lane quad per (block, mode, fiber): the exact
  *                            length of the fiber's V-ANS stream (the coder of
  *                            literals.go:289-341 without its stores), each lane
@@ -343,51 +344,70 @@ przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__res
     }
 }
 
-/* the counting jobs (block, mode, fiber) by fiber length in 256-byte classes, longest first
- * (one workgroup): a wave of przm_sheaf_count_kernel steps as often as its longest job */
-extern "C" __global__ void __launch_bounds__(1024)
-przm_sheaf_order_kernel(const uint8_t *__restrict__ sws, uint32_t nblocks, uint32_t *__restrict__ perm) {
+/*
+ * The counting jobs (block, mode, fiber) by fiber length in 256-byte classes, longest first, so
+ * that a wave of przm_sheaf_count_kernel steps as often as its longest job: one thread per job,
+ * 256 jobs (16 blocks) a workgroup.  przm_sheaf_class_kernel adds the workgroup's class counts
+ * into bins[0, 256); przm_sheaf_place_kernel takes the exclusive scan of those, reserves the
+ * workgroup's run in each class with bins[256 + class], and places its jobs there.  bins start at
+ * zero.  The order inside a class is whatever the atomics give: every job writes only its own
+ * outputs, so the results do not depend on it.
+ */
+__device__ __forceinline__ uint32_t sheaf_job_class(const uint8_t *__restrict__ sws, uint32_t j) {
+    const przm_sheaf_t *H = (const przm_sheaf_t *)(sws + (uint64_t)(j >> 4) * PRZM_SWS_BYTES + PRZM_SWS_HDR);
+    const uint32_t c = H->l[(j >> 2) & 3u][j & 3u] >> 8;
+    return 255u - (c < 255u ? c : 255u);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+przm_sheaf_class_kernel(const uint8_t *__restrict__ sws, uint32_t nblocks, uint32_t *__restrict__ bins) {
     __shared__ uint32_t cnt[256];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, njobs = 16u * nblocks;
-    auto cls = [&](uint32_t j) -> uint32_t {
-        const przm_sheaf_t *H = (const przm_sheaf_t *)(sws + (uint64_t)(j >> 4) * PRZM_SWS_BYTES + PRZM_SWS_HDR);
-        const uint32_t c = H->l[(j >> 2) & 3u][j & 3u] >> 8;
-        return 255u - (c < 255u ? c : 255u);
-    };
-    if (tid < 256) cnt[tid] = 0;
+    const uint32_t tid = threadIdx.x, j = blockIdx.x * 256u + tid;
+    cnt[tid] = 0;
     __syncthreads();
-    for (uint32_t j = tid; j < njobs; j += blockDim.x) atomicAdd(&cnt[cls(j)], 1u);
+    if (j < 16u * nblocks) atomicAdd(&cnt[sheaf_job_class(sws, j)], 1u);
     __syncthreads();
-    if (tid < 64) { /* exclusive scan of the 256 counts, four per lane */
+    if (cnt[tid]) atomicAdd(&bins[tid], cnt[tid]);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+przm_sheaf_place_kernel(const uint8_t *__restrict__ sws, uint32_t nblocks, uint32_t *__restrict__ bins,
+                        uint32_t *__restrict__ perm) {
+    __shared__ uint32_t cnt[256], base[256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, j = blockIdx.x * 256u + tid;
+    const bool has = j < 16u * nblocks;
+    cnt[tid] = 0;
+    __syncthreads();
+    const uint32_t c = has ? sheaf_job_class(sws, j) : 0u;
+    const uint32_t r = has ? atomicAdd(&cnt[c], 1u) : 0u; /* the job's rank in the workgroup's run */
+    if (tid < 64u) { /* exclusive scan of the class counts, four per lane */
         uint32_t c4[4], t = 0;
-        for (int k = 0; k < 4; k++) { c4[k] = cnt[lane * 4 + k]; t += c4[k]; }
-        uint32_t x = t;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o);
-            if (lane >= (uint32_t)o) x += y;
-        }
-        uint32_t run = x - t;
-        for (int k = 0; k < 4; k++) { cnt[lane * 4 + k] = run; run += c4[k]; }
+        for (int k = 0; k < 4; k++) { c4[k] = bins[lane * 4 + k]; t += c4[k]; }
+        uint32_t run = wave_scan_incl(t) - t;
+        for (int k = 0; k < 4; k++) { base[lane * 4 + k] = run; run += c4[k]; }
     }
     __syncthreads();
-    for (uint32_t j = tid; j < njobs; j += blockDim.x) perm[atomicAdd(&cnt[cls(j)], 1u)] = j;
+    if (cnt[tid]) base[tid] += atomicAdd(&bins[256u + tid], cnt[tid]);
+    __syncthreads();
+    if (has) perm[base[c] + r] = j;
 }
 
 /* one counting step of quad lane j for step g with table entry e (freq | cdf << 16), live for
  * symbols below nl: the coder step of literals.go:305-325 without its stores, counting the bytes
- * its renormalisation emits in cnt */
+ * its renormalisation emits in cnt.  A step that is not live codes the entry freq 4096, cdf 0,
+ * which leaves every state below 2^24 as it is and emits nothing, so no lane branches. */
 __device__ __forceinline__ void count_step(uint32_t g, uint32_t e, uint32_t j, uint32_t nl, uint32_t &x,
                                            uint32_t &cnt) {
-    const bool v = 4u * g + j < nl;
+    e = 4u * g + j < nl ? e : 4096u;
     const uint32_t f = e & 0xFFFFu, t = 4096u * f - 1u;
-    const bool r1 = v && x > t;
+    const bool r1 = x > t;
     const uint32_t x1 = r1 ? x >> 8 : x;
     const bool r2 = r1 && x1 > t;
     const uint32_t xn = r2 ? x1 >> 8 : x1;
     cnt += (uint32_t)r1 + (uint32_t)r2;
     uint32_t qq, rr;
     divmod(xn, f, &qq, &rr);
-    x = v ? qq * 4096u + (e >> 16) + rr : x;
+    x = qq * 4096u + (e >> 16) + rr;
 }
 
 /*
@@ -442,7 +462,7 @@ __device__ uint32_t vans_count_quad(const uint8_t *__restrict__ src, uint32_t n,
 }
 
 /* one quad per (block, mode, fiber) job, 16 jobs a workgroup in the order perm gives
- * (przm_sheaf_order_kernel); job = block * 16 + mode * 4 + fiber */
+ * (przm_sheaf_place_kernel); job = block * 16 + mode * 4 + fiber */
 extern "C" __global__ void __launch_bounds__(64)
 przm_sheaf_count_kernel(uint8_t *__restrict__ sws, uint32_t nblocks, const uint32_t *__restrict__ perm) {
     __shared__ uint32_t vt[16][256];
