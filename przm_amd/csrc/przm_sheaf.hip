@@ -392,13 +392,14 @@ przm_sheaf_place_kernel(const uint8_t *__restrict__ sws, uint32_t nblocks, uint3
     if (has) perm[base[c] + r] = j;
 }
 
-/* one counting step of quad lane j for step g with table entry e (freq | cdf << 16), live for
- * symbols below nl: the coder step of literals.go:305-325 without its stores, counting the bytes
- * its renormalisation emits in cnt.  A step that is not live codes the entry freq 4096, cdf 0,
- * which leaves every state below 2^24 as it is and emits nothing, so no lane branches. */
+/* one counting step of quad lane j for step g with table entry e (freq | cdf << 16): the coder
+ * step of literals.go:305-325 without its stores, counting the bytes its renormalisation emits in
+ * cnt.  LIVE: the step may lie past the symbols below nl; such a step codes the entry freq 4096,
+ * cdf 0, which leaves every state below 2^24 as it is and emits nothing, so no lane branches. */
+template <bool LIVE>
 __device__ __forceinline__ void count_step(uint32_t g, uint32_t e, uint32_t j, uint32_t nl, uint32_t &x,
                                            uint32_t &cnt) {
-    e = 4u * g + j < nl ? e : 4096u;
+    if (LIVE) e = 4u * g + j < nl ? e : 4096u;
     const uint32_t f = e & 0xFFFFu, t = 4096u * f - 1u;
     const bool r1 = x > t;
     const uint32_t x1 = r1 ? x >> 8 : x;
@@ -410,6 +411,30 @@ __device__ __forceinline__ void count_step(uint32_t g, uint32_t e, uint32_t j, u
     x = qq * 4096u + (e >> 16) + rr;
 }
 
+/* one turn of the chunk ring (RANS_PF chunks from cb down): the steps of the ring's chunks, the
+ * table entries of each chunk's successor read while it is coded, the ring's slots reloaded */
+template <bool LIVE>
+__device__ __forceinline__ void count_turn(const uint4 *__restrict__ src4, int32_t own, int32_t cb,
+                                           const uint32_t *__restrict__ tab, uint32_t j, uint32_t nl, uint4 (&buf)[RANS_PF],
+                                           uint32_t (&e)[4], uint32_t &x, uint32_t &cnt) {
+    auto ent = [&](uint32_t dw) -> uint32_t { return tab[(dw >> (8u * j)) & 0xFFu]; };
+#pragma unroll
+    for (uint32_t d = 0; d < RANS_PF; d++) {
+        const int32_t c = cb - (int32_t)d;
+        const uint4 nb = buf[(d + 1) % RANS_PF];
+        const uint32_t en[4] = {ent(nb.w), ent(nb.z), ent(nb.y), ent(nb.x)};
+        const uint32_t g0 = 4u * (uint32_t)c;
+        count_step<LIVE>(g0 + 3, e[0], j, nl, x, cnt);
+        count_step<LIVE>(g0 + 2, e[1], j, nl, x, cnt);
+        count_step<LIVE>(g0 + 1, e[2], j, nl, x, cnt);
+        count_step<LIVE>(g0, e[3], j, nl, x, cnt);
+#pragma unroll
+        for (int u = 0; u < 4; u++) e[u] = en[u];
+        const int32_t cn = c - (int32_t)RANS_PF;
+        buf[d] = *(cn >= 0 && cn < own ? src4 + cn : &przm_zero16);
+    }
+}
+
 /*
  * Stream bytes of the V-ANS coding of fiber src[0, n) with table tab, by one lane quad: lane
  * j runs state j over the symbols i = j (mod 4) from the top (literals.go:302-325), counting
@@ -418,39 +443,33 @@ __device__ __forceinline__ void count_step(uint32_t g, uint32_t e, uint32_t j, u
  *
  * The chunks are read as rans_quad reads them: RANS_PF 16-byte chunks in flight in a register
  * ring, every load issued (outside the fiber from przm_zero16), every quad of the wave stepping
- * as often as the wave's longest fiber (gmax, in 4-symbol groups) in uniform control flow.  The
- * limit is checked once per turn of the ring (the count only grows), and the wave stops when
- * all its quads are past it.
+ * as often as the wave's longest fiber (gmax, in 4-symbol groups) in uniform control flow.  A
+ * turn of the ring below nmin, the wave's shortest fiber, has every step live and takes no
+ * liveness test (a quad past its limit keeps stepping there: its result is overflow whatever
+ * its state).  The limit is checked once per turn (the count only grows), and the wave stops
+ * when all its quads are past it.
  */
 __device__ uint32_t vans_count_quad(const uint8_t *__restrict__ src, uint32_t n, const uint32_t *__restrict__ tab,
-                                    uint32_t limit, uint32_t gmax) {
+                                    uint32_t limit, uint32_t gmax, uint32_t nmin) {
     const uint32_t j = threadIdx.x & 3u;
     const int32_t own = (int32_t)((n + 15u) >> 4);
     const int32_t NC = (int32_t)((((gmax + 3u) >> 2) + RANS_PF - 1u) / RANS_PF * RANS_PF);
     const uint4 *src4 = (const uint4 *)src;
-    auto chunk = [&](int32_t c) -> const uint4 * { return c >= 0 && c < own ? src4 + c : &przm_zero16; };
     auto ent = [&](uint32_t dw) -> uint32_t { return tab[(dw >> (8u * j)) & 0xFFu]; };
     uint32_t x = 65536u, cnt = 0, nl = n;
     bool over = false;
     uint4 buf[RANS_PF];
 #pragma unroll
-    for (uint32_t d = 0; d < RANS_PF; d++) buf[d] = *chunk(NC - 1 - (int32_t)d);
+    for (uint32_t d = 0; d < RANS_PF; d++) {
+        const int32_t c = NC - 1 - (int32_t)d;
+        buf[d] = *(c >= 0 && c < own ? src4 + c : &przm_zero16);
+    }
     uint32_t e[4] = {ent(buf[0].w), ent(buf[0].z), ent(buf[0].y), ent(buf[0].x)};
     for (int32_t cb = NC - 1; cb >= 0; cb -= (int32_t)RANS_PF) {
-#pragma unroll
-        for (uint32_t d = 0; d < RANS_PF; d++) {
-            const int32_t c = cb - (int32_t)d;
-            const uint4 nb = buf[(d + 1) % RANS_PF];
-            const uint32_t en[4] = {ent(nb.w), ent(nb.z), ent(nb.y), ent(nb.x)};
-            const uint32_t g0 = 4u * (uint32_t)c;
-            count_step(g0 + 3, e[0], j, nl, x, cnt);
-            count_step(g0 + 2, e[1], j, nl, x, cnt);
-            count_step(g0 + 1, e[2], j, nl, x, cnt);
-            count_step(g0, e[3], j, nl, x, cnt);
-#pragma unroll
-            for (int u = 0; u < 4; u++) e[u] = en[u];
-            buf[d] = *chunk(c - (int32_t)RANS_PF);
-        }
+        if (16u * (uint32_t)cb + 16u <= nmin)
+            count_turn<false>(src4, own, cb, tab, j, nl, buf, e, x, cnt);
+        else
+            count_turn<true>(src4, own, cb, tab, j, nl, buf, e, x, cnt);
         uint32_t tot = cnt + QPERM(cnt, 0xB1);
         tot += QPERM(tot, 0x4E);
         if (tot >= limit) { over = true; nl = 0; }
@@ -485,15 +504,18 @@ przm_sheaf_count_kernel(uint8_t *__restrict__ sws, uint32_t nblocks, const uint3
     /* coded = matrix + 4 states + stream, taken only when shorter than the fiber */
     const uint32_t head = mat + 16u;
     const bool coded = has && n >= SHEAF_MIN_CODED && n > head;
-    uint32_t gmax = coded ? (n + 3u) >> 2 : 0u;
+    /* the wave's longest and shortest coded fibers */
+    uint32_t gmax = coded ? (n + 3u) >> 2 : 0u, nmin = coded ? n : 0xFFFFFFFFu;
     for (uint32_t o = 32; o >= 4; o >>= 1) {
         const uint32_t v = (uint32_t)__shfl_xor((int)gmax, (int)o, 64);
+        const uint32_t u = (uint32_t)__shfl_xor((int)nmin, (int)o, 64);
         gmax = v > gmax ? v : gmax;
+        nmin = u < nmin ? u : nmin;
     }
     __syncthreads();
     if (!has) return;
     const uint32_t K = coded ? vans_count_quad(sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP + fs, n, vt[q],
-                                               n - head, gmax)
+                                               n - head, gmax, nmin)
                              : PRZM_ENC_OVERFLOW;
     if (j == 0) H->k[m][k] = K;
 }
