@@ -313,6 +313,8 @@ __device__ __forceinline__ uint4 rev16(uint4 v) {
 
 #define RANS_PF 8u /* 16-byte symbol chunks in flight per quad (4 steps each) */
 
+template <bool B> struct RansLive { static constexpr bool value = B; };
+
 /* a readable zero chunk: the load address of chunks outside a job's stream */
 __device__ __align__(16) uint4 przm_zero16;
 
@@ -332,12 +334,14 @@ __device__ __forceinline__ uint32_t rans_entry(uint32_t g, uint32_t dw, uint32_t
 
 /* One coder step of quad lane j for step g with its table entry e (freq | cdf << 16), live for
  * symbols below nl; a lane that is not live has some entry of its table and changes nothing.
- * Updates the state x and the emitted byte count K. */
+ * Updates the state x and the emitted byte count K.  LIVE = false: the caller knows the step is
+ * live in every lane of the wave. */
+template <bool LIVE = true>
 __device__ __forceinline__ void rans_step(uint32_t g, uint32_t e, uint32_t j, uint32_t nl,
                                           uint8_t *__restrict__ stg, uint8_t *dummy,
                                           uint8_t *__restrict__ slot_end, uint32_t &x, uint32_t &K) {
     const uint32_t i = 4 * g + j;
-    const bool v = i < nl;
+    const bool v = !LIVE || i < nl;
     const uint32_t f = e & 0xFFFFu, cdf = e >> 16;
     const uint32_t t = 4096u * f - 1u;
     const bool r1 = v && x > t;
@@ -431,7 +435,10 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
         e[2] = rans_entry<M8>(g0 + 1, buf[0].y, buf[0].x, j, tab);
         e[3] = rans_entry<M8>(g0, buf[0].x, buf[1].w, j, tab);
     }
-    for (int32_t cb = NC - 1; cb >= 0; cb -= (int32_t)RANS_PF) {
+    /* one turn of the ring from chunk cb down; LIVE = false: every step of the turn is live in
+     * every lane and no quad reaches its cap in it, so neither is tested */
+    auto turn = [&](int32_t cb, auto live) {
+        constexpr bool LIVE = decltype(live)::value;
 #pragma unroll
         for (uint32_t d = 0; d < RANS_PF; d++) {
             const int32_t c = cb - (int32_t)d;
@@ -445,18 +452,31 @@ __device__ __forceinline__ uint32_t rans_quad(const uint8_t *__restrict__ src, u
             en[2] = rans_entry<M8>(gn + 1, nb.y, nb.x, j, tab);
             en[3] = rans_entry<M8>(gn, nb.x, nlw, j, tab);
             const uint32_t g0 = 4u * (uint32_t)c;
-            rans_step(g0 + 3, e[0], j, nl, stg, dummy, slot_end, x, K);
-            rans_step(g0 + 2, e[1], j, nl, stg, dummy, slot_end, x, K);
-            rans_step(g0 + 1, e[2], j, nl, stg, dummy, slot_end, x, K);
-            rans_step(g0, e[3], j, nl, stg, dummy, slot_end, x, K);
+            rans_step<LIVE>(g0 + 3, e[0], j, nl, stg, dummy, slot_end, x, K);
+            rans_step<LIVE>(g0 + 2, e[1], j, nl, stg, dummy, slot_end, x, K);
+            rans_step<LIVE>(g0 + 1, e[2], j, nl, stg, dummy, slot_end, x, K);
+            rans_step<LIVE>(g0, e[3], j, nl, stg, dummy, slot_end, x, K);
 #pragma unroll
             for (int u = 0; u < 4; u++) e[u] = en[u];
             buf[d] = *chunk(c - (int32_t)RANS_PF);
-            if (K > cap) { /* the job is not chosen any more: stop it */
+            if (LIVE && K > cap) { /* the job is not chosen any more: stop it */
                 over = true;
                 nl = 0;
             }
         }
+    };
+    /* the wave's shortest job: a turn below it is live in every lane (idle quads have n = 0) */
+    uint32_t nmin = n;
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t y = (uint32_t)__shfl_xor((int)nmin, o);
+        nmin = y < nmin ? y : nmin;
+    }
+    for (int32_t cb = NC - 1; cb >= 0; cb -= (int32_t)RANS_PF) {
+        /* a turn emits at most 2 bytes per lane and step: 256 per quad */
+        if (16u * (uint32_t)cb + 16u <= nmin && __all(K + 256u <= cap))
+            turn(cb, RansLive<false>{});
+        else
+            turn(cb, RansLive<true>{});
     }
     if (!over) { /* the last, partial run [128 (K >> 7), K) */
         const uint32_t base = K & ~127u;
