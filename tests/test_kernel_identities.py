@@ -49,6 +49,24 @@ def test_biased_reciprocal_divmod():
             assert np.array_equal(rr, (x % f).astype(np.int64)), f
 
 
+def test_identity_entry_step():
+    """count_step / rans_step: a step that is not live codes the entry freq 4096, cdf 0.  For
+    every state below 2^24 that step renormalises nothing (x <= 4096 * 4096 - 1) and the
+    biased-reciprocal divmod gives back x itself, with the device rcp exact or one ulp off."""
+    x = np.arange(1 << 24, dtype=np.int64)
+    f = 4096
+    assert (x <= 4096 * f - 1).all()
+    bias = np.float64(2.0 ** -9)
+    r0 = np.float32(1.0) / np.float32(f)
+    for r in (np.nextafter(r0, np.float32(0)), r0, np.nextafter(r0, np.float32(1))):
+        qq = (x.astype(np.float32).astype(np.float64) * np.float64(r) + bias).astype(np.float32).astype(np.int64)
+        rr = x - qq * f
+        low = rr < 0
+        qq = np.where(low, qq - 1, qq)
+        rr = np.where(low, rr + f, rr)
+        assert np.array_equal(qq * 4096 + 0 + rr, x)
+
+
 def _sad0(x):
     """v_sad_u8(x, 0, 0): the sum of the four bytes of x"""
     return sum((x >> (8 * i)) & 0xFF for i in range(4))
