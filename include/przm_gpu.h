@@ -216,7 +216,7 @@ int32_t przm_gpu_encode_device(przm_gpu *g, const void *d_src, uint64_t n, int32
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc); it has no counterpart in the reference.  Allocates
  * nothing: every intermediate buffer lives in one static device array of the
- * library (about 56 MB per GPU, held until the process ends), so n is at most
+ * library (53.2 MB per GPU, held until the process ends), so n is at most
  * 1 MiB (16 frames; PRZM_GPU_ERR_CAPACITY above), out_cap must hold
  * przm_gpu_frames_bound(n) and frame_sizes is required.  All calls on one GPU
  * share that array: calls that overlap on one GPU (from two handles or two
