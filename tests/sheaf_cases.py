@@ -166,3 +166,40 @@ def modal_text(mode, n, seed):
     while len(out) < n:
         out.append(step(rng, out[-2] if len(out) > 1 else 0, out[-1]))
     return np.frombuffer(bytes(out[:n]), np.uint8).copy()
+
+
+def _next_mode0(rng, p2, p1):
+    """a byte class after the class of p1 (white space, '"' ':', other syntax, identifier,
+    digit), as mode 0's contexts tell them apart"""
+    groups = (b" \n", b'":', b"{},.-", b"abcdefghijk", b"0123456789")
+    k = 0 if p1 in groups[0] else 1 if p1 in groups[1] else 2 if p1 in groups[2] else \
+        3 if p1 in groups[3] else 4 if p1 in groups[4] else 3
+    pr = ((0, .1, 0, .8, .1), (.2, 0, 0, .5, .3), (.4, .3, 0, 0, .3), (.15, 0, .15, .7, 0), (.1, 0, .3, 0, .6))[k]
+    g = groups[rng.choice(5, p=pr)]
+    return g[rng.integers(len(g))]
+
+
+def unique_text(mode, n, seed):
+    """n bytes drawn like modal_text (mode 0: by the class of the byte before) in which no
+    4-byte string occurs twice (a draw that would repeat one is drawn again, and after 32
+    draws the first byte that does not is taken), so a block has no match of its own: a block
+    that repeats its first T bytes has exactly T literals, in contexts that make the sheaf
+    containers of `mode` win."""
+    rng = np.random.default_rng(seed)
+    step = {0: _next_mode0, 1: _next_mode1, 2: _next_mode2, 3: _next_mode3}[mode]
+    out = bytearray(b"a")
+    seen = set()
+    while len(out) < n:
+        tail = bytes(out[-3:])
+        c = None
+        for _ in range(32):
+            d = step(rng, out[-2] if len(out) > 1 else 0, out[-1])
+            if len(out) < 3 or tail + bytes([d]) not in seen:
+                c = d
+                break
+        if c is None:
+            c = next(b for b in range(256) if tail + bytes([b]) not in seen)
+        out.append(c)
+        if len(out) >= 4:
+            seen.add(bytes(out[-4:]))
+    return np.frombuffer(bytes(out[:n]), np.uint8).copy()

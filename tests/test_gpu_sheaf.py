@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle_lib import oracle, refdec, have_refdec
-from sheaf_cases import modal_text
+from sheaf_cases import modal_text, unique_text
 
 from przm_amd import datagen as D
 from przm_amd import decode as X
@@ -95,6 +95,26 @@ def test_levels_equal_oracle(g, level):
     a = np.concatenate([D.mixture(1 << 20, 95 + level), _json(300000, 96 + level)])
     frames, sizes, _ = g.CompressFrames(a, level)
     assert frames == oracle().encode_buffer(a, level, False, True)[0]
+
+
+def test_literal_round_boundaries(g):
+    """Blocks of 64 KiB that repeat their first T bytes of a text with no repeated 4-byte
+    string, so each has exactly T literals (checked with the oracle's tokenizer): literal
+    counts at the split kernel's 4,096-literal rounds and around them, up to a block of
+    literals only, in contexts that make each mode's containers win."""
+    O = oracle()
+    targets = (1, 15, 16, 17, 21, 22, 23, 4095, 4096, 4097, 8191, 8192, 8193, 20480, 40961, 65535, 65536)
+    for mode in range(4):
+        src = unique_text(mode, 65536, 800 + mode)
+        blocks = [np.resize(src[:t], 65536) for t in targets]
+        counts = [O.tokenize(b)["lit"].size for b in blocks]
+        assert {4095, 4096, 4097, 65536} <= set(counts), (mode, counts)
+        a = np.concatenate(blocks)
+        frames, sizes, _ = g.CompressFrames(a)
+        of, osz = O.encode_buffer(a, 6, False, True)
+        assert np.array_equal(sizes, osz), mode
+        assert frames == of, mode
+        assert 2 + mode in set(_kinds(frames, sizes)), (mode, _kinds(frames, sizes))
 
 
 def test_batched_multichunk_equals_oracle(g):
