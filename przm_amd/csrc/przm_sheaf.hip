@@ -291,14 +291,16 @@ przm_sheaf_split_kernel(const uint8_t *__restrict__ ws, const przm_meta_t *__res
     __syncthreads();
     /* the (mode, fiber) histograms: thread s sums bin s over the signatures of each fiber */
     for (uint32_t sb_ = tid; sb_ < 256; sb_ += SHEAF_NT) {
-        uint32_t h[8], acc[16];
-        for (uint32_t g = 0; g < 8; g++) h[g] = hs[g * 256u + sb_];
+        uint32_t acc[16] = {};
 #pragma unroll
-        for (uint32_t r = 0; r < 16; r++) acc[r] = 0;
+        for (uint32_t g = 0; g < 8; g++) {
+            const uint32_t h = hs[g * 256u + sb_];
+            acc[sheaf_fiber(0, g)] += h;
+            acc[4u + sheaf_fiber(1, g)] += h;
+            acc[8u + sheaf_fiber(2, g)] += h;
+            acc[12u + sheaf_fiber(3, g)] += h;
+        }
 #pragma unroll
-        for (uint32_t m = 0; m < 4; m++)
-#pragma unroll
-            for (uint32_t g = 0; g < 8; g++) acc[4u * m + sheaf_fiber(m, g)] += h[g];
         for (uint32_t r = 0; r < 16; r++) cnt[r][sb_] = acc[r];
     }
     __syncthreads();
