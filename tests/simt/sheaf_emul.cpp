@@ -1,0 +1,75 @@
+/*
+ * The sheaf split kernel (przm_amd/csrc/przm_sheaf.hip, przm_sheaf_split_kernel) and the
+ * histogram kernel (przm_amd/csrc/przm_entropy.hip, przm_hist_kernel) run on the CPU through
+ * the fiber emulation of include/hip/hip_runtime.h, for tests/test_simt_kernels.py.  Test
+ * infrastructure: built with g++ into a temporary directory by that test.
+ */
+#include <hip/hip_runtime.h>
+
+#include "../../przm_amd/csrc/przm_entropy.hip"
+#include "../../przm_amd/csrc/przm_sheaf.hip"
+
+#include <cstddef>
+#include <vector>
+
+extern "C" {
+
+/* layout constants the test reads the outputs with */
+uint64_t simt_const(int which) {
+    switch (which) {
+    case 0: return PRZM_SWS_BYTES;
+    case 1: return PRZM_SWS_HDR;
+    case 2: return PRZM_SWS_FTAB;
+    case 3: return PRZM_SWS_FIB;
+    case 4: return PRZM_SWS_FIB_CAP;
+    case 5: return sizeof(przm_ftab_t);
+    case 6: return offsetof(przm_sheaf_t, n);
+    case 7: return offsetof(przm_sheaf_t, l);
+    case 8: return offsetof(przm_sheaf_t, mat);
+    case 9: return offsetof(przm_ftab_t, mat);
+    case 10: return sizeof(przm_meta_t);
+    case 11: return offsetof(przm_meta_t, mat_len);
+    default: return 0;
+    }
+}
+
+/* przm_sheaf_split_kernel over nblocks blocks whose literal streams are lits[off[b], off[b + 1]);
+ * sws: nblocks * PRZM_SWS_BYTES bytes, filled by the caller (the kernel writes only its outputs).
+ * order: 0 ascending, 1 descending, 2 + seed random thread order between synchronisations. */
+int simt_sheaf_split(const uint8_t *lits, const uint32_t *off, uint32_t nblocks, uint8_t *sws, int order) {
+    std::vector<uint8_t> ws((size_t)nblocks * PRZM_WS_BYTES + 64, 0);
+    std::vector<przm_meta_t> meta(nblocks);
+    memset(meta.data(), 0, meta.size() * sizeof(przm_meta_t));
+    for (uint32_t b = 0; b < nblocks; b++) {
+        const uint32_t n = off[b + 1] - off[b];
+        memcpy(ws.data() + (size_t)b * PRZM_WS_BYTES + PRZM_WS_LIT, lits + off[b], n);
+        meta[b].len[PRZM_S_LIT] = n;
+    }
+    return simt::launch(nblocks, SHEAF_NT, order,
+                        [&]() { przm_sheaf_split_kernel(ws.data(), meta.data(), sws, nblocks); });
+}
+
+/* przm_hist_kernel over nblocks blocks whose four streams (lit, op, meta, var) are
+ * streams[off[4 b + k], off[4 b + k + 1]); ftab: nblocks * 4 tables out, mat_len: nblocks * 4 */
+int simt_hist(const uint8_t *streams, const uint32_t *off, uint32_t nblocks, uint8_t *ftab, uint32_t *mat_len,
+              int order) {
+    std::vector<uint8_t> ws((size_t)nblocks * PRZM_WS_BYTES + 64, 0);
+    std::vector<przm_meta_t> meta(nblocks);
+    memset(meta.data(), 0, meta.size() * sizeof(przm_meta_t));
+    const uint32_t slot[4] = {PRZM_WS_LIT, PRZM_WS_OP, PRZM_WS_META, PRZM_WS_VAR};
+    const uint32_t sid[4] = {PRZM_S_LIT, PRZM_S_OP, PRZM_S_META, PRZM_S_VAR};
+    for (uint32_t b = 0; b < nblocks; b++)
+        for (int k = 0; k < 4; k++) {
+            const uint32_t n = off[4 * b + k + 1] - off[4 * b + k];
+            memcpy(ws.data() + (size_t)b * PRZM_WS_BYTES + slot[k], streams + off[4 * b + k], n);
+            meta[b].len[sid[k]] = n;
+        }
+    const int rc = simt::launch(nblocks, 256, order, [&]() {
+        przm_hist_kernel(ws.data(), meta.data(), (przm_ftab_t *)ftab, nblocks);
+    });
+    for (uint32_t b = 0; b < nblocks; b++)
+        for (int k = 0; k < 4; k++) mat_len[4 * b + k] = meta[b].mat_len[k];
+    return rc;
+}
+
+} // extern "C"

@@ -1,0 +1,220 @@
+"""Two workgroup kernels run on the CPU under a fiber emulation of HIP (tests/simt): every GPU
+thread is a fiber, __syncthreads a barrier over the workgroup, and every wave operation (DPP
+moves, readlane, ballot, shuffles, the wave barrier) a rendezvous of the wave's lanes.  The
+scheduler resumes the threads in ascending, descending or random order between
+synchronisations, so a missing barrier (a thread reading LDS another thread has not written
+yet, or overwriting what another has not read) gives a different result under some order.
+
+  * przm_sheaf_split_kernel (przm_amd/csrc/przm_sheaf.hip): the fibers of every mode
+    (DecomposeFibers, core/literals/sheaf.go:125-152, with the context classes of the oracle),
+    their lengths, the zero bytes after each fiber up to its 16-byte boundary, and the
+    normalised table and matrix of every fiber long enough to be coded (NormalizeFrequencies
+    and the joint-nibble matrix, literals.go:78-184, from the oracle), for literal counts at and
+    around the kernel's 4,096-literal rounds.
+  * przm_hist_kernel (przm_amd/csrc/przm_entropy.hip): the tables and matrix lengths of the
+    four streams of a block, against the oracle's.
+
+The GPU parity tests check the same kernels bit for bit on the device; this checks their
+synchronisation on the CPU, where a schedule can be chosen.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle_lib import oracle, ptr
+from sheaf_cases import modal_text, unique_text
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "simt", "sheaf_emul.cpp")
+ORDERS = (0, 1, 7)  # ascending, descending, random (seed 7)
+MIN_CODED = 22
+
+
+def build(out, src=SRC, include=os.path.join(HERE, "simt", "include")):
+    subprocess.run(["g++", "-std=c++17", "-O1", "-fPIC", "-shared", "-I", include, "-o", out, src], check=True)
+
+
+def load(out):
+    lib = ctypes.CDLL(out)
+    lib.simt_const.restype = ctypes.c_uint64
+    lib.simt_const.argtypes = [ctypes.c_int]
+    lib.simt_sheaf_split.restype = ctypes.c_int
+    lib.simt_sheaf_split.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int]
+    lib.simt_hist.restype = ctypes.c_int
+    lib.simt_hist.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.c_int]
+    return lib
+
+
+@pytest.fixture(scope="module")
+def simt(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("simt") / "libsimt.so")
+    build(out)
+    return load(out)
+
+
+def class_table():
+    """cls[m][p2 * 256 + p1]: the fiber of a literal after (p2, p1) in mode m (the oracle's
+    ContextClassify / ContextClassifyOrder2)"""
+    O = oracle()
+    return np.array([[O.sheaf_class(p2, p1, m) for p2 in range(256) for p1 in range(256)] for m in range(4)],
+                    np.uint8)
+
+
+@pytest.fixture(scope="module")
+def classes():
+    return class_table()
+
+
+def _table(counts):
+    """the oracle's normalised frequencies, freq | cdf << 16, and matrix of a histogram"""
+    O = oracle()
+    f = O.normalize(counts).astype(np.uint32)
+    cdf = np.concatenate([[0], np.cumsum(f)[:-1]]).astype(np.uint32)
+    return f | (cdf << 16), O.matrix(f.astype(np.uint16))
+
+
+def _lit_blocks():
+    rng = np.random.default_rng(31)
+    src = {m: unique_text(m, 9000, 900 + m) for m in range(4)}
+    out = [np.zeros(0, np.uint8), src[0][:1], src[1][:17], src[2][:21], src[3][:22]]
+    out += [src[0][:4095], src[1][:4096], src[2][:4097], src[3][:9000]]
+    out += [modal_text(3, 5000, 41), rng.integers(0, 256, 8200, dtype=np.uint8)]
+    return out
+
+
+def _split(lib, blocks, order):
+    off = np.zeros(len(blocks) + 1, np.uint32)
+    off[1:] = np.cumsum([b.size for b in blocks])
+    lits = np.concatenate(blocks + [np.zeros(16, np.uint8)])
+    sws_bytes = int(lib.simt_const(0))
+    sws = np.full(len(blocks) * sws_bytes, 0xA5, np.uint8)  # the kernel must write every byte it owns
+    assert lib.simt_sheaf_split(ptr(lits), ptr(off), len(blocks), ptr(sws), order) == 0
+    return sws.reshape(len(blocks), sws_bytes)
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_sheaf_split_under_schedules(simt, classes, order):
+    check_split(simt, classes, order)
+
+
+def check_split(lib, classes, order):
+    hdr, ftab, fib, cap = (int(lib.simt_const(i)) for i in (1, 2, 3, 4))
+    ftsz, off_n, off_l, off_mat, off_fmat = (int(lib.simt_const(i)) for i in (5, 6, 7, 8, 9))
+    blocks = _lit_blocks()
+    sws = _split(lib, blocks, order)
+    for bi, lits in enumerate(blocks):
+        S = sws[bi]
+        u32 = lambda o: int(S[o: o + 4].view(np.uint32)[0])
+        assert u32(hdr + off_n) == lits.size, bi
+        p2 = np.concatenate([[0, 0], lits[:-2]]).astype(np.int64) if lits.size else np.zeros(0, np.int64)
+        p1 = np.concatenate([[0], lits[:-1]]).astype(np.int64) if lits.size else np.zeros(0, np.int64)
+        for m in range(4):
+            cl = classes[m][p2[: lits.size] * 256 + p1[: lits.size]]
+            pos = fib + m * cap
+            for k in range(4):
+                want = lits[cl == k]
+                r = 4 * m + k
+                assert u32(hdr + off_l + 4 * r) == want.size, (bi, m, k)
+                got = S[pos: pos + want.size]
+                assert np.array_equal(got, want), (bi, m, k, order)
+                pad = (-want.size) % 16
+                assert not S[pos + want.size: pos + want.size + pad].any(), (bi, m, k, "padding")
+                pos += want.size + pad
+                mat_len = u32(hdr + off_mat + 4 * r)
+                if want.size < MIN_CODED:
+                    assert mat_len == 0, (bi, m, k)
+                    continue
+                fc, mat = _table(np.bincount(want, minlength=256))
+                t = ftab + r * ftsz
+                assert np.array_equal(S[t: t + 1024].view(np.uint32), fc), (bi, m, k, "table")
+                assert mat_len == len(mat), (bi, m, k, "matrix length")
+                assert S[t + off_fmat: t + off_fmat + mat_len].tobytes() == mat, (bi, m, k, "matrix")
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_hist_under_schedules(simt, order):
+    check_hist(simt, order)
+
+
+def check_hist(lib, order):
+    rng = np.random.default_rng(32)
+    streams = [unique_text(0, 3000, 33), modal_text(1, 700, 34), rng.integers(0, 40, 15, dtype=np.uint8),
+               rng.integers(0, 256, 2500, dtype=np.uint8),
+               np.zeros(0, np.uint8), unique_text(2, 64, 35), modal_text(2, 4100, 36), np.full(300, 7, np.uint8)]
+    nb = len(streams) // 4
+    off = np.zeros(len(streams) + 1, np.uint32)
+    off[1:] = np.cumsum([s.size for s in streams])
+    data = np.concatenate(streams + [np.zeros(16, np.uint8)])
+    ftsz, off_fmat = int(lib.simt_const(5)), int(lib.simt_const(9))
+    ftab = np.zeros(nb * 4 * ftsz, np.uint8)
+    mat_len = np.zeros(nb * 4, np.uint32)
+    assert lib.simt_hist(ptr(data), ptr(off), nb, ptr(ftab), ptr(mat_len), order) == 0
+    for j, s in enumerate(streams):
+        if s.size < (1 if j % 4 == 0 else 16):  # the kernel leaves short streams alone
+            continue
+        fc, mat = _table(np.bincount(s, minlength=256))
+        t = ftab[j * ftsz: (j + 1) * ftsz]
+        assert np.array_equal(t[:1024].view(np.uint32), fc), (j, order)
+        assert mat_len[j] == len(mat) and t[off_fmat: off_fmat + len(mat)].tobytes() == mat, (j, order)
+
+
+# Mutants: one synchronisation taken out of a copy of the kernel sources each.  Every one must
+# make some schedule give a wrong result (or stop: a deadlock or a crash in the emulation, which
+# the subprocess reports as a failure).  The split kernel's barrier before the scatter pass has
+# no mutant: the first round writes the staging only after its own first barrier, which every
+# wave reaches after its normalisations, so that barrier orders the reuse as well.
+MUTANTS = [
+    ("split_no_sync_after_scan", "przm_sheaf.hip",
+     "    if (lane == 63u) *(uint4 *)part[w] = make_uint4(in4[0], in4[1], in4[2], in4[3]);\n    __syncthreads();\n",
+     "    if (lane == 63u) *(uint4 *)part[w] = make_uint4(in4[0], in4[1], in4[2], in4[3]);\n"),
+    ("split_no_sync_after_staging", "przm_sheaf.hip",
+     "            P[m] += 1ull << sh;\n        }\n    }\n    __syncthreads();\n",
+     "            P[m] += 1ull << sh;\n        }\n    }\n"),
+    ("split_no_sync_after_copy", "przm_sheaf.hip",
+     "        my_dst = F.dst[tid] + 16u * F.nw[tid];\n    }\n    __syncthreads();\n",
+     "        my_dst = F.dst[tid] + 16u * F.nw[tid];\n    }\n"),
+    ("split_no_sync_after_counting", "przm_sheaf.hip",
+     "        else if (i0 < n) sheaf_hist_round<false>(L, n, i0, kind, sigw, hs);\n    }\n    __syncthreads();\n",
+     "        else if (i0 < n) sheaf_hist_round<false>(L, n, i0, kind, sigw, hs);\n    }\n"),
+    ("hist_no_wave_barrier", "przm_entropy.hip",
+     "    for (int i = 0; i < 4; i++) cnt[w][lane * 4 + i] = 0;\n    __builtin_amdgcn_wave_barrier();\n",
+     "    for (int i = 0; i < 4; i++) cnt[w][lane * 4 + i] = 0;\n"),
+]
+
+RUNNER = """
+import sys
+sys.path.insert(0, %r)
+import test_simt_kernels as T
+lib = T.load(sys.argv[1])
+cls = T.class_table()
+bad = []
+for order in T.ORDERS:
+    for name, fn in (("split", lambda: T.check_split(lib, cls, order)), ("hist", lambda: T.check_hist(lib, order))):
+        try:
+            fn()
+        except AssertionError:
+            bad.append((name, order))
+print(bad)
+sys.exit(1 if bad else 0)
+"""
+
+
+@pytest.mark.parametrize("name,fname,old,new", MUTANTS, ids=[m[0] for m in MUTANTS])
+def test_missing_synchronisation_is_caught(tmp_path, name, fname, old, new):
+    import shutil
+    root = os.path.dirname(HERE)
+    csrc = tmp_path / "przm_amd" / "csrc"
+    shutil.copytree(os.path.join(root, "przm_amd", "csrc"), csrc)
+    text = (csrc / fname).read_text()
+    assert text.count(old) == 1, name
+    (csrc / fname).write_text(text.replace(old, new))
+    (tmp_path / "tests" / "simt").mkdir(parents=True)
+    shutil.copy(SRC, tmp_path / "tests" / "simt" / "sheaf_emul.cpp")
+    out = str(tmp_path / "libmut.so")
+    build(out, str(tmp_path / "tests" / "simt" / "sheaf_emul.cpp"))
+    r = subprocess.run(["python3", "-c", RUNNER % HERE, out], capture_output=True, text=True, timeout=600)
+    assert r.returncode != 0, (name, r.stdout[-500:])
