@@ -13,9 +13,11 @@
  *    rendezvous of the wave's live lanes, in the order each lane reaches them.  They must be
  *    reached by every live lane of the wave (uniform control flow per wave), which holds for
  *    the kernels this is used on;
- *  - the scheduler resumes runnable fibers in ascending, descending or seeded random order,
- *    each fiber running until it blocks at a synchronisation or exits, and reports a deadlock
- *    when a full sweep makes no progress;
+ *  - the scheduler runs the fibers by sweeps (each live fiber resumed once per sweep, in
+ *    ascending, descending or seeded random order: the waves move about in step) or greedily
+ *    (always the first fiber, in ascending or descending thread order, that can go on: a wave
+ *    runs ahead until a barrier stops it); a fiber runs until it blocks at a synchronisation or
+ *    exits, and a schedule that can make no progress is reported as a deadlock;
  *  - atomics are plain read-modify-writes (one OS thread runs every fiber).
  * DPP controls: quad_perm (0x00-0xFF), row_shr:1..15 (0x111-0x11F), row_bcast:15 (0x142),
  * row_bcast:31 (0x143), with row masks; a lane whose source is invalid keeps `old` (bound_ctrl
@@ -57,6 +59,8 @@ struct Fiber {
     std::vector<char> stack;
     uint32_t tid = 0;
     bool done = false;
+    int wait = 0;      /* 0 running, 1 at a barrier, 2 at a wave operation */
+    uint64_t wait_gen = 0;
 };
 
 struct Wave {
@@ -96,7 +100,10 @@ inline void barrier() {
         G.bar_gen++;
         return;
     }
+    me().wait = 1;
+    me().wait_gen = gen;
     while (G.bar_gen == gen) yield();
+    me().wait = 0;
 }
 
 /* the wave's values of v, after every live lane of the wave has given its own; *mask: the lanes
@@ -115,16 +122,28 @@ inline const uint64_t *wave_xchg(uint64_t v, uint64_t *mask = nullptr) {
         W.arrived = 0;
         W.gen++;
     } else {
+        me().wait = 2;
+        me().wait_gen = gen;
         while (W.gen == gen) yield();
+        me().wait = 0;
     }
     if (mask) *mask = W.mask[b];
     return W.vals[b];
 }
 
-/* run body on a grid of nblk workgroups of nthr threads; order 0 ascending, 1 descending,
- * 2 + seed: a random order per sweep */
+/* fiber F can go on */
+inline bool ready(const Group &G, const Fiber &F) {
+    if (F.done) return false;
+    if (F.wait == 1) return G.bar_gen != F.wait_gen;
+    if (F.wait == 2) return G.waves[F.tid / 64].gen != F.wait_gen;
+    return true;
+}
+
+/* run body on a grid of nblk workgroups of nthr threads.  order: 0 / 1 sweeps in ascending /
+ * descending thread order, 2 / 3 greedy in ascending / descending thread order, 4 and above
+ * sweeps in a random order per sweep (seeded by order) */
 inline int launch(uint32_t nblk, uint32_t nthr, int order, const std::function<void()> &body) {
-    std::mt19937 rng((uint32_t)(order > 1 ? order : 0));
+    std::mt19937 rng((uint32_t)(order > 3 ? order : 0));
     for (uint32_t b = 0; b < nblk; b++) {
         Group G;
         G.block_idx = Dim{b, 0, 0};
@@ -159,9 +178,20 @@ inline int launch(uint32_t nblk, uint32_t nthr, int order, const std::function<v
             }, 0);
         }
         std::vector<uint32_t> ord(nthr);
-        for (uint32_t t = 0; t < nthr; t++) ord[t] = order == 1 ? nthr - 1 - t : t;
+        for (uint32_t t = 0; t < nthr; t++) ord[t] = (order == 1 || order == 3) ? nthr - 1 - t : t;
+        while (G.live && (order == 2 || order == 3)) {
+            uint32_t k = 0;
+            while (k < nthr && !ready(G, G.fibers[ord[k]])) k++;
+            if (k == nthr) {
+                fprintf(stderr, "simt: deadlock in workgroup %u (%u threads live)\n", b, G.live);
+                g_grp = nullptr;
+                return -1;
+            }
+            G.cur = (int)ord[k];
+            swapcontext(&G.sched, &G.fibers[ord[k]].ctx);
+        }
         while (G.live) {
-            if (order > 1) std::shuffle(ord.begin(), ord.end(), rng);
+            if (order > 3) std::shuffle(ord.begin(), ord.end(), rng);
             const uint64_t ev = G.events;
             for (uint32_t k = 0; k < nthr; k++) {
                 Fiber &F = G.fibers[ord[k]];

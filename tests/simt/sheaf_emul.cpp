@@ -1,12 +1,14 @@
 /*
- * The sheaf split kernel (przm_amd/csrc/przm_sheaf.hip, przm_sheaf_split_kernel) and the
- * histogram kernel (przm_amd/csrc/przm_entropy.hip, przm_hist_kernel) run on the CPU through
- * the fiber emulation of include/hip/hip_runtime.h, for tests/test_simt_kernels.py.  Test
- * infrastructure: built with g++ into a temporary directory by that test.
+ * The sheaf split kernel (przm_amd/csrc/przm_sheaf.hip, przm_sheaf_split_kernel), the histogram
+ * kernel (przm_amd/csrc/przm_entropy.hip, przm_hist_kernel) and the offsets prep kernel
+ * (przm_amd/csrc/przm_offsets.hip, przm_offs_prep_kernel) run on the CPU through the fiber
+ * emulation of include/hip/hip_runtime.h, for tests/test_simt_kernels.py.  Test infrastructure:
+ * built with g++ into a temporary directory by that test.
  */
 #include <hip/hip_runtime.h>
 
 #include "../../przm_amd/csrc/przm_entropy.hip"
+#include "../../przm_amd/csrc/przm_offsets.hip"
 #include "../../przm_amd/csrc/przm_sheaf.hip"
 
 #include <cstddef>
@@ -29,6 +31,11 @@ uint64_t simt_const(int which) {
     case 9: return offsetof(przm_ftab_t, mat);
     case 10: return sizeof(przm_meta_t);
     case 11: return offsetof(przm_meta_t, mat_len);
+    case 12: return PRZM_OWS_BYTES;
+    case 13: return PRZM_OWS_FTAB;
+    case 14: return PRZM_OWS_CODES;
+    case 15: return PRZM_OWS_EXTRA;
+    case 16: return sizeof(przm_offs_t);
     default: return 0;
     }
 }
@@ -70,6 +77,21 @@ int simt_hist(const uint8_t *streams, const uint32_t *off, uint32_t nblocks, uin
     for (uint32_t b = 0; b < nblocks; b++)
         for (int k = 0; k < 4; k++) mat_len[4 * b + k] = meta[b].mat_len[k];
     return rc;
+}
+
+/* przm_offs_prep_kernel over nblocks blocks whose LEB128 offset streams are
+ * offs[off[b], off[b + 1]); ows: nblocks * PRZM_OWS_BYTES bytes, filled by the caller */
+int simt_offs_prep(const uint8_t *offs, const uint32_t *off, uint32_t nblocks, uint8_t *ows, int order) {
+    std::vector<uint8_t> ws((size_t)nblocks * PRZM_WS_BYTES + 64, 0);
+    std::vector<przm_meta_t> meta(nblocks);
+    memset(meta.data(), 0, meta.size() * sizeof(przm_meta_t));
+    for (uint32_t b = 0; b < nblocks; b++) {
+        const uint32_t n = off[b + 1] - off[b];
+        memcpy(ws.data() + (size_t)b * PRZM_WS_BYTES + PRZM_WS_OFF, offs + off[b], n);
+        meta[b].len[PRZM_S_OFF] = n;
+    }
+    return simt::launch(nblocks, OFFS_NT, order,
+                        [&]() { przm_offs_prep_kernel(ws.data(), meta.data(), ows, nblocks); });
 }
 
 } // extern "C"

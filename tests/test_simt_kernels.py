@@ -2,8 +2,10 @@
 thread is a fiber, __syncthreads a barrier over the workgroup, and every wave operation (DPP
 moves, readlane, ballot, shuffles, the wave barrier) a rendezvous of the wave's lanes.  The
 scheduler resumes the threads in ascending, descending or random order between
-synchronisations, so a missing barrier (a thread reading LDS another thread has not written
-yet, or overwriting what another has not read) gives a different result under some order.
+synchronisations, by sweeps (the waves move about in step) or greedily (a wave runs ahead
+until a barrier stops it), so a missing barrier (a thread reading LDS another thread has not
+written yet, or overwriting what another has not read) gives a different result under some
+order.
 
   * przm_sheaf_split_kernel (przm_amd/csrc/przm_sheaf.hip): the fibers of every mode
     (DecomposeFibers, core/literals/sheaf.go:125-152, with the context classes of the oracle),
@@ -13,6 +15,9 @@ yet, or overwriting what another has not read) gives a different result under so
     around the kernel's 4,096-literal rounds.
   * przm_hist_kernel (przm_amd/csrc/przm_entropy.hip): the tables and matrix lengths of the
     four streams of a block, against the oracle's.
+  * przm_offs_prep_kernel (przm_amd/csrc/przm_offsets.hip): the distance codes, the extra bits
+    (MSB first) and the codes' table of LEB128 offset streams over several 1,024-byte rounds,
+    against the oracle's DistanceToCode (distance.go:171-196) and BitWriter (distance.go:217-247).
 
 The GPU parity tests check the same kernels bit for bit on the device; this checks their
 synchronisation on the CPU, where a schedule can be chosen.
@@ -29,7 +34,7 @@ from sheaf_cases import modal_text, unique_text
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "simt", "sheaf_emul.cpp")
-ORDERS = (0, 1, 7)  # ascending, descending, random (seed 7)
+ORDERS = (0, 1, 2, 3, 7)  # sweeps ascending / descending, greedy ascending / descending, random sweeps
 MIN_CODED = 22
 
 
@@ -46,6 +51,8 @@ def load(out):
     lib.simt_hist.restype = ctypes.c_int
     lib.simt_hist.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                               ctypes.c_int]
+    lib.simt_offs_prep.restype = ctypes.c_int
+    lib.simt_offs_prep.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int]
     return lib
 
 
@@ -162,6 +169,54 @@ def check_hist(lib, order):
         assert mat_len[j] == len(mat) and t[off_fmat: off_fmat + len(mat)].tobytes() == mat, (j, order)
 
 
+def _leb128(ds):
+    out = bytearray()
+    for d in ds:
+        d = int(d)
+        while d >= 0x80:
+            out.append((d & 0x7F) | 0x80)
+            d >>= 7
+        out.append(d)
+    return np.frombuffer(bytes(out), np.uint8)
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_offs_prep_under_schedules(simt, order):
+    check_offs(simt, order)
+
+
+def check_offs(lib, order):
+    O = oracle()
+    rng = np.random.default_rng(37)
+    dists = [np.zeros(0, np.int64), rng.integers(1, 65536, 10), rng.integers(1, 5, 63), rng.integers(1, 65536, 64),
+             rng.integers(1, 300, 1000), np.concatenate([rng.integers(1, 5, 700), rng.integers(16000, 65536, 2300)])]
+    streams = [_leb128(d) for d in dists]
+    off = np.zeros(len(streams) + 1, np.uint32)
+    off[1:] = np.cumsum([s.size for s in streams])
+    data = np.concatenate(streams + [np.zeros(16, np.uint8)])
+    ows_bytes, o_ftab, o_codes, o_extra = (int(lib.simt_const(i)) for i in (12, 13, 14, 15))
+    ftsz, off_fmat = int(lib.simt_const(5)), int(lib.simt_const(9))
+    ows = np.full(len(streams) * ows_bytes, 0xA5, np.uint8)
+    assert lib.simt_offs_prep(ptr(data), ptr(off), len(streams), ptr(ows), order) == 0
+    for j, (d, st) in enumerate(zip(dists, streams)):
+        B = ows[j * ows_bytes: (j + 1) * ows_bytes]
+        H = B[:64].view(np.uint32)  # n, orig, extra, m12_len, m12_state, v_len, v_state[4], mat_len
+        cz = [O.dist_code(int(x)) for x in d]
+        codes = np.array([c for c, _, _ in cz], np.uint8)
+        bits = "".join(format(e, "0%db" % nb) for _, e, nb in cz if nb)
+        bits += "0" * ((-len(bits)) % 8)
+        extra = bytes(int(bits[i: i + 8], 2) for i in range(0, len(bits), 8))
+        assert (H[0], H[1], H[2]) == (d.size, st.size, len(extra)), (j, order)
+        assert B[o_codes: o_codes + d.size].tobytes() == codes.tobytes(), (j, order)
+        assert B[o_extra: o_extra + len(extra)].tobytes() == extra, (j, order)
+        if d.size >= 64:
+            fc, mat = _table(np.bincount(codes, minlength=256))
+            assert np.array_equal(B[o_ftab: o_ftab + 1024].view(np.uint32), fc), (j, order)
+            assert H[10] == len(mat) and B[o_ftab + off_fmat: o_ftab + off_fmat + len(mat)].tobytes() == mat, (j, order)
+        else:
+            assert H[10] == 0, (j, order)
+
+
 # Mutants: one synchronisation taken out of a copy of the kernel sources each.  Every one must
 # make some schedule give a wrong result (or stop: a deadlock or a crash in the emulation, which
 # the subprocess reports as a failure).  The split kernel's barrier before the scatter pass has
@@ -180,6 +235,11 @@ MUTANTS = [
     ("split_no_sync_after_counting", "przm_sheaf.hip",
      "        else if (i0 < n) sheaf_hist_round<false>(L, n, i0, kind, sigw, hs);\n    }\n    __syncthreads();\n",
      "        else if (i0 < n) sheaf_hist_round<false>(L, n, i0, kind, sigw, hs);\n    }\n"),
+    ("offs_no_sync_after_scan_read", "przm_offsets.hip",
+     "    __syncthreads(); /* part is reused by the next call */\n", ""),
+    ("offs_no_sync_after_rounds", "przm_offsets.hip",
+     "        nbits += tot & 0xFFFFFu;\n    }\n    __syncthreads();\n",
+     "        nbits += tot & 0xFFFFFu;\n    }\n"),
     ("hist_no_wave_barrier", "przm_entropy.hip",
      "    for (int i = 0; i < 4; i++) cnt[w][lane * 4 + i] = 0;\n    __builtin_amdgcn_wave_barrier();\n",
      "    for (int i = 0; i < 4; i++) cnt[w][lane * 4 + i] = 0;\n"),
@@ -191,15 +251,15 @@ sys.path.insert(0, %r)
 import test_simt_kernels as T
 lib = T.load(sys.argv[1])
 cls = T.class_table()
-bad = []
 for order in T.ORDERS:
-    for name, fn in (("split", lambda: T.check_split(lib, cls, order)), ("hist", lambda: T.check_hist(lib, order))):
+    for name, fn in (("split", lambda: T.check_split(lib, cls, order)), ("hist", lambda: T.check_hist(lib, order)),
+                     ("offs", lambda: T.check_offs(lib, order))):
         try:
             fn()
         except AssertionError:
-            bad.append((name, order))
-print(bad)
-sys.exit(1 if bad else 0)
+            print("caught by", name, "under order", order)
+            sys.exit(1)
+sys.exit(0)
 """
 
 
