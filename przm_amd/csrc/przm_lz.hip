@@ -72,6 +72,10 @@ typedef __attribute__((address_space(1))) uint8_t gu8;
 #define LSZ (LIX(QP + 1) + 8)
 #define GWS_SLOT (2u * PRZM_BLOCK * 2u + 256u * 4u) /* one large bucket: positions, order, group starts */
 #define GWS_STRIDE (2u * GWS_SLOT)                   /* per wave: the queued bucket + one worked in place */
+/* global scratch of one LZ workgroup: the half blocks' bucketed positions (T0, T1: u16), the
+ * per-position results (D: u32), 2 * PRZM_HALF * 2 bytes unused, the waves' workspaces */
+constexpr uint64_t LZ_SCRATCH_STRIDE = 2ull * PRZM_HALF * 2 + 2ull * PRZM_HALF * 4 + 2ull * PRZM_HALF * 2 +
+                                       (uint64_t)LZ_NW * GWS_STRIDE;
 
 enum { T_LIT = 0, T_RLE = 1, T_MATCH = 2 };
 

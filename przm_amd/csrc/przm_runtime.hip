@@ -54,8 +54,6 @@ std::atomic<int32_t> g_mock_error{0};
         if (r_ != 0) return r_; \
     } while (0)
 
-constexpr uint64_t LZ_SCRATCH_STRIDE = 2ull * PRZM_HALF * 2 /* T0,T1 */ + 2ull * PRZM_HALF * 4 /* R0,R1 */ +
-                                       2ull * PRZM_HALF * 2 /* L0,L1 */ + (uint64_t)LZ_NW * GWS_STRIDE;
 
 struct DevBuf {
     void *p = nullptr;

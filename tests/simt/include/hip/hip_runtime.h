@@ -19,8 +19,8 @@
  *    runs ahead until a barrier stops it); a fiber runs until it blocks at a synchronisation or
  *    exits, and a schedule that can make no progress is reported as a deadlock;
  *  - atomics are plain read-modify-writes (one OS thread runs every fiber).
- * DPP controls: quad_perm (0x00-0xFF), row_shr:1..15 (0x111-0x11F), row_bcast:15 (0x142),
- * row_bcast:31 (0x143), with row masks; a lane whose source is invalid keeps `old` (bound_ctrl
+ * DPP controls: quad_perm (0x00-0xFF), row_shr:1..15 (0x111-0x11F), wave_shr:1 (0x138),
+ * row_bcast:15 (0x142), row_bcast:31 (0x143), with row masks; a lane whose source is invalid keeps `old` (bound_ctrl
  * false).  v_rcp_f32 is the correctly rounded reciprocal.
  */
 #pragma once
@@ -42,6 +42,8 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define __align__(n) __attribute__((aligned(n)))
+/* one address space on the host: __attribute__((address_space(1))) becomes an empty attribute */
+#define address_space(n)
 
 struct uint4 {
     uint32_t x, y, z, w;
@@ -231,6 +233,7 @@ static inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int ro
     int s = -1;
     if (ctrl <= 0xFF) s = (int)((l & ~3u) | ((uint32_t)(ctrl >> (2 * (l & 3u))) & 3u));
     else if (ctrl >= 0x111 && ctrl <= 0x11F) s = (l & 15u) >= (uint32_t)(ctrl & 15) ? (int)l - (ctrl & 15) : -1;
+    else if (ctrl == 0x138) s = l > 0 ? (int)l - 1 : -1; /* wave_shr:1 */
     else if (ctrl == 0x142) s = row > 0 ? (int)(16 * row - 1) : -1;
     else if (ctrl == 0x143) s = row >= 2 ? 31 : -1;
     else {
@@ -276,6 +279,22 @@ template <typename T> static inline T __shfl_up(T x, unsigned d, int width = 64)
     return (l - base) >= d ? (T)(uint32_t)v[l - d] : x;
 }
 
+template <typename T> static inline T __shfl(T x, int src, int width = 64) {
+    const uint32_t l = simt::me().tid & 63u;
+    const uint64_t *v = simt::wave_xchg((uint64_t)(uint32_t)x);
+    const uint32_t base = l & ~(uint32_t)(width - 1);
+    return (T)(uint32_t)v[base + ((uint32_t)src & (uint32_t)(width - 1))];
+}
+static inline void __builtin_amdgcn_sched_barrier(int) {}
+static inline uint64_t __builtin_amdgcn_s_memrealtime() {
+    static uint64_t t = 0;
+    return ++t;
+}
+static inline uint32_t __builtin_amdgcn_alignbit(uint32_t hi, uint32_t lo, uint32_t s) {
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> (s & 31u));
+}
+static inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
+static inline uint32_t atomicMin(uint32_t *p, uint32_t v) { const uint32_t o = *p; *p = o < v ? o : v; return o; }
 static inline uint32_t __builtin_amdgcn_sad_u8(uint32_t a, uint32_t b, uint32_t c) {
     for (int k = 0; k < 4; k++) {
         const int x = (int)((a >> (8 * k)) & 0xFF), y = (int)((b >> (8 * k)) & 0xFF);
