@@ -160,7 +160,7 @@ inline int launch(uint32_t nblk, uint32_t nthr, int order, const std::function<v
         for (uint32_t t = 0; t < nthr; t++) {
             Fiber &F = G.fibers[t];
             F.tid = t;
-            F.stack.resize(256 * 1024);
+            F.stack.resize(nthr > 512 ? 96 * 1024 : 256 * 1024);
             getcontext(&F.ctx);
             F.ctx.uc_stack.ss_sp = F.stack.data();
             F.ctx.uc_stack.ss_size = F.stack.size();

@@ -1,6 +1,7 @@
 /*
- * The sheaf split kernel (przm_amd/csrc/przm_sheaf.hip, przm_sheaf_split_kernel), the histogram
- * kernel (przm_amd/csrc/przm_entropy.hip, przm_hist_kernel) and the offsets prep kernel
+ * The sheaf split kernel (przm_amd/csrc/przm_sheaf.hip, przm_sheaf_split_kernel), the histogram,
+ * coder-order and rANS kernels (przm_amd/csrc/przm_entropy.hip: przm_hist_kernel,
+ * przm_rans_order_kernel, przm_rans_kernel) and the offsets prep kernel
  * (przm_amd/csrc/przm_offsets.hip, przm_offs_prep_kernel) run on the CPU through the fiber
  * emulation of include/hip/hip_runtime.h, for tests/test_simt_kernels.py.  Test infrastructure:
  * built with g++ into a temporary directory by that test.
@@ -36,6 +37,7 @@ uint64_t simt_const(int which) {
     case 14: return PRZM_OWS_CODES;
     case 15: return PRZM_OWS_EXTRA;
     case 16: return sizeof(przm_offs_t);
+    case 17: return PRZM_J_N;
     default: return 0;
     }
 }
@@ -76,6 +78,49 @@ int simt_hist(const uint8_t *streams, const uint32_t *off, uint32_t nblocks, uin
     });
     for (uint32_t b = 0; b < nblocks; b++)
         for (int k = 0; k < 4; k++) mat_len[4 * b + k] = meta[b].mat_len[k];
+    return rc;
+}
+
+/* the coder stages of the pipeline: przm_hist_kernel, przm_rans_order_kernel, then
+ * przm_rans_kernel (16 jobs a workgroup) over nblocks blocks whose four streams (lit, op, meta,
+ * var) are streams[off[4 b + k], off[4 b + k + 1]).  Per block and job kind (PRZM_J_N of them):
+ * klen the coded stream bytes (0xFFFFFFFF when the job did not run or overflowed), states the
+ * four final states, and the stream bytes at out + (b * PRZM_J_N + kind) * stride. */
+int simt_rans(const uint8_t *streams, const uint32_t *off, uint32_t nblocks, int order, uint8_t *out,
+              uint32_t stride, uint32_t *klen, uint32_t *states) {
+    std::vector<uint8_t> ws((size_t)nblocks * PRZM_WS_BYTES + 64, 0);
+    std::vector<przm_meta_t> meta(nblocks);
+    std::vector<przm_ftab_t> ftab((size_t)nblocks * 4);
+    std::vector<uint32_t> perm((size_t)nblocks * PRZM_J_N);
+    memset(meta.data(), 0, meta.size() * sizeof(przm_meta_t));
+    const uint32_t slot[4] = {PRZM_WS_LIT, PRZM_WS_OP, PRZM_WS_META, PRZM_WS_VAR};
+    const uint32_t sid[4] = {PRZM_S_LIT, PRZM_S_OP, PRZM_S_META, PRZM_S_VAR};
+    for (uint32_t b = 0; b < nblocks; b++)
+        for (int k = 0; k < 4; k++) {
+            const uint32_t n = off[4 * b + k + 1] - off[4 * b + k];
+            memcpy(ws.data() + (size_t)b * PRZM_WS_BYTES + slot[k], streams + off[4 * b + k], n);
+            meta[b].len[sid[k]] = n;
+        }
+    int rc = simt::launch(nblocks, 256, order, [&]() { przm_hist_kernel(ws.data(), meta.data(), ftab.data(), nblocks); });
+    if (!rc)
+        rc = simt::launch(PRZM_J_N, 1024, order, [&]() { przm_rans_order_kernel(meta.data(), nblocks, perm.data()); });
+    const uint32_t per_kind = (nblocks + RANS_JOBS - 1) / RANS_JOBS;
+    if (!rc)
+        rc = simt::launch(PRZM_J_N * per_kind, RANS_NT, order, [&]() {
+            przm_rans_kernel(ws.data(), meta.data(), ftab.data(), nblocks, perm.data());
+        });
+    for (uint32_t b = 0; b < nblocks; b++)
+        for (uint32_t k = 0; k < PRZM_J_N; k++) {
+            const uint8_t *src;
+            uint32_t n, tab;
+            uint8_t *end;
+            int m8;
+            job_layout(k, ws.data() + (size_t)b * PRZM_WS_BYTES, &meta[b], &src, &n, &end, &m8, &tab);
+            const uint32_t K = meta[b].enc_len[k];
+            klen[b * PRZM_J_N + k] = K;
+            for (int j = 0; j < 4; j++) states[(b * PRZM_J_N + k) * 4 + j] = meta[b].enc_state[k][j];
+            if (K != PRZM_ENC_OVERFLOW && K <= stride) memcpy(out + (size_t)(b * PRZM_J_N + k) * stride, end - K, K);
+        }
     return rc;
 }
 

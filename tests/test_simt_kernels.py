@@ -15,6 +15,11 @@ order.
     around the kernel's 4,096-literal rounds.
   * przm_hist_kernel (przm_amd/csrc/przm_entropy.hip): the tables and matrix lengths of the
     four streams of a block, against the oracle's.
+  * przm_hist_kernel, przm_rans_order_kernel and przm_rans_kernel in sequence (the coder stages
+    of the pipeline): every job's coded stream, length (or overflow past its stream's length)
+    and final states, against the oracle's V-ANS and Meta8 containers (literals.go:289-341,
+    vans_meta.go:104-150), on streams long enough for the coder's turns that skip the liveness
+    and cap tests and on an incompressible one that overflows.
   * przm_offs_prep_kernel (przm_amd/csrc/przm_offsets.hip): the distance codes, the extra bits
     (MSB first) and the codes' table of LEB128 offset streams over several 1,024-byte rounds,
     against the oracle's DistanceToCode (distance.go:171-196) and BitWriter (distance.go:217-247).
@@ -51,6 +56,9 @@ def load(out):
     lib.simt_hist.restype = ctypes.c_int
     lib.simt_hist.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                               ctypes.c_int]
+    lib.simt_rans.restype = ctypes.c_int
+    lib.simt_rans.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
+                              ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
     lib.simt_offs_prep.restype = ctypes.c_int
     lib.simt_offs_prep.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int]
     return lib
@@ -169,6 +177,57 @@ def check_hist(lib, order):
         assert mat_len[j] == len(mat) and t[off_fmat: off_fmat + len(mat)].tobytes() == mat, (j, order)
 
 
+@pytest.mark.parametrize("order", ORDERS)
+def test_rans_under_schedules(simt, order):
+    check_rans(simt, order)
+
+
+OVERFLOW = 0xFFFFFFFF
+# job kinds: lit V-ANS, op Meta8 / V-ANS, meta Meta8 / V-ANS, var Meta8 / V-ANS
+JOBS = [(0, False), (1, True), (1, False), (2, True), (2, False), (3, True), (3, False)]
+
+
+def check_rans(lib, order):
+    O = oracle()
+    rng = np.random.default_rng(38)
+    streams = [unique_text(0, 40000, 39), rng.integers(0, 6, 5000, dtype=np.uint8) * 7,
+               rng.integers(0, 256, 3000, dtype=np.uint8), modal_text(2, 1000, 40),
+               rng.integers(0, 256, 9000, dtype=np.uint8), rng.integers(0, 256, 23, dtype=np.uint8),
+               rng.integers(0, 30, 24, dtype=np.uint8), rng.integers(0, 4, 16, dtype=np.uint8),
+               np.zeros(1, np.uint8), np.zeros(0, np.uint8), rng.integers(0, 9, 15, dtype=np.uint8),
+               modal_text(1, 20000, 41)]
+    nb = len(streams) // 4
+    off = np.zeros(len(streams) + 1, np.uint32)
+    off[1:] = np.cumsum([x.size for x in streams])
+    data = np.concatenate(streams + [np.zeros(16, np.uint8)])
+    nj = int(lib.simt_const(17))
+    stride = 2 * max(x.size for x in streams) + 64
+    out = np.zeros(nb * nj * stride, np.uint8)
+    klen = np.zeros(nb * nj, np.uint32)
+    states = np.zeros(nb * nj * 4, np.uint32)
+    assert lib.simt_rans(ptr(data), ptr(off), nb, order, ptr(out), stride, ptr(klen), ptr(states)) == 0
+    for b in range(nb):
+        for kind, (k, m8) in enumerate(JOBS):
+            src = streams[4 * b + k]
+            j = b * nj + kind
+            if src.size < (24 if m8 else 16 if k else 1):
+                assert klen[j] == OVERFLOW, (b, kind)
+                continue
+            if m8:
+                c = O.meta8(src)
+                head = 24
+            else:
+                c = O.vans(src)
+                head = 8 + len(O.matrix(O.normalize(np.bincount(src, minlength=256)))) + 16
+            K = len(c) - head
+            if K > src.size:
+                assert klen[j] == OVERFLOW, (b, kind, order)
+                continue
+            assert klen[j] == K, (b, kind, order)
+            assert states[4 * j: 4 * j + 4].tobytes() == c[head - 16: head], (b, kind, order)
+            assert out[j * stride: j * stride + K].tobytes() == c[head:], (b, kind, order)
+
+
 def _leb128(ds):
     out = bytearray()
     for d in ds:
@@ -221,7 +280,10 @@ def check_offs(lib, order):
 # make some schedule give a wrong result (or stop: a deadlock or a crash in the emulation, which
 # the subprocess reports as a failure).  The split kernel's barrier before the scatter pass has
 # no mutant: the first round writes the staging only after its own first barrier, which every
-# wave reaches after its normalisations, so that barrier orders the reuse as well.
+# wave reaches after its normalisations, so that barrier orders the reuse as well.  Nor has the
+# coder step's wave barrier after its store branch: the store reads the completed half of the
+# 256-byte staging ring while later steps write the other half, and the step's quad exchanges
+# keep the lanes within a step of each other, so that barrier only keeps the compiler's order.
 MUTANTS = [
     ("split_no_sync_after_scan", "przm_sheaf.hip",
      "    if (lane == 63u) *(uint4 *)part[w] = make_uint4(in4[0], in4[1], in4[2], in4[3]);\n    __syncthreads();\n",
@@ -240,6 +302,9 @@ MUTANTS = [
     ("offs_no_sync_after_rounds", "przm_offsets.hip",
      "        nbits += tot & 0xFFFFFu;\n    }\n    __syncthreads();\n",
      "        nbits += tot & 0xFFFFFu;\n    }\n"),
+    ("rans_no_wave_barrier_before_store", "przm_entropy.hip",
+     "    const uint32_t K2 = K + tot;\n    __builtin_amdgcn_wave_barrier();\n",
+     "    const uint32_t K2 = K + tot;\n"),
     ("hist_no_wave_barrier", "przm_entropy.hip",
      "    for (int i = 0; i < 4; i++) cnt[w][lane * 4 + i] = 0;\n    __builtin_amdgcn_wave_barrier();\n",
      "    for (int i = 0; i < 4; i++) cnt[w][lane * 4 + i] = 0;\n"),
@@ -253,7 +318,7 @@ lib = T.load(sys.argv[1])
 cls = T.class_table()
 for order in T.ORDERS:
     for name, fn in (("split", lambda: T.check_split(lib, cls, order)), ("hist", lambda: T.check_hist(lib, order)),
-                     ("offs", lambda: T.check_offs(lib, order))):
+                     ("offs", lambda: T.check_offs(lib, order)), ("rans", lambda: T.check_rans(lib, order))):
         try:
             fn()
         except AssertionError:
