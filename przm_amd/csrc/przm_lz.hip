@@ -595,17 +595,44 @@ __device__ __forceinline__ void bucket_search_range(LzShared &S, const Scr &sc, 
                 mc = mc > sco[u] ? mc : sco[u];
             }
             if (__ballot(mc >= lim12)) {
-                /* some candidate agrees on all 12 key bytes: its exact length decides.  Once
-                 * the best reaches 12 bytes a candidate can only beat it if it agrees at byte
-                 * `bl` (one byte compare before the full one). */
+                /* Some candidate agrees on all 12 key bytes: its exact length decides.  While
+                 * the best is shorter than 12 bytes, the nearest such candidate of the step is
+                 * extended first (one lcp_ext per lane).  Then a candidate can only beat the
+                 * best if it agrees at byte bl, the best's length (one that differs there is at
+                 * most as long and farther, so it loses a tie): those bytes of the step's other
+                 * 12-byte candidates are read together, and only the ones that agree are
+                 * extended. */
+                uint32_t qf = 0, uf = B;
 #pragma unroll
-                for (uint32_t u = 0; u < B; u++) {
-                    if (sco[u] >= lim12) {
-                        const uint32_t q = sco[u] & 0xFFFFu, bl = best >> 16;
-                        if (bl < cap && (bl < 12u || in[p + bl] == in[q + bl]))
-                            sco[u] = (lcp_ext(in, p, q, cap) << 16) | q;
+                for (uint32_t u = B; u-- > 0;)
+                    if (sco[u] >= lim12) { qf = sco[u] & 0xFFFFu; uf = u; }
+                /* the step's key scores are exact below 12 bytes and lower bounds at 12, which
+                 * the extended nearest candidate, or a best of 12 bytes or more from a nearer
+                 * step, dominates */
+                if (best < lim12 && uf < B) {
+                    const uint32_t sf = (lcp_ext(in, p, qf, cap) << 16) | qf;
+                    best = best > sf ? best : sf;
+                } else {
+                    uf = B; /* no candidate extended in this step */
+                }
+                best = best > mc ? best : mc;
+                const uint32_t bl = best >> 16;
+                if (bl < cap) {
+                    const uint32_t pb = in[p + bl];
+                    uint32_t pass = 0;
+#pragma unroll
+                    for (uint32_t u = 0; u < B; u++) {
+                        const uint32_t q = sco[u] & 0xFFFFu;
+                        const bool cand = sco[u] >= lim12 && u != uf;
+                        pass |= (cand && in[q + bl] == pb) ? 1u << u : 0u;
                     }
-                    best = best > sco[u] ? best : sco[u];
+#pragma unroll
+                    for (uint32_t u = 0; u < B; u++) {
+                        if ((pass >> u) & 1u) {
+                            const uint32_t q = sco[u] & 0xFFFFu, su = (lcp_ext(in, p, q, cap) << 16) | q;
+                            best = best > su ? best : su;
+                        }
+                    }
                 }
             } else {
                 best = best > mc ? best : mc;
