@@ -516,10 +516,12 @@ przm_sheaf_count_kernel(uint8_t *__restrict__ sws, uint32_t nblocks, const uint3
     }
     __syncthreads();
     if (!has) return;
-    const uint32_t K = coded ? vans_count_quad(sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP + fs, n, vt[q],
-                                               n - head, gmax, nmin)
-                             : PRZM_ENC_OVERFLOW;
-    if (j == 0) H->k[m][k] = K;
+    /* every quad with a job steps, in uniform control flow: one whose fiber is not coded counts
+     * an empty fiber with limit 0 (past it at once; in a turn without liveness tests it steps on
+     * zero chunks with whatever its table slot holds, which touches no memory but that slot) */
+    const uint32_t K = vans_count_quad(sb + PRZM_SWS_FIB + (uint64_t)m * PRZM_SWS_FIB_CAP + fs, coded ? n : 0u,
+                                       vt[q], coded ? n - head : 0u, gmax, nmin);
+    if (j == 0) H->k[m][k] = coded ? K : PRZM_ENC_OVERFLOW;
 }
 
 /* the section's length for choice c (2 + mode for a sheaf container) */

@@ -1,7 +1,8 @@
 /*
  * The sheaf split kernel (przm_amd/csrc/przm_sheaf.hip, przm_sheaf_split_kernel), the histogram,
  * coder-order and rANS kernels (przm_amd/csrc/przm_entropy.hip: przm_hist_kernel,
- * przm_rans_order_kernel, przm_rans_kernel) and the offsets prep kernel
+ * przm_rans_order_kernel, przm_rans_kernel), the sheaf option's kernels after the coder
+ * (przm_sheaf_split, _class, _place, _count and _encode kernels) and the offsets prep kernel
  * (przm_amd/csrc/przm_offsets.hip, przm_offs_prep_kernel) run on the CPU through the fiber
  * emulation of include/hip/hip_runtime.h, for tests/test_simt_kernels.py.  Test infrastructure:
  * built with g++ into a temporary directory by that test.
@@ -38,6 +39,9 @@ uint64_t simt_const(int which) {
     case 15: return PRZM_OWS_EXTRA;
     case 16: return sizeof(przm_offs_t);
     case 17: return PRZM_J_N;
+    case 18: return offsetof(przm_sheaf_t, k);
+    case 19: return offsetof(przm_sheaf_t, choice);
+    case 20: return offsetof(przm_sheaf_t, len);
     default: return 0;
     }
 }
@@ -121,6 +125,42 @@ int simt_rans(const uint8_t *streams, const uint32_t *off, uint32_t nblocks, int
             for (int j = 0; j < 4; j++) states[(b * PRZM_J_N + k) * 4 + j] = meta[b].enc_state[k][j];
             if (K != PRZM_ENC_OVERFLOW && K <= stride) memcpy(out + (size_t)(b * PRZM_J_N + k) * stride, end - K, K);
         }
+    return rc;
+}
+
+/* the coder stages and then the sheaf option's stages, in the order the runtime launches them,
+ * over nblocks blocks of four streams as in simt_rans; sws: nblocks * PRZM_SWS_BYTES bytes out,
+ * meta: nblocks przm_meta_t out */
+int simt_sheaf_pipeline(const uint8_t *streams, const uint32_t *off, uint32_t nblocks, int order, uint8_t *sws,
+                        uint8_t *meta_out) {
+    std::vector<uint8_t> ws((size_t)nblocks * PRZM_WS_BYTES + 64, 0);
+    std::vector<przm_meta_t> meta(nblocks);
+    std::vector<przm_ftab_t> ftab((size_t)nblocks * 4);
+    std::vector<uint32_t> perm((size_t)nblocks * PRZM_J_N), sperm((size_t)nblocks * 16), bins(PRZM_SHEAF_BINS, 0);
+    memset(meta.data(), 0, meta.size() * sizeof(przm_meta_t));
+    const uint32_t slot[4] = {PRZM_WS_LIT, PRZM_WS_OP, PRZM_WS_META, PRZM_WS_VAR};
+    const uint32_t sid[4] = {PRZM_S_LIT, PRZM_S_OP, PRZM_S_META, PRZM_S_VAR};
+    for (uint32_t b = 0; b < nblocks; b++)
+        for (int k = 0; k < 4; k++) {
+            const uint32_t n = off[4 * b + k + 1] - off[4 * b + k];
+            memcpy(ws.data() + (size_t)b * PRZM_WS_BYTES + slot[k], streams + off[4 * b + k], n);
+            meta[b].len[sid[k]] = n;
+        }
+    const uint32_t per_kind = (nblocks + RANS_JOBS - 1) / RANS_JOBS, g16 = (nblocks + 15) / 16;
+    int rc = simt::launch(nblocks, 256, order, [&]() { przm_hist_kernel(ws.data(), meta.data(), ftab.data(), nblocks); });
+    if (!rc) rc = simt::launch(PRZM_J_N, 1024, order, [&]() { przm_rans_order_kernel(meta.data(), nblocks, perm.data()); });
+    if (!rc)
+        rc = simt::launch(PRZM_J_N * per_kind, RANS_NT, order, [&]() {
+            przm_rans_kernel(ws.data(), meta.data(), ftab.data(), nblocks, perm.data());
+        });
+    if (!rc)
+        rc = simt::launch(nblocks, SHEAF_NT, order, [&]() { przm_sheaf_split_kernel(ws.data(), meta.data(), sws, nblocks); });
+    if (!rc) rc = simt::launch(g16, 256, order, [&]() { przm_sheaf_class_kernel(sws, nblocks, bins.data()); });
+    if (!rc) rc = simt::launch(g16, 256, order, [&]() { przm_sheaf_place_kernel(sws, nblocks, bins.data(), sperm.data()); });
+    if (!rc) rc = simt::launch(nblocks, 64, order, [&]() { przm_sheaf_count_kernel(sws, nblocks, sperm.data()); });
+    if (!rc)
+        rc = simt::launch((nblocks + 3) / 4, 64, order, [&]() { przm_sheaf_encode_kernel(meta.data(), sws, nblocks); });
+    memcpy(meta_out, meta.data(), nblocks * sizeof(przm_meta_t));
     return rc;
 }
 

@@ -20,6 +20,10 @@ order.
     and final states, against the oracle's V-ANS and Meta8 containers (literals.go:289-341,
     vans_meta.go:104-150), on streams long enough for the coder's turns that skip the liveness
     and cap tests and on an incompressible one that overflows.
+  * the sheaf option after the coder (the split, class, place, count and encode kernels of
+    przm_amd/csrc/przm_sheaf.hip, in launch order): every fiber's exact coded length or
+    overflow (the counting coder, literals.go:289-341 and 622) and the section's choice and
+    length (PackageLiteralsSection, literals.go:667-693), against the oracle's.
   * przm_offs_prep_kernel (przm_amd/csrc/przm_offsets.hip): the distance codes, the extra bits
     (MSB first) and the codes' table of LEB128 offset streams over several 1,024-byte rounds,
     against the oracle's DistanceToCode (distance.go:171-196) and BitWriter (distance.go:217-247).
@@ -59,6 +63,9 @@ def load(out):
     lib.simt_rans.restype = ctypes.c_int
     lib.simt_rans.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
                               ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+    lib.simt_sheaf_pipeline.restype = ctypes.c_int
+    lib.simt_sheaf_pipeline.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p]
     lib.simt_offs_prep.restype = ctypes.c_int
     lib.simt_offs_prep.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int]
     return lib
@@ -228,6 +235,53 @@ def check_rans(lib, order):
             assert out[j * stride: j * stride + K].tobytes() == c[head:], (b, kind, order)
 
 
+@pytest.mark.parametrize("order", ORDERS)
+def test_sheaf_pipeline_under_schedules(simt, classes, order):
+    check_sheaf_pipeline(simt, classes, order)
+
+
+def check_sheaf_pipeline(lib, classes, order):
+    O = oracle()
+    rng = np.random.default_rng(42)
+    lits = [unique_text(0, 12000, 43), unique_text(1, 6000, 44), unique_text(2, 3000, 45),
+            unique_text(3, 9000, 46), rng.integers(0, 256, 5000, dtype=np.uint8), modal_text(2, 40, 47),
+            np.zeros(0, np.uint8), rng.integers(0, 3, 700, dtype=np.uint8)]
+    streams = []
+    for i, l in enumerate(lits):
+        streams += [l, rng.integers(0, 50, 30 + 7 * i, dtype=np.uint8), rng.integers(0, 20, 25, dtype=np.uint8),
+                    rng.integers(0, 256, 17, dtype=np.uint8)]
+    nb = len(lits)
+    off = np.zeros(len(streams) + 1, np.uint32)
+    off[1:] = np.cumsum([x.size for x in streams])
+    data = np.concatenate(streams + [np.zeros(16, np.uint8)])
+    sws_bytes, hdr = int(lib.simt_const(0)), int(lib.simt_const(1))
+    off_l, off_mat, off_k, off_choice, off_len = (int(lib.simt_const(i)) for i in (7, 8, 18, 19, 20))
+    sws = np.zeros(nb * sws_bytes, np.uint8)
+    meta = np.zeros(nb * int(lib.simt_const(10)), np.uint8)
+    assert lib.simt_sheaf_pipeline(ptr(data), ptr(off), nb, order, ptr(sws), ptr(meta)) == 0
+    for b, lit in enumerate(lits):
+        H = sws[b * sws_bytes + hdr: b * sws_bytes + hdr + 512]
+        u32 = lambda o: int(H[o: o + 4].view(np.uint32)[0])
+        p2 = np.concatenate([[0, 0], lit[:-2]]).astype(np.int64)[: lit.size]
+        p1 = np.concatenate([[0], lit[:-1]]).astype(np.int64)[: lit.size]
+        for m in range(4):
+            cl = classes[m][p2 * 256 + p1]
+            for k in range(4):
+                f = lit[cl == k]
+                r = 4 * m + k
+                assert u32(off_l + 4 * r) == f.size, (b, m, k)
+                want = OVERFLOW
+                if f.size >= MIN_CODED:
+                    head = len(O.matrix(O.normalize(np.bincount(f, minlength=256)))) + 16
+                    assert u32(off_mat + 4 * r) + 16 == head, (b, m, k)
+                    K = len(O.vans(f)) - 8 - head
+                    if f.size > head and K < f.size - head:
+                        want = K
+                assert u32(off_k + 4 * r) == want, (b, m, k, order)
+        sec, kind = O.package_literals(lit)
+        assert (u32(off_choice), u32(off_len)) == (kind, len(sec)), (b, order)
+
+
 def _leb128(ds):
     out = bytearray()
     for d in ds:
@@ -284,6 +338,8 @@ def check_offs(lib, order):
 # coder step's wave barrier after its store branch: the store reads the completed half of the
 # 256-byte staging ring while later steps write the other half, and the step's quad exchanges
 # keep the lanes within a step of each other, so that barrier only keeps the compiler's order.
+# The sheaf counting kernel's barrier after its table loads has none either: its workgroup is one
+# wave, whose shuffles for the longest and shortest fiber order the loads before the counting.
 MUTANTS = [
     ("split_no_sync_after_scan", "przm_sheaf.hip",
      "    if (lane == 63u) *(uint4 *)part[w] = make_uint4(in4[0], in4[1], in4[2], in4[3]);\n    __syncthreads();\n",
@@ -318,7 +374,8 @@ lib = T.load(sys.argv[1])
 cls = T.class_table()
 for order in T.ORDERS:
     for name, fn in (("split", lambda: T.check_split(lib, cls, order)), ("hist", lambda: T.check_hist(lib, order)),
-                     ("offs", lambda: T.check_offs(lib, order)), ("rans", lambda: T.check_rans(lib, order))):
+                     ("offs", lambda: T.check_offs(lib, order)), ("rans", lambda: T.check_rans(lib, order)),
+                     ("sheaf", lambda: T.check_sheaf_pipeline(lib, cls, order))):
         try:
             fn()
         except AssertionError:
