@@ -42,6 +42,8 @@ uint64_t simt_const(int which) {
     case 18: return offsetof(przm_sheaf_t, k);
     case 19: return offsetof(przm_sheaf_t, choice);
     case 20: return offsetof(przm_sheaf_t, len);
+    case 21: return PRZM_OWS_M12 + PRZM_OWS_M12_CAP;
+    case 22: return PRZM_OWS_VANS + PRZM_OWS_VANS_CAP;
     default: return 0;
     }
 }
@@ -164,9 +166,10 @@ int simt_sheaf_pipeline(const uint8_t *streams, const uint32_t *off, uint32_t nb
     return rc;
 }
 
-/* przm_offs_prep_kernel over nblocks blocks whose LEB128 offset streams are
- * offs[off[b], off[b + 1]); ows: nblocks * PRZM_OWS_BYTES bytes, filled by the caller */
-int simt_offs_prep(const uint8_t *offs, const uint32_t *off, uint32_t nblocks, uint8_t *ows, int order) {
+/* przm_offs_prep_kernel, then (code != 0) przm_offs_code_kernel, over nblocks blocks whose LEB128
+ * offset streams are offs[off[b], off[b + 1]); ows: nblocks * PRZM_OWS_BYTES bytes, filled by the
+ * caller */
+int simt_offs_prep(const uint8_t *offs, const uint32_t *off, uint32_t nblocks, uint8_t *ows, int order, int code) {
     std::vector<uint8_t> ws((size_t)nblocks * PRZM_WS_BYTES + 64, 0);
     std::vector<przm_meta_t> meta(nblocks);
     memset(meta.data(), 0, meta.size() * sizeof(przm_meta_t));
@@ -175,8 +178,13 @@ int simt_offs_prep(const uint8_t *offs, const uint32_t *off, uint32_t nblocks, u
         memcpy(ws.data() + (size_t)b * PRZM_WS_BYTES + PRZM_WS_OFF, offs + off[b], n);
         meta[b].len[PRZM_S_OFF] = n;
     }
-    return simt::launch(nblocks, OFFS_NT, order,
-                        [&]() { przm_offs_prep_kernel(ws.data(), meta.data(), ows, nblocks); });
+    int rc = simt::launch(nblocks, OFFS_NT, order,
+                          [&]() { przm_offs_prep_kernel(ws.data(), meta.data(), ows, nblocks); });
+    const uint32_t m12_groups = (nblocks + 63) / 64;
+    if (!rc && code)
+        rc = simt::launch(m12_groups + (nblocks + 15) / 16, 64, order,
+                          [&]() { przm_offs_code_kernel(ows, nblocks, m12_groups); });
+    return rc;
 }
 
 } // extern "C"

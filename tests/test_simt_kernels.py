@@ -24,9 +24,11 @@ order.
     przm_amd/csrc/przm_sheaf.hip, in launch order): every fiber's exact coded length or
     overflow (the counting coder, literals.go:289-341 and 622) and the section's choice and
     length (PackageLiteralsSection, literals.go:667-693), against the oracle's.
-  * przm_offs_prep_kernel (przm_amd/csrc/przm_offsets.hip): the distance codes, the extra bits
-    (MSB first) and the codes' table of LEB128 offset streams over several 1,024-byte rounds,
-    against the oracle's DistanceToCode (distance.go:171-196) and BitWriter (distance.go:217-247).
+  * przm_offs_prep_kernel and przm_offs_code_kernel (przm_amd/csrc/przm_offsets.hip): the
+    distance codes, the extra bits (MSB first) and the codes' table of LEB128 offset streams over
+    several 1,024-byte rounds, and the codes' Mode-12 and V-ANS streams, lengths and states,
+    against the oracle's DistanceToCode (distance.go:171-196), BitWriter (distance.go:217-247),
+    Mode-12 coder (distance.go:323-376) and V-ANS coder.
 
 The GPU parity tests check the same kernels bit for bit on the device; this checks their
 synchronisation on the CPU, where a schedule can be chosen.
@@ -67,7 +69,8 @@ def load(out):
     lib.simt_sheaf_pipeline.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
                                         ctypes.c_void_p, ctypes.c_void_p]
     lib.simt_offs_prep.restype = ctypes.c_int
-    lib.simt_offs_prep.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int]
+    lib.simt_offs_prep.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int,
+                                   ctypes.c_int]
     return lib
 
 
@@ -310,7 +313,8 @@ def check_offs(lib, order):
     ows_bytes, o_ftab, o_codes, o_extra = (int(lib.simt_const(i)) for i in (12, 13, 14, 15))
     ftsz, off_fmat = int(lib.simt_const(5)), int(lib.simt_const(9))
     ows = np.full(len(streams) * ows_bytes, 0xA5, np.uint8)
-    assert lib.simt_offs_prep(ptr(data), ptr(off), len(streams), ptr(ows), order) == 0
+    assert lib.simt_offs_prep(ptr(data), ptr(off), len(streams), ptr(ows), order, 1) == 0
+    m12_end, v_end = int(lib.simt_const(21)), int(lib.simt_const(22))
     for j, (d, st) in enumerate(zip(dists, streams)):
         B = ows[j * ows_bytes: (j + 1) * ows_bytes]
         H = B[:64].view(np.uint32)  # n, orig, extra, m12_len, m12_state, v_len, v_state[4], mat_len
@@ -322,12 +326,22 @@ def check_offs(lib, order):
         assert (H[0], H[1], H[2]) == (d.size, st.size, len(extra)), (j, order)
         assert B[o_codes: o_codes + d.size].tobytes() == codes.tobytes(), (j, order)
         assert B[o_extra: o_extra + len(extra)].tobytes() == extra, (j, order)
+        if d.size:  # the Mode-12 stream: [0x0C][u32 state][bytes]
+            m12 = O.dist_m12(codes)
+            K = len(m12) - 5
+            assert (H[3], H[4]) == (K, int.from_bytes(m12[1:5], "little")), (j, order, "m12")
+            assert B[m12_end - K: m12_end].tobytes() == m12[5:], (j, order, "m12 bytes")
         if d.size >= 64:
             fc, mat = _table(np.bincount(codes, minlength=256))
             assert np.array_equal(B[o_ftab: o_ftab + 1024].view(np.uint32), fc), (j, order)
             assert H[10] == len(mat) and B[o_ftab + off_fmat: o_ftab + off_fmat + len(mat)].tobytes() == mat, (j, order)
+            v = O.vans(codes)
+            head = 8 + len(mat) + 16
+            K = len(v) - head
+            assert H[5] == K and H[6:10].tobytes() == v[head - 16: head], (j, order, "vans")
+            assert B[v_end - K: v_end].tobytes() == v[head:], (j, order, "vans bytes")
         else:
-            assert H[10] == 0, (j, order)
+            assert H[10] == 0 and H[5] == OVERFLOW, (j, order)
 
 
 # Mutants: one synchronisation taken out of a copy of the kernel sources each.  Every one must
