@@ -11,8 +11,8 @@
  *                            builds the 16 fiber lengths and histograms, the
  *                            waves normalise the fibers' tables
  *                            (NormalizeFrequencies and the matrix,
- *                            literals.go:78-184), and a scatter pass per mode
- *                            writes the fibers (DecomposeFibers,
+ *                            literals.go:78-184), and one scatter pass writes
+ *                            the fibers of all four modes (DecomposeFibers,
  *                            sheaf.go:125-152), each 16-byte aligned.
  *  przm_sheaf_class_kernel,
  *  przm_sheaf_place_kernel   the counting jobs by fiber length.
