@@ -6,6 +6,10 @@ every block equal the oracle's (orc_tokenize_streams: przm_tokenizer.cuh:115-466
 matches of compress.go:681-707), and a copy of the kernel with one synchronisation taken out
 gives a wrong result under some schedule.  The GPU parity tests check the same streams on the
 device.
+
+The whole batched encode of a buffer also runs emulated, kernel by kernel in the runtime's launch
+order (tests/simt/pipeline_emul.cpp: check, LZ, histograms, coder order, rANS, frame sizes, scan,
+frame writer), and its frames equal the oracle's (orc_encode_block per 64 KiB block).
 """
 import ctypes
 import os
@@ -37,6 +41,22 @@ def load(out):
     lib.simt_lz.restype = ctypes.c_int
     lib.simt_lz.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    return lib
+
+
+PIPE = os.path.join(HERE, "simt", "pipeline_emul.cpp")
+
+
+@pytest.fixture(scope="module")
+def pipe(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("simt_pipe") / "libpipe.so")
+    build(out, PIPE)
+    lib = ctypes.CDLL(out)
+    lib.simt_frames_bound.restype = ctypes.c_uint64
+    lib.simt_frames_bound.argtypes = [ctypes.c_uint64]
+    lib.simt_encode.restype = ctypes.c_int
+    lib.simt_encode.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_uint64, ctypes.c_void_p]
     return lib
 
 
@@ -80,6 +100,18 @@ def check_lz(lib, a, order, B=8, grid=1, depth=16):
 @pytest.mark.parametrize("order", ORDERS)
 def test_lz_under_schedules(lz, order):
     check_lz(lz, inputs(), order)
+
+
+@pytest.mark.parametrize("order", (0, 3))
+def test_encode_pipeline_under_schedules(pipe, order):
+    a = inputs()
+    cap = int(pipe.simt_frames_bound(a.size))
+    out = np.zeros(cap, np.uint8)
+    fl = np.zeros((a.size + 65535) // 65536, np.uint32)
+    assert pipe.simt_encode(ptr(a), a.size, 16, order, ptr(out), cap, ptr(fl)) == 0
+    of, osz = oracle().encode_buffer(a, 6, False, False)
+    assert np.array_equal(fl, osz), order
+    assert out[: int(fl.sum())].tobytes() == of, order
 
 
 def test_lz_b2_two_workgroups(lz):
