@@ -2,13 +2,17 @@
  * The whole batched encode of one buffer on the CPU through the fiber emulation of
  * include/hip/hip_runtime.h, kernel by kernel in the runtime's launch order (przm_runtime.hip):
  * the high-entropy check, the LZ kernel, the histograms, the coder order, the rANS coder, the
- * frame sizes, their scan and the frame writer.  For tests/test_simt_lz.py; test
- * infrastructure, built with g++ into a temporary directory by that test.
+ * section options' kernels when asked for (Section 3 prefix: offsets prep and code; Section 4
+ * sheaf: split, class, place, count, encode), the frame sizes, their scan and the frame writer.
+ * For tests/test_simt_lz.py; test infrastructure, built with g++ into a temporary directory by
+ * that test.
  */
 #include <hip/hip_runtime.h>
 
 #include "../../przm_amd/csrc/przm_entropy.hip"
 #include "lz_emul.cpp"
+#include "../../przm_amd/csrc/przm_offsets.hip"
+#include "../../przm_amd/csrc/przm_sheaf.hip"
 
 extern "C" {
 
@@ -18,13 +22,15 @@ uint64_t simt_frames_bound(uint64_t n) {
 }
 
 /* frames of src[0, n) (64 KiB blocks) at chain depth `depth` into out (out_cap bytes), their
- * sizes into frame_len (one per block); returns the launch status, or -2 when out_cap is short */
+ * sizes into frame_len (one per block), with the Section-3 prefix option (prefix) and the
+ * Section-4 sheaf option (sheaf); returns the launch status, or -2 when out_cap is short */
 int simt_encode(const uint8_t *src, uint64_t n, uint32_t depth, int order, uint8_t *out, uint64_t out_cap,
-                uint32_t *frame_len) {
+                uint32_t *frame_len, int prefix, int sheaf) {
     const uint32_t nb = (uint32_t)((n + PRZM_BLOCK - 1) / PRZM_BLOCK);
     Guarded in(n + 64), blocks(nb * sizeof(przm_block_t)), ws((size_t)nb * PRZM_WS_BYTES), meta(nb * sizeof(przm_meta_t)),
         ftab((size_t)nb * 4 * sizeof(przm_ftab_t)), perm((size_t)nb * PRZM_J_N * 4), flen(nb * 4), foff(nb * 8),
-        run(64 + 16), scratch(LZ_SCRATCH_STRIDE), dout(out_cap);
+        run(64 + 16), scratch(LZ_SCRATCH_STRIDE), dout(out_cap), ows(prefix ? (size_t)nb * PRZM_OWS_BYTES : 16),
+        sws(sheaf ? (size_t)nb * PRZM_SWS_BYTES : 16), sperm((size_t)nb * 16 * 4), bins(PRZM_SHEAF_BINS * 4);
     memcpy(in.p, src, n);
     memset(in.p + n, 0, 64);
     przm_block_t *bt = (przm_block_t *)blocks.p;
@@ -50,9 +56,23 @@ int simt_encode(const uint8_t *src, uint64_t n, uint32_t depth, int order, uint8
         rc = simt::launch(PRZM_J_N * per_kind, RANS_NT, order, [&]() {
             przm_rans_kernel(ws.p, mt, (const przm_ftab_t *)ftab.p, nb, (const uint32_t *)perm.p);
         });
+    uint8_t *o = prefix ? ows.p : nullptr, *sh = sheaf ? sws.p : nullptr;
+    const uint32_t m12_groups = (nb + 63) / 64, g16 = (nb + 15) / 16;
+    if (!rc && prefix) rc = simt::launch(nb, OFFS_NT, order, [&]() { przm_offs_prep_kernel(ws.p, mt, o, nb); });
+    if (!rc && prefix)
+        rc = simt::launch(m12_groups + g16, 64, order, [&]() { przm_offs_code_kernel(o, nb, m12_groups); });
+    if (!rc && sheaf) rc = simt::launch(nb, SHEAF_NT, order, [&]() { przm_sheaf_split_kernel(ws.p, mt, sh, nb); });
+    if (sheaf) memset(bins.p, 0, PRZM_SHEAF_BINS * 4);
+    if (!rc && sheaf) rc = simt::launch(g16, 256, order, [&]() { przm_sheaf_class_kernel(sh, nb, (uint32_t *)bins.p); });
+    if (!rc && sheaf)
+        rc = simt::launch(g16, 256, order,
+                          [&]() { przm_sheaf_place_kernel(sh, nb, (uint32_t *)bins.p, (uint32_t *)sperm.p); });
+    if (!rc && sheaf)
+        rc = simt::launch(nb, 64, order, [&]() { przm_sheaf_count_kernel(sh, nb, (const uint32_t *)sperm.p); });
+    if (!rc && sheaf) rc = simt::launch((nb + 3) / 4, 64, order, [&]() { przm_sheaf_encode_kernel(mt, sh, nb); });
     if (!rc)
         rc = simt::launch((nb + 255) / 256, 256, order, [&]() {
-            przm_frame_size_kernel(bt, mt, ws.p, nb, (uint32_t *)flen.p, nullptr, nullptr);
+            przm_frame_size_kernel(bt, mt, ws.p, nb, (uint32_t *)flen.p, o, sh);
         });
     if (!rc)
         rc = simt::launch(1, 1024, order, [&]() {
@@ -63,7 +83,7 @@ int simt_encode(const uint8_t *src, uint64_t n, uint32_t depth, int order, uint8
     if (!rc)
         rc = simt::launch(nb, 256, order, [&]() {
             przm_frame_write_kernel(in.p, bt, mt, ws.p, (const przm_ftab_t *)ftab.p, (const uint64_t *)foff.p, dout.p,
-                                    nb, nullptr, nullptr);
+                                    nb, o, sh);
         });
     if (!rc) {
         memcpy(out, dout.p, total);

@@ -56,7 +56,7 @@ def pipe(tmp_path_factory):
     lib.simt_frames_bound.argtypes = [ctypes.c_uint64]
     lib.simt_encode.restype = ctypes.c_int
     lib.simt_encode.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
-                                ctypes.c_uint64, ctypes.c_void_p]
+                                ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     return lib
 
 
@@ -102,16 +102,17 @@ def test_lz_under_schedules(lz, order):
     check_lz(lz, inputs(), order)
 
 
-@pytest.mark.parametrize("order", (0, 3))
-def test_encode_pipeline_under_schedules(pipe, order):
+@pytest.mark.parametrize("order,prefix,sheaf", [(0, False, False), (3, False, False), (2, True, True),
+                                                (1, True, False), (0, False, True)])
+def test_encode_pipeline_under_schedules(pipe, order, prefix, sheaf):
     a = inputs()
     cap = int(pipe.simt_frames_bound(a.size))
     out = np.zeros(cap, np.uint8)
     fl = np.zeros((a.size + 65535) // 65536, np.uint32)
-    assert pipe.simt_encode(ptr(a), a.size, 16, order, ptr(out), cap, ptr(fl)) == 0
-    of, osz = oracle().encode_buffer(a, 6, False, False)
-    assert np.array_equal(fl, osz), order
-    assert out[: int(fl.sum())].tobytes() == of, order
+    assert pipe.simt_encode(ptr(a), a.size, 16, order, ptr(out), cap, ptr(fl), int(prefix), int(sheaf)) == 0
+    of, osz = oracle().encode_buffer(a, 6, prefix, sheaf)
+    assert np.array_equal(fl, osz), (order, prefix, sheaf)
+    assert out[: int(fl.sum())].tobytes() == of, (order, prefix, sheaf)
 
 
 def test_lz_b2_two_workgroups(lz):
