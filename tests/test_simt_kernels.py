@@ -386,7 +386,7 @@ sys.path.insert(0, %r)
 import test_simt_kernels as T
 lib = T.load(sys.argv[1])
 cls = T.class_table()
-for order in T.ORDERS:
+for order in (2, 3, 0, 1, 7):  # the greedy schedules first: they catch the most
     for name, fn in (("split", lambda: T.check_split(lib, cls, order)), ("hist", lambda: T.check_hist(lib, order)),
                      ("offs", lambda: T.check_offs(lib, order)), ("rans", lambda: T.check_rans(lib, order)),
                      ("sheaf", lambda: T.check_sheaf_pipeline(lib, cls, order))):

@@ -81,6 +81,13 @@ def inputs():
                            D.markov(40000, 63)])
 
 
+def pipeline_input():
+    """four blocks of inputs(): mixture, JSON-like text (where the section options pay), long
+    runs, and the partial Markov block"""
+    a = inputs()
+    return np.concatenate([a[: 1 << 16], a[5 << 16: 6 << 16], a[3 << 16: 4 << 16], a[7 << 16:]])
+
+
 def check_lz(lib, a, order, B=8, grid=1, depth=16):
     C = [int(lib.simt_lz_const(i)) for i in range(9)]
     O = oracle()
@@ -97,7 +104,7 @@ def check_lz(lib, a, order, B=8, grid=1, depth=16):
             assert np.array_equal(W[C[1 + k]: C[1 + k] + lens[k]], want[name]), (b, name, order, B)
 
 
-@pytest.mark.parametrize("order", ORDERS)
+@pytest.mark.parametrize("order", (1, 2, 7))  # the pipeline tests below run orders 0 and 3
 def test_lz_under_schedules(lz, order):
     check_lz(lz, inputs(), order)
 
@@ -105,7 +112,7 @@ def test_lz_under_schedules(lz, order):
 @pytest.mark.parametrize("order,prefix,sheaf", [(0, False, False), (3, False, False), (2, True, True),
                                                 (1, True, False), (0, False, True)])
 def test_encode_pipeline_under_schedules(pipe, order, prefix, sheaf):
-    a = inputs()
+    a = pipeline_input()
     cap = int(pipe.simt_frames_bound(a.size))
     out = np.zeros(cap, np.uint8)
     fl = np.zeros((a.size + 65535) // 65536, np.uint32)
