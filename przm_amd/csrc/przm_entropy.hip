@@ -619,11 +619,13 @@ przm_rans1_kernel(const uint8_t *__restrict__ src, uint32_t n, const przm_ftab_t
     __shared__ __align__(16) uint8_t stg[RANS_STAGE + 16];
     for (uint32_t s = threadIdx.x; s < 256; s += 64) vt[s] = ft->fc[s];
     __syncthreads();
-    if (threadIdx.x >= 4) return;
+    /* the whole wave steps (rans_quad's wave-wide exchanges read every lane): quads 1..15 have
+     * an empty job, which emits nothing and keeps the coder on its tested turns */
+    const bool job = threadIdx.x < 4;
     uint32_t x;
-    const uint32_t K = rans_quad<false>(src, n, vt, stg, slot_end, cap, (n + 3) >> 2, &x);
+    const uint32_t K = rans_quad<false>(src, job ? n : 0u, vt, stg, slot_end, cap, (n + 3) >> 2, &x);
     if (threadIdx.x == 0) out[0] = K;
-    out[1 + threadIdx.x] = x;
+    if (job) out[1 + threadIdx.x] = x;
 }
 
 /* ---------------- frame assembly ---------------- */

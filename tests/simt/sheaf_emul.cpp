@@ -166,6 +166,35 @@ int simt_sheaf_pipeline(const uint8_t *streams, const uint32_t *off, uint32_t nb
     return rc;
 }
 
+/* the V-ANS stage entry point's kernels (przm_stage_api.hip: przm_gpu_vans_encode): the
+ * grid histogram (grid workgroups), the normalised table, the single-job coder and the
+ * container; out gets the container (cap bytes), counts the histogram; returns the container
+ * length, 0 when the stream overflows its slot, or -1 on a deadlock */
+int64_t simt_vans_stage(const uint8_t *src, uint32_t n, uint32_t grid, int order, uint8_t *out, uint64_t cap,
+                        uint32_t *counts) {
+    std::vector<uint32_t> cnt(256, 0), res(8, 0);
+    std::vector<przm_ftab_t> ft(1);
+    uint32_t mat_len = 0;
+    const uint64_t slot = (2ull * n + 64 + 15) & ~15ull;
+    std::vector<uint8_t> tmp(slot + 64, 0);
+    uint8_t *slot_end = (uint8_t *)(((uintptr_t)tmp.data() + slot + 15) & ~(uintptr_t)15) - 16;
+    int rc = n ? simt::launch(grid, 256, order, [&]() { przm_hist_grid_kernel(src, n, cnt.data()); }) : 0;
+    if (!rc) rc = simt::launch(1, 64, order, [&]() { przm_normalize_kernel(cnt.data(), ft.data(), &mat_len); });
+    if (!rc)
+        rc = simt::launch(1, 64, order, [&]() {
+            przm_rans1_kernel(src, n, ft.data(), slot_end, 2u * n + 16u, res.data());
+        });
+    if (rc) return -1;
+    memcpy(counts, cnt.data(), 1024);
+    if (res[0] == PRZM_ENC_OVERFLOW) return 0;
+    const uint64_t total = 8ull + mat_len + 16 + res[0];
+    if (total > cap) return 0;
+    rc = simt::launch(1, 256, order, [&]() {
+        przm_vans_assemble_kernel(slot_end, res.data(), ft.data(), mat_len, n, out);
+    });
+    return rc ? -1 : (int64_t)total;
+}
+
 /* przm_offs_prep_kernel, then (code != 0) przm_offs_code_kernel, over nblocks blocks whose LEB128
  * offset streams are offs[off[b], off[b + 1]); ows: nblocks * PRZM_OWS_BYTES bytes, filled by the
  * caller */

@@ -24,6 +24,9 @@ order.
     przm_amd/csrc/przm_sheaf.hip, in launch order): every fiber's exact coded length or
     overflow (the counting coder, literals.go:289-341 and 622) and the section's choice and
     length (PackageLiteralsSection, literals.go:667-693), against the oracle's.
+  * the V-ANS stage entry point's kernels (przm_hist_grid_kernel, przm_normalize_kernel,
+    przm_rans1_kernel, przm_vans_assemble_kernel): the histogram and the container of a whole
+    buffer, against numpy's bincount and the oracle's adaptive V-ANS container.
   * przm_offs_prep_kernel and przm_offs_code_kernel (przm_amd/csrc/przm_offsets.hip): the
     distance codes, the extra bits (MSB first) and the codes' table of LEB128 offset streams over
     several 1,024-byte rounds, and the codes' Mode-12 and V-ANS streams, lengths and states,
@@ -68,6 +71,9 @@ def load(out):
     lib.simt_sheaf_pipeline.restype = ctypes.c_int
     lib.simt_sheaf_pipeline.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
                                         ctypes.c_void_p, ctypes.c_void_p]
+    lib.simt_vans_stage.restype = ctypes.c_int64
+    lib.simt_vans_stage.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_uint64, ctypes.c_void_p]
     lib.simt_offs_prep.restype = ctypes.c_int
     lib.simt_offs_prep.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int,
                                    ctypes.c_int]
@@ -285,6 +291,25 @@ def check_sheaf_pipeline(lib, classes, order):
         assert (u32(off_choice), u32(off_len)) == (kind, len(sec)), (b, order)
 
 
+@pytest.mark.parametrize("order", ORDERS)
+def test_vans_stage_under_schedules(simt, order):
+    check_vans_stage(simt, order)
+
+
+def check_vans_stage(lib, order):
+    O = oracle()
+    rng = np.random.default_rng(48)
+    for a, grid in ((unique_text(0, 30000, 49), 3), (modal_text(3, 5000, 50), 1), (np.full(100, 9, np.uint8), 2),
+                    (rng.integers(0, 256, 3, dtype=np.uint8), 1)):
+        cap = 8 + 600 + 16 + 2 * a.size + 64
+        out = np.zeros(cap, np.uint8)
+        counts = np.zeros(256, np.uint32)
+        n = lib.simt_vans_stage(ptr(a), a.size, grid, order, ptr(out), cap, ptr(counts))
+        assert n > 0, (a.size, order)
+        assert np.array_equal(counts, np.bincount(a, minlength=256)), (a.size, order)
+        assert out[:n].tobytes() == O.vans(a), (a.size, order)
+
+
 def _leb128(ds):
     out = bytearray()
     for d in ds:
@@ -389,7 +414,8 @@ cls = T.class_table()
 for order in (2, 3, 0, 1, 7):  # the greedy schedules first: they catch the most
     for name, fn in (("split", lambda: T.check_split(lib, cls, order)), ("hist", lambda: T.check_hist(lib, order)),
                      ("offs", lambda: T.check_offs(lib, order)), ("rans", lambda: T.check_rans(lib, order)),
-                     ("sheaf", lambda: T.check_sheaf_pipeline(lib, cls, order))):
+                     ("sheaf", lambda: T.check_sheaf_pipeline(lib, cls, order)),
+                     ("vans", lambda: T.check_vans_stage(lib, order))):
         try:
             fn()
         except AssertionError:
