@@ -2,8 +2,10 @@
 [1 KiB, 8 MiB], mixture content, seed 1, 9,473,959,928 bytes): the frames of one GPU batched
 encode (przm_gpu_encode_batch through GPUCompressor.EncodeBatch) against the oracle's frames of
 every buffer (orc_encode_buffer), compared by BLAKE2 digests computed in a pool of 16 oracle
-workers.  A one-off check run on a GPU machine (python3 tests/full_workload_parity.py; about two
-minutes), not collected by pytest; exits non-zero on any mismatch."""
+workers.  With --options both section options are on (the Section-3 prefix container and the
+Section-4 sheaf trials) on the GPU and in the oracle.  A one-off check run on a GPU machine
+(python3 tests/full_workload_parity.py [--options]; a few minutes), not collected by pytest;
+exits non-zero on any mismatch."""
 import hashlib, os, sys, time
 import numpy as np
 from multiprocessing import Pool
@@ -17,11 +19,12 @@ def init(arena, offs, sizes):
     from oracle_lib import oracle
     ARENA, OFFS, SIZES = arena, offs, sizes
     O = oracle()
-def work(rng):
+def work(job):
+    lo, hi, opt = job
     out = []
-    for i in range(*rng):
+    for i in range(lo, hi):
         a = ARENA[int(OFFS[i]): int(OFFS[i]) + int(SIZES[i])]
-        f, s = O.encode_buffer(a, 6, False, False)
+        f, s = O.encode_buffer(a, 6, opt, opt)
         out.append((i, hashlib.blake2b(f, digest_size=16).hexdigest(), int(len(f))))
     return out
 
@@ -31,6 +34,10 @@ if __name__ == "__main__":
     print("[parity] generated %.2f GiB in %.1fs" % (int(sizes.sum()) / 2**30, time.time() - t0), flush=True)
     from przm_amd import gpu as G
     g = G.GPUCompressor(0)
+    opt = "--options" in sys.argv
+    if opt:
+        g.SetOffsetsMode(G.OffsetsPrefix)
+        g.SetLiteralsMode(G.LiteralsSheaf)
     bufs = [arena[int(offs[i]): int(offs[i]) + int(sizes[i])] for i in range(10000)]
     t = time.time()
     res, _ = g.EncodeBatch(bufs, 6)
@@ -38,7 +45,7 @@ if __name__ == "__main__":
     glen = [len(f) for f, s in res]
     print("[parity] GPU batch in %.1fs, %d frames bytes" % (time.time() - t, sum(glen)), flush=True)
     t = time.time()
-    chunks = [(k, min(k + 50, 10000)) for k in range(0, 10000, 50)]
+    chunks = [(k, min(k + 50, 10000), opt) for k in range(0, 10000, 50)]
     bad, done = [], 0
     with Pool(16, initializer=init, initargs=(arena, offs, sizes)) as pool:
         for out in pool.imap_unordered(work, chunks):
@@ -48,5 +55,6 @@ if __name__ == "__main__":
             done += len(out)
             if done % 1000 == 0:
                 print("[parity] oracle %d buffers, %d mismatches, %.0fs" % (done, len(bad), time.time() - t), flush=True)
-    print("[parity] RESULT buffers 10000 bytes %d mismatches %d %s" % (int(sizes.sum()), len(bad), bad[:10]), flush=True)
+    print("[parity] RESULT options %s buffers 10000 bytes %d frame bytes %d mismatches %d %s" % (
+        opt, int(sizes.sum()), sum(glen), len(bad), bad[:10]), flush=True)
     sys.exit(1 if bad else 0)
