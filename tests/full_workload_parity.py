@@ -3,9 +3,11 @@
 encode (przm_gpu_encode_batch through GPUCompressor.EncodeBatch) against the oracle's frames of
 every buffer (orc_encode_buffer), compared by BLAKE2 digests computed in a pool of 16 oracle
 workers.  With --options both section options are on (the Section-3 prefix container and the
-Section-4 sheaf trials) on the GPU and in the oracle.  A one-off check run on a GPU machine
-(python3 tests/full_workload_parity.py [--options]; a few minutes), not collected by pytest;
-exits non-zero on any mismatch."""
+Section-4 sheaf trials) on the GPU and in the oracle.  With --config2 the buffer is instead the
+1 GiB mixture buffer (seed 2) through the plain entry point (CompressFrames), every one of its
+16,384 frames against the oracle's.  A one-off check run on a GPU machine
+(python3 tests/full_workload_parity.py [--options | --config2]; a few minutes), not collected by
+pytest; exits non-zero on any mismatch."""
 import hashlib, os, sys, time
 import numpy as np
 from multiprocessing import Pool
@@ -28,7 +30,33 @@ def work(job):
         out.append((i, hashlib.blake2b(f, digest_size=16).hexdigest(), int(len(f))))
     return out
 
+def config2():
+    """the 1 GiB mixture buffer through the plain entry point: every frame against the oracle's"""
+    from przm_amd import gpu as G
+    t = time.time()
+    a = D.mixture(1 << 30, 2)
+    g = G.GPUCompressor(0)
+    frames, sizes, _ = g.CompressFrames(a)
+    offs = np.concatenate([[0], np.cumsum(sizes.astype(np.int64))])
+    gh = [hashlib.blake2b(frames[offs[b]: offs[b + 1]], digest_size=16).hexdigest() for b in range(sizes.size)]
+    print("[parity] 1 GiB plain encode: %d frames, %d bytes, %.1fs" % (sizes.size, len(frames), time.time() - t),
+          flush=True)
+    blk_offs = np.arange(sizes.size, dtype=np.int64) * 65536
+    blk_sizes = np.minimum(65536, a.size - blk_offs)
+    t = time.time()
+    bad, seen = [], 0
+    with Pool(16, initializer=init, initargs=(a, blk_offs, blk_sizes)) as pool:
+        for out in pool.imap_unordered(work, [(k, min(k + 256, sizes.size), False) for k in range(0, sizes.size, 256)]):
+            bad += [i for i, h, l in out if h != gh[i] or l != int(sizes[i])]
+            seen += len(out)
+    print("[parity] RESULT config2 frames %d compared %d bytes %d frame bytes %d mismatches %d %s (%.0fs)" % (
+        sizes.size, seen, a.size, len(frames), len(bad), bad[:10], time.time() - t), flush=True)
+    sys.exit(1 if bad or seen != sizes.size else 0)
+
+
 if __name__ == "__main__":
+    if "--config2" in sys.argv:
+        config2()
     t0 = time.time()
     arena, offs, sizes = D.batch(10000, 1024, 8 << 20, 1)
     print("[parity] generated %.2f GiB in %.1fs" % (int(sizes.sum()) / 2**30, time.time() - t0), flush=True)
@@ -55,6 +83,6 @@ if __name__ == "__main__":
             done += len(out)
             if done % 1000 == 0:
                 print("[parity] oracle %d buffers, %d mismatches, %.0fs" % (done, len(bad), time.time() - t), flush=True)
-    print("[parity] RESULT options %s buffers 10000 bytes %d frame bytes %d mismatches %d %s" % (
-        opt, int(sizes.sum()), sum(glen), len(bad), bad[:10]), flush=True)
-    sys.exit(1 if bad else 0)
+    print("[parity] RESULT options %s buffers 10000 compared %d bytes %d frame bytes %d mismatches %d %s" % (
+        opt, done, int(sizes.sum()), sum(glen), len(bad), bad[:10]), flush=True)
+    sys.exit(1 if bad or done != 10000 else 0)
