@@ -88,14 +88,14 @@ def pipeline_input():
     return np.concatenate([a[: 1 << 16], a[5 << 16: 6 << 16], a[3 << 16: 4 << 16], a[7 << 16:]])
 
 
-def check_lz(lib, a, order, B=8, grid=1, depth=16):
+def check_lz(lib, a, order, B=8, grid=1, level=6):
     C = [int(lib.simt_lz_const(i)) for i in range(9)]
     O = oracle()
+    depth = int(O.l.orc_search_depth(level))
     nb = (a.size + 65535) // 65536
     ws = np.zeros(nb * C[0], np.uint8)
     meta = np.zeros(nb * C[7], np.uint8)
     assert lib.simt_lz(ptr(a), a.size, depth, B, grid, ptr(ws), ptr(meta), order) == 0, "deadlock"
-    level = {16: 6}.get(depth)
     for b in range(nb):
         want = O.tokenize(a[b * 65536: (b + 1) * 65536], level)
         lens = meta[b * C[7] + C[8]: b * C[7] + C[8] + 24].view(np.uint32)
@@ -126,6 +126,13 @@ def test_lz_b2_two_workgroups(lz):
     """the two-candidate search on a persistent grid of two workgroups (run one after the other:
     the first takes every block, the second finds the counter past the end)"""
     check_lz(lz, inputs()[: 3 << 16], 2, B=2, grid=2)
+
+
+@pytest.mark.parametrize("level,order", [(1, 3), (11, 1)])
+def test_lz_other_depths(lz, level, order):
+    """chain depth 2 (level 1: fewer candidates than a search step holds, the two-candidate
+    search of large buckets) and 128 (level 11: the deep search of long chains) on three blocks"""
+    check_lz(lz, inputs()[: 3 << 16], order, level=level)
 
 
 MUTANTS = [
