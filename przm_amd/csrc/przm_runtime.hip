@@ -1265,12 +1265,21 @@ struct Slot {
     std::vector<przm_span> spans;
     std::vector<przm_block_t> blocks;
     std::vector<uint64_t> first;
-    uint8_t *h_in = nullptr, *h_out = nullptr;
-    uint64_t h_in_cap = 0, h_out_cap = 0;
+    /* pinned host buffers: the staged input, the frames, the block table and the frame sizes.
+     * All four copies of a launch are asynchronous, so the worker never waits for a launch's
+     * kernels while it has other slots to retire or fill */
+    uint8_t *h_in = nullptr, *h_out = nullptr, *h_blk = nullptr, *h_flen = nullptr;
+    uint64_t h_in_cap = 0, h_out_cap = 0, h_blk_cap = 0, h_flen_cap = 0;
     uint64_t dev_bytes = 0;
     bool dead = false; /* its stream reported an error: never launched on again */
     hipEvent_t done = nullptr;
-    std::vector<uint32_t> flen;
+    void free_host() {
+        for (uint8_t **p : {&h_in, &h_out, &h_blk, &h_flen}) {
+            if (*p) (void)hipHostFree(*p);
+            *p = nullptr;
+        }
+        h_in_cap = h_out_cap = h_blk_cap = h_flen_cap = 0;
+    }
 };
 
 }  // namespace
@@ -1328,6 +1337,8 @@ int32_t slot_reserve(przm_server *s, Slot &sl, uint64_t arena, uint32_t nb) {
     const uint64_t bound = arena + 8ull * nb + 64;
     CHK(host_grow(&sl.h_in, &sl.h_in_cap, arena + 64));
     CHK(host_grow(&sl.h_out, &sl.h_out_cap, bound));
+    CHK(host_grow(&sl.h_blk, &sl.h_blk_cap, (uint64_t)std::max<uint32_t>(nb, 1) * sizeof(przm_block_t)));
+    CHK(host_grow(&sl.h_flen, &sl.h_flen_cap, (uint64_t)std::max<uint32_t>(nb, 1) * 4));
     CHK(sl.c.in.ensure(arena + 64, &sl.c.gen));
     CHK(sl.c.out.ensure(bound, &sl.c.gen));
     CHK(ensure_ctx(sl.c, std::max<uint32_t>(nb, 1), std::max<uint32_t>(nb, 1), g->lz_grid, false, g->s3_prefix,
@@ -1385,11 +1396,11 @@ int32_t launch_slot(przm_server *s, Slot &sl) {
     Ctx &c = sl.c;
     const uint32_t chunk = std::max<uint32_t>(nb, 1);
     HIPCHK(hipMemcpyAsync(c.in.p, sl.h_in, arena, hipMemcpyHostToDevice, c.st));
-    HIPCHK(hipMemcpyAsync(c.blocks.p, sl.blocks.data(), nb * sizeof(przm_block_t), hipMemcpyHostToDevice, c.st));
+    std::memcpy(sl.h_blk, sl.blocks.data(), nb * sizeof(przm_block_t));
+    HIPCHK(hipMemcpyAsync(c.blocks.p, sl.h_blk, nb * sizeof(przm_block_t), hipMemcpyHostToDevice, c.st));
     CHK(enqueue(g, c, (const uint8_t *)c.in.p, nb, (uint32_t)depth_of_level(s->level), (uint8_t *)c.out.p, true, false,
                 false, g->lz_grid, chunk, nullptr));
-    sl.flen.resize(nb);
-    HIPCHK(hipMemcpyAsync(sl.flen.data(), c.flen.p, nb * 4ull, hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipMemcpyAsync(sl.h_flen, c.flen.p, nb * 4ull, hipMemcpyDeviceToHost, c.st));
     /* frames: the device bound covers them; copy the bound and split on completion */
     HIPCHK(hipMemcpyAsync(sl.h_out, c.out.p, bound, hipMemcpyDeviceToHost, c.st));
     if (!sl.done) HIPCHK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
@@ -1399,6 +1410,7 @@ int32_t launch_slot(przm_server *s, Slot &sl) {
 
 /* split a finished launch's frames into its requests (worker thread, lock not held) */
 void split_slot(Slot &sl) {
+    const uint32_t *flen = (const uint32_t *)sl.h_flen;
     uint64_t pos = 0;
     std::vector<std::pair<uint8_t *, const uint8_t *>> ds;
     std::vector<uint64_t> ln;
@@ -1406,7 +1418,7 @@ void split_slot(Slot &sl) {
         Req &r = *sl.reqs[i];
         uint64_t len = 0;
         r.fsz.clear();
-        for (uint64_t b = sl.first[i]; b < sl.first[i + 1]; b++) { r.fsz.push_back(sl.flen[b]); len += sl.flen[b]; }
+        for (uint64_t b = sl.first[i]; b < sl.first[i + 1]; b++) { r.fsz.push_back(flen[b]); len += flen[b]; }
         r.out.resize(len);
         ds.push_back({r.out.data(), sl.h_out + pos});
         ln.push_back(len);
@@ -1556,8 +1568,7 @@ int32_t przm_serve_create(przm_gpu *g, int32_t level, uint64_t hbm_budget, uint6
     const uint64_t arena = s->max_batch + 256ull * s->max_blocks;
     auto cleanup = [&]() {
         for (auto &x : s->slots) {
-            if (x->h_in) (void)hipHostFree(x->h_in);
-            if (x->h_out) (void)hipHostFree(x->h_out);
+            x->free_host();
             x->c.release();
         }
         delete s;
@@ -1572,8 +1583,7 @@ int32_t przm_serve_create(przm_gpu *g, int32_t level, uint64_t hbm_budget, uint6
         const uint64_t bytes = ctx_bytes(sl->c);
         if (rc == 0 && s->reserved + bytes > s->budget) {
             /* this slot would exceed the budget: keep the ones that fit */
-            if (sl->h_in) (void)hipHostFree(sl->h_in);
-            if (sl->h_out) (void)hipHostFree(sl->h_out);
+            sl->free_host();
             sl->c.release();
             if (i == 0) {
                 snprintf(g_err, sizeof g_err, "HBM budget %llu below one launch slot (%llu bytes for max_batch_bytes %llu)",
@@ -1668,8 +1678,7 @@ int32_t przm_serve_destroy(przm_server *s) {
     for (auto &sl : s->slots) {
         if (sl->c.st) (void)hipStreamSynchronize(sl->c.st);
         if (sl->done) (void)hipEventDestroy(sl->done);
-        if (sl->h_in) (void)hipHostFree(sl->h_in);
-        if (sl->h_out) (void)hipHostFree(sl->h_out);
+        sl->free_host();
         sl->c.release();
     }
     delete s;
