@@ -1351,9 +1351,10 @@ int32_t slot_reserve(przm_server *s, Slot &sl, uint64_t arena, uint32_t nb) {
 void par_copy(const std::vector<std::pair<uint8_t *, const uint8_t *>> &dst_src, const std::vector<uint64_t> &len) {
     uint64_t total = 0;
     for (uint64_t l : len) total += l;
-    const unsigned nt = total < (8ull << 20) ? 1u : 4u;
+    /* one thread per MiB, at most 8: a launch's copies sit on its requests' latency path */
+    const unsigned nt = (unsigned)std::min<uint64_t>(8, std::max<uint64_t>(1, total >> 20));
     auto work = [&](unsigned t) {
-        /* thread t copies the t-th quarter of the concatenated byte range */
+        /* thread t copies the t-th of nt equal parts of the concatenated byte range */
         const uint64_t lo = total * t / nt, hi = total * (t + 1) / nt;
         uint64_t pos = 0;
         for (size_t i = 0; i < len.size(); i++) {
